@@ -1,0 +1,322 @@
+// SPDX-License-Identifier: CeCILL-2.1
+//
+// sph_oracle_step.hpp -- CPU restatement of one full SPH timestep of the reference
+// (shammodels/sph/src/Solver.cpp:1942-3272, Solver<f64_3, Kern>::evolve_once) for one patch.
+// Test oracle / CPU baseline only; the HIP product never uses it.
+#pragma once
+
+#include "sph_oracle.hpp"
+#include <chrono>
+
+namespace orc {
+
+    // C layout shared with the Python test harness (tests/oracle_lib.py)
+    struct OrcConfig {
+        int32_t kernel;               // 0 M4, 1 M6, 2 M8, 3 C2, 4 C4, 5 C6
+        int32_t periodic;             // 1 periodic box, 0 free boundaries
+        int32_t eos_kind;             // EosKind
+        int32_t tree_reduction_level; // SolverConfig.hpp:584 (default 3)
+        int32_t h_iter_per_subcycles; // SolverConfig.hpp:608 (default 50)
+        int32_t h_max_subcycles;      // SolverConfig.hpp:609 (default 100)
+        double box_min[3], box_max[3];
+        double gpart_mass;
+        double gamma, eos_cs, eos_cs0, eos_q, eos_r0;
+        double alpha_u, alpha_AV, beta_AV;
+        double htol_up_coarse, htol_up_fine, epsilon_h; // SolverConfig.hpp:605-607
+        double cfl_cour, cfl_force, cfl_multiplier_stiffness;
+    };
+
+    struct StepStats {
+        double t_total, t_ghost, t_tree, t_neigh, t_hiter, t_omega, t_eos, t_force, t_corr, t_cfl;
+        uint32_t n_merged, n_leaf, subcycles, h_iterations, corrector_iterations;
+        uint64_t n_pairs;
+        double energy, eps_v, next_dt;
+    };
+
+    struct Timer {
+        std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+        double sec() const {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    };
+
+    template<class K>
+    struct Solver {
+        OrcConfig cfg;
+        u32 N = 0;
+        std::vector<f64> x, y, z, vx, vy, vz, ax, ay, az, h, u, du;
+        f64 t = 0, dt = 0, cfl_mult = 1e-2; // Solver.hpp:124-147 time-state edges
+        StepStats stats{};
+
+        // transient state of the last step, kept for inspection by the parity tests
+        u32 Nm = 0;
+        std::vector<u32> ghost_src;
+        std::vector<vec3> ghost_off;
+        std::vector<f64> mx, my, mz, mh, mvx, mvy, mvz, mu, momega, P, cs, vsig, omega;
+        vec3 tree_bmin{}, tree_bmax{};
+        CLBVH<u32> tree;
+        std::vector<f64> rint_tree;
+        NeighOut neigh;
+
+        EosParams eos() const {
+            EosParams e;
+            e.kind  = cfg.eos_kind;
+            e.gamma = cfg.gamma;
+            e.cs    = cfg.eos_cs;
+            e.cs0   = cfg.eos_cs0;
+            e.q     = cfg.eos_q;
+            e.r0    = cfg.eos_r0;
+            return e;
+        }
+
+        void resize(u32 n) {
+            N = n;
+            for (auto *v : {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az, &h, &u, &du}) v->assign(n, 0.);
+        }
+
+        // ghosts + merge of positions and h: Solver.cpp:984-1046, BasicSPHGhosts.cpp:260-579
+        void build_ghosts_and_merge() {
+            ghost_src.clear();
+            ghost_off.clear();
+            if (cfg.periodic) {
+                f64 hmax = std::numeric_limits<f64>::lowest();
+                for (u32 i = 0; i < N; i++) hmax = std::fmax(hmax, h[i]);
+                f64 int_range = hmax * cfg.htol_up_coarse * K::Rkern;
+                vec3 bmin{cfg.box_min[0], cfg.box_min[1], cfg.box_min[2]};
+                vec3 bmax{cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
+                ghost_src = periodic_ghost_ids(x.data(), y.data(), z.data(), N, bmin, bmax,
+                                               int_range, ghost_off);
+            }
+            u32 G = (u32) ghost_src.size();
+            Nm    = N + G;
+            mx.resize(Nm);
+            my.resize(Nm);
+            mz.resize(Nm);
+            mh.resize(Nm);
+            for (u32 i = 0; i < N; i++) {
+                mx[i] = x[i];
+                my[i] = y[i];
+                mz[i] = z[i];
+                mh[i] = h[i];
+            }
+            for (u32 g = 0; g < G; g++) {
+                u32 s     = ghost_src[g];
+                mx[N + g] = x[s] + ghost_off[g].x;
+                my[N + g] = y[s] + ghost_off[g].y;
+                mz[N + g] = z[s] + ghost_off[g].z;
+                mh[N + g] = h[s];
+            }
+        }
+
+        // BuildTrees.cpp:27-65 + Solver.cpp:1322-1356 (rint field)
+        void build_tree() {
+            bbox_nextafter(mx.data(), my.data(), mz.data(), Nm, tree_bmin, tree_bmax);
+            tree.build_no_aabb(mx.data(), my.data(), mz.data(), Nm, tree_bmin, tree_bmax,
+                               (u32) cfg.tree_reduction_level);
+            tree.compute_aabb(mx.data(), my.data(), mz.data());
+            rint_tree = tree.field_max(mh.data());
+            for (auto &v : rint_tree) v *= cfg.htol_up_coarse;
+        }
+
+        // ghost field exchange: Solver.cpp:1394-1633 (h, u, v, omega of ghost sources)
+        void merge_fields() {
+            u32 G = (u32) ghost_src.size();
+            for (auto *v : {&mh, &mvx, &mvy, &mvz, &mu, &momega}) v->resize(Nm);
+            for (u32 i = 0; i < N; i++) {
+                mh[i]     = h[i];
+                mvx[i]    = vx[i];
+                mvy[i]    = vy[i];
+                mvz[i]    = vz[i];
+                mu[i]     = u[i];
+                momega[i] = omega[i];
+            }
+            for (u32 g = 0; g < G; g++) {
+                u32 s         = ghost_src[g];
+                mh[N + g]     = h[s];
+                mvx[N + g]    = vx[s];
+                mvy[N + g]    = vy[s];
+                mvz[N + g]    = vz[s];
+                mu[N + g]     = u[s];
+                momega[N + g] = omega[s];
+            }
+        }
+
+        // Solver.cpp:1060-1304
+        void prestep() {
+            u32 hstep = 0, hmax = (u32) cfg.h_max_subcycles;
+            std::vector<f64> eps(N), h_old(N);
+            stats.h_iterations = 0;
+            for (; hstep < hmax; hstep++) {
+                Timer tg;
+                build_ghosts_and_merge();
+                stats.t_ghost += tg.sec();
+                Timer tt;
+                build_tree();
+                stats.t_tree += tt.sec();
+                Timer tn;
+                neigh = neigh_cache_2stages(tree, rint_tree, mx.data(), my.data(), mz.data(),
+                                            mh.data(), N, cfg.htol_up_coarse, K::Rkern);
+                stats.t_neigh += tn.sec();
+                if (cfg.gpart_mass == 0)
+                    throw std::runtime_error("invalid gpart_mass 0, this configuration can not converge");
+                Timer th;
+                std::fill(eps.begin(), eps.end(), 100.);
+                h_old = h;
+                f64 max_eps = std::numeric_limits<f64>::max();
+                for (int it = 0; it < cfg.h_iter_per_subcycles; it++) {
+                    iterate_h<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(),
+                                 h_old.data(), h.data(), eps.data(), cfg.gpart_mass,
+                                 cfg.htol_up_coarse, cfg.htol_up_fine);
+                    stats.h_iterations++;
+                    max_eps = std::numeric_limits<f64>::lowest();
+                    for (u32 i = 0; i < N; i++) max_eps = std::fmax(max_eps, eps[i]);
+                    if (max_eps < cfg.epsilon_h) break;
+                }
+                f64 min_eps = std::numeric_limits<f64>::max();
+                for (u32 i = 0; i < N; i++) min_eps = std::fmin(min_eps, eps[i]);
+                stats.t_hiter += th.sec();
+                bool converged = (max_eps < cfg.epsilon_h) && !(min_eps < 0);
+                if (!converged) continue;
+                break;
+            }
+            stats.subcycles = hstep + 1;
+            if (hstep == hmax) throw std::runtime_error("the h iterator is not converged");
+            Timer to;
+            omega.assign(N, 0.);
+            compute_omega<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(), h.data(),
+                             omega.data(), cfg.gpart_mass);
+            stats.t_omega += to.sec();
+        }
+
+        void evolve_once() {
+            stats = StepStats{};
+            Timer tstep;
+            f64 t_cur = t, dtc = dt;
+
+            // leapfrog predictor: Solver.cpp:390-524, 582-608; ForwardEuler.hpp:62-69
+            f64 dt_half = dtc / 2;
+            for (u32 i = 0; i < N; i++) {
+                vx[i] = vx[i] + dt_half * ax[i];
+                vy[i] = vy[i] + dt_half * ay[i];
+                vz[i] = vz[i] + dt_half * az[i];
+                u[i]  = u[i] + dt_half * du[i];
+                x[i]  = x[i] + dtc * vx[i];
+                y[i]  = y[i] + dtc * vy[i];
+                z[i]  = z[i] + dtc * vz[i];
+                vx[i] = vx[i] + dt_half * ax[i];
+                vy[i] = vy[i] + dt_half * ay[i];
+                vz[i] = vz[i] + dt_half * az[i];
+                u[i]  = u[i] + dt_half * du[i];
+            }
+            // apply_position_boundary: Solver.cpp:938-981
+            if (cfg.periodic) {
+                f64 dx = cfg.box_max[0] - cfg.box_min[0], dy = cfg.box_max[1] - cfg.box_min[1],
+                    dz = cfg.box_max[2] - cfg.box_min[2];
+                for (u32 i = 0; i < N; i++) {
+                    x[i] = position_modulo(x[i], cfg.box_min[0], dx);
+                    y[i] = position_modulo(y[i], cfg.box_min[1], dy);
+                    z[i] = position_modulo(z[i], cfg.box_min[2], dz);
+                }
+            }
+
+            prestep();
+
+            // corrector loop: Solver.cpp:2247-3171
+            std::vector<f64> ax_old(N), ay_old(N), az_old(N), du_old(N), eps_v2(N);
+            f64 next_cfl   = 0;
+            u32 corr_iter  = 0;
+            bool rerun     = false;
+            do {
+                if (corr_iter == 50)
+                    throw std::runtime_error("the corrector has made over 50 loops");
+                merge_fields();
+                Timer te;
+                P.assign(Nm, 0.);
+                cs.assign(Nm, 0.);
+                compute_eos<K>(Nm, mh.data(), mu.data(), mx.data(), my.data(), mz.data(),
+                               P.data(), cs.data(), cfg.gpart_mass, eos());
+                stats.t_eos += te.sec();
+                ax_old = ax;
+                ay_old = ay;
+                az_old = az;
+                du_old = du;
+                Timer tf;
+                AVParams av{cfg.alpha_u, cfg.alpha_AV, cfg.beta_AV};
+                update_derivs_constant_av<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(),
+                                             mvx.data(), mvy.data(), mvz.data(), mh.data(),
+                                             momega.data(), mu.data(), P.data(), cs.data(),
+                                             ax.data(), ay.data(), az.data(), du.data(),
+                                             cfg.gpart_mass, av);
+                stats.t_force += tf.sec();
+                Timer tc;
+                // leapfrog_corrector: integrators.cpp:87-119 with hdt = dt/2
+                f64 hdt = dtc / 2;
+                f64 max_e2 = std::numeric_limits<f64>::lowest();
+                for (u32 i = 0; i < N; i++) {
+                    vec3 incr{hdt * (ax[i] - ax_old[i]), hdt * (ay[i] - ay_old[i]),
+                              hdt * (az[i] - az_old[i])};
+                    vx[i] = vx[i] + incr.x;
+                    vy[i] = vy[i] + incr.y;
+                    vz[i] = vz[i] + incr.z;
+                    max_e2 = std::fmax(max_e2, dot(incr, incr));
+                    f64 ui = hdt * (du[i] - du_old[i]);
+                    u[i]   = u[i] + ui;
+                }
+                f64 rank_veps_v = std::sqrt(max_e2);
+                f64 sum_vsq     = 0;
+                for (u32 i = 0; i < N; i++)
+                    sum_vsq += dot({vx[i], vy[i], vz[i]}, {vx[i], vy[i], vz[i]});
+                f64 vmean = std::sqrt(sum_vsq / f64(N));
+                f64 eps_v = rank_veps_v / vmean;
+                if (vmean <= 0) eps_v = 0;
+                stats.eps_v = eps_v;
+                stats.t_corr += tc.sec();
+                if (eps_v > 1e-2) {
+                    rerun = true;
+                    cfl_mult /= 2;
+                } else {
+                    rerun = false;
+                    Timer tl;
+                    vsig.assign(N, 0.);
+                    compute_vsig<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(),
+                                    mvx.data(), mvy.data(), mvz.data(), mh.data(), cs.data(),
+                                    vsig.data());
+                    // ComputeCFLCourant.hpp:40-60, ComputeCFLForce.hpp:50-62, Solver.cpp:3095-3124
+                    f64 C_cour  = cfg.cfl_cour * cfl_mult;
+                    f64 C_force = cfg.cfl_force * cfl_mult;
+                    f64 rank_dt = std::numeric_limits<f64>::infinity();
+                    for (u32 i = 0; i < N; i++) {
+                        f64 h_a    = mh[i];
+                        f64 dt_c   = C_cour * h_a / vsig[i];
+                        f64 abs_a  = std::sqrt(dot({ax[i], ay[i], az[i]}, {ax[i], ay[i], az[i]}));
+                        f64 dt_f   = C_force * std::sqrt(h_a / abs_a);
+                        f64 cfl_dt = std::fmin(std::numeric_limits<f64>::infinity(), dt_c);
+                        cfl_dt     = std::fmin(cfl_dt, dt_f);
+                        rank_dt    = std::fmin(rank_dt, cfl_dt);
+                    }
+                    next_cfl = rank_dt;
+                    stats.t_cfl += tl.sec();
+                }
+                corr_iter++;
+            } while (rerun);
+            stats.corrector_iterations = corr_iter;
+
+            stats.t_total  = tstep.sec();
+            stats.n_merged = Nm;
+            stats.n_leaf   = tree.L;
+            stats.n_pairs  = neigh.part_cache.ids.size();
+            f64 E          = 0;
+            for (u32 i = 0; i < N; i++)
+                E += cfg.gpart_mass * (u[i] + 0.5 * dot({vx[i], vy[i], vz[i]}, {vx[i], vy[i], vz[i]}));
+            stats.energy  = E;
+            stats.next_dt = next_cfl;
+
+            // Solver.cpp:3235-3263
+            dt = next_cfl;
+            t  = t_cur + dtc;
+            f64 stiff = cfg.cfl_multiplier_stiffness;
+            cfl_mult  = (cfl_mult * stiff + 1.) / (stiff + 1.);
+        }
+    };
+
+} // namespace orc
