@@ -1,0 +1,199 @@
+"""Benchmark: particle-updates/s of one full SPH step (FP64 and mixed) on one MI355X.
+
+Workload: Sedov-Taylor blast, 128^3 = 2,097,152 particles on a lattice in the periodic box
+[-0.5, 0.5]^3, M6 kernel, gamma = 5/3, constant AV (1, 1, 2); synthetic initial data generated
+in-process (shamrock_amd/setups.py).  A step is one complete Solver::evolve_once (predictor,
+periodic images, tree, neighbour cache, smoothing-length Newton loop, Omega, EOS, force,
+corrector, CFL) with the particle fields resident in HBM.
+
+Prints ONE JSON line (rank 0).  `value` = FP64 particle-updates/s = N * K / (time of K steps).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s per full SPH step, FP64 vs mixed (err vs FP64) at 2.1M parts"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+SORT_BYTES_PER_KEY = 68   # 30-bit keys, 8-bit digits: key read for the histogram + 4 x 16 B
+FORCE_FLOPS_PER_PAIR = 120
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(cfg, fields, steps, warmup):
+    import torch
+
+    from shamrock_amd.solver import GpuSolver
+
+    g = GpuSolver(cfg, fields)
+    logs = []
+    for i in range(warmup):
+        lg = g.evolve_once()
+        log(f"  warmup {i}: {lg['t_total'] * 1e3:.2f} ms (h-iter {lg['h_iterations']}, "
+            f"subcycles {lg['subcycles']})")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        logs.append(g.evolve_once())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    return g, logs, elapsed
+
+
+def relL2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def mixed_error(cfg, g64):
+    """One step from the same state in FP64 and mixed: rel-L2 of a and du/dt."""
+    from shamrock_amd.solver import GpuSolver
+
+    st = {k: g64.get(k) for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u",
+                                  "du")}
+    tm = g64.time()
+    out = {}
+    for mixed in (0, 1):
+        s = GpuSolver(dict(cfg, mixed_precision=mixed), st)
+        s.set_time(tm["t"], tm["dt"], tm["cfl_mult"])
+        s.evolve_once()
+        out[mixed] = (np.stack([s.get(k) for k in ("ax", "ay", "az")]), s.get("du"))
+        s.close()
+    return relL2(out[1][0], out[0][0]), relL2(out[1][1], out[0][1])
+
+
+def cpu_info():
+    info = {"model": platform.processor() or "unknown"}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return info
+
+
+def cpu_baseline(cfg, fields, threads):
+    """Oracle (C++ restatement of the reference step, OpenMP) on the same workload."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+
+    s = orc.Solver(cfg, fields)
+    t0 = time.perf_counter()
+    first = s.evolve_once()  # dt = 0 step (not timed)
+    t1 = time.perf_counter()
+    st = s.evolve_once()
+    t2 = time.perf_counter()
+    n = s.n
+    return {"value": n / (t2 - t1), "unit": "particle-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle evolve_once on the same Sedov 128^3 M6 FP64 workload: the dt=0 "
+                      f"step ({t1 - t0:.1f} s, untimed) then one timed step ({t2 - t1:.1f} s), "
+                      f"OpenMP {threads} threads, {cpu_info()['model']}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-side", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU design "
+                                   "(patch decomposition with ghost exchange) is described in "
+                                   "DESIGN.md and not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from shamrock_amd import _lib, setups
+
+    _lib.lib()
+    cfg, fields = setups.sedov(args.n_side)
+    n = fields["x"].size
+    log(f"[bench] Sedov {args.n_side}^3 N={n} FP64: warmup {args.warmup}, steps {args.steps}")
+    g64, logs64, el64 = run_gpu(cfg, fields, args.steps, args.warmup)
+    log(f"[bench] FP64 {el64 / args.steps * 1e3:.2f} ms/step")
+    cfg_m = dict(cfg, mixed_precision=1)
+    log("[bench] mixed")
+    gm, logsm, elm = run_gpu(cfg_m, fields, args.steps, args.warmup)
+    log(f"[bench] mixed {elm / args.steps * 1e3:.2f} ms/step")
+    err_a, err_du = mixed_error(cfg, g64)
+    e64 = [lg["energy"] for lg in logs64]
+    em = [lg["energy"] for lg in logsm]
+
+    value = n * args.steps / el64
+    value_m = n * args.steps / elm
+    # roofline of the dominant kernels, from the library's HIP-event timings on its stream
+    t_sort = sum(lg["t_sort"] for lg in logs64)
+    n_sorts = sum(lg["n_sorts"] for lg in logs64)
+    nm = logs64[-1]["n_merged"]
+    sort_gbs = SORT_BYTES_PER_KEY * nm * n_sorts / t_sort / 1e9 if t_sort > 0 else 0.0
+    t_force = sum(lg["t_force"] for lg in logs64)
+    n_fe = sum(lg["n_force_evals"] for lg in logs64)
+    pairs = logs64[-1]["n_interacting"]
+    force_tfs = FORCE_FLOPS_PER_PAIR * pairs * n_fe / t_force / 1e12 if t_force > 0 else 0.0
+    phases = {k: sum(lg[k] for lg in logs64) / args.steps * 1e3
+              for k in ("t_ghost", "t_tree", "t_sort", "t_neigh", "t_hiter", "t_omega", "t_eos",
+                        "t_force", "t_corr")}
+    kernels = {
+        "sort": {"bound": "hbm", "achieved": sort_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": sort_gbs / HBM_PEAK_GBS, "traffic": None,
+                 "avg_ms": t_sort / max(n_sorts, 1) * 1e3,
+                 "algorithmic": f"{SORT_BYTES_PER_KEY} B x {nm} keys"},
+        "force": {"bound": "mfma", "achieved": force_tfs, "peak": FP64_PEAK_TFS,
+                  "unit": "TFLOP/s", "frac": force_tfs / FP64_PEAK_TFS, "traffic": None,
+                  "avg_ms": t_force / max(n_fe, 1) * 1e3,
+                  "algorithmic": f"{FORCE_FLOPS_PER_PAIR} FP64 flop x {pairs} interacting pairs",
+                  "note": "FP64 VALU loop; the MI355X FP64 vector peak equals its FP64 matrix "
+                          "peak (78.6 TF/s)"},
+    }
+    dominant = "force" if t_force >= t_sort else "sort"
+    roof = {k: kernels[dominant][k] for k in ("bound", "achieved", "peak", "unit", "frac",
+                                               "traffic")}
+    roof["kernel"] = dominant
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": el64 / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"sedov_{args.n_side}^3_M6_periodic", "n_particles": n,
+                   "n_merged_with_ghosts": nm, "kernel": "M6", "gamma": 5 / 3,
+                   "av": "constant(1,1,2)", "parallelism": "single_gpu"},
+        "mixed": {"value": value_m, "ms_per_step": elm / args.steps * 1e3,
+                  "speedup_vs_fp64": value_m / value, "rel_l2_accel_vs_fp64": err_a,
+                  "rel_l2_du_vs_fp64": err_du,
+                  "energy_drift_rel": abs(em[-1] - em[0]) / abs(em[0])},
+        "fp64_energy_drift_rel": abs(e64[-1] - e64[0]) / abs(e64[0]),
+        "phase_ms_fp64": phases,
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    g64.close()
+    gm.close()
+    if not args.no_cpu_baseline:
+        log(f"[bench] CPU baseline (oracle, {args.cpu_threads} threads)")
+        out["cpu_baseline"] = cpu_baseline(cfg, fields, args.cpu_threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

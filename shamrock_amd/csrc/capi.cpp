@@ -1,0 +1,774 @@
+// SPDX-License-Identifier: CeCILL-2.1
+//
+// capi.cpp -- C++20 host side of the library: the Solver that runs one SPH timestep on the
+// GPU (restating shammodels::sph::Solver<f64_3,Kern>::evolve_once, Solver.cpp:1942-3272, for
+// one patch) and the extern "C" entry points declared in include/shamx_sph.h.
+#include "../../include/shamx_sph.h"
+#include "launch.hpp"
+#include "tree_layout.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+
+using namespace shamx;
+
+namespace {
+
+    thread_local std::string g_last_error;
+
+    template<class F>
+    int guarded(F &&f) {
+        try {
+            f();
+            return SHAMX_OK;
+        } catch (const std::invalid_argument &e) {
+            g_last_error = e.what();
+            return SHAMX_E_INVALID_ARG;
+        } catch (const HipError &e) {
+            g_last_error = e.what();
+            return SHAMX_E_HIP;
+        } catch (const std::bad_alloc &e) {
+            g_last_error = e.what();
+            return SHAMX_E_OOM;
+        } catch (const std::exception &e) {
+            g_last_error = e.what();
+            return SHAMX_E_INTERNAL;
+        }
+    }
+
+    void need(bool c, const char *msg) {
+        if (!c) throw std::invalid_argument(msg);
+    }
+
+    f64 kernel_rkern(int k) {
+        f64 r = 0;
+        dispatch_kernel(k, [&](auto K) { r = decltype(K)::Rkern; });
+        return r;
+    }
+
+    // small device scratch for the phase entries that do not take a context
+    struct Slots {
+        DevArray<u64> v;
+        Slots() { v.resize(8); }
+    };
+    thread_local std::unique_ptr<Slots> g_slots;
+    u64 *slots() {
+        if (!g_slots) g_slots = std::make_unique<Slots>();
+        return g_slots->v.data();
+    }
+
+    // phase timing with HIP events, read back once the step has synchronised
+    struct PhaseTimer {
+        std::vector<hipEvent_t> pool;
+        std::vector<std::pair<int, std::pair<int, int>>> marks; // phase, (ev start, ev end)
+        size_t used = 0;
+        hipEvent_t next() {
+            if (used == pool.size()) {
+                hipEvent_t e;
+                SHAMX_HIP(hipEventCreate(&e));
+                pool.push_back(e);
+            }
+            return pool[used++];
+        }
+        int begin(hipStream_t s) {
+            int i = (int) used;
+            SHAMX_HIP(hipEventRecord(next(), s));
+            return i;
+        }
+        void end(hipStream_t s, int phase, int b) {
+            int i = (int) used;
+            SHAMX_HIP(hipEventRecord(next(), s));
+            marks.push_back({phase, {b, i}});
+        }
+        void reset() {
+            marks.clear();
+            used = 0;
+        }
+        void collect(double *acc) {
+            for (auto &m : marks) {
+                float ms = 0;
+                SHAMX_HIP(hipEventElapsedTime(&ms, pool[m.second.first], pool[m.second.second]));
+                acc[m.first] += ms * 1e-3;
+            }
+        }
+        ~PhaseTimer() {
+            for (auto e : pool) (void) hipEventDestroy(e);
+        }
+    };
+    enum Phase { PH_GHOST, PH_TREE, PH_SORT, PH_NEIGH, PH_HITER, PH_OMEGA, PH_EOS, PH_FORCE, PH_CORR, PH_N };
+
+} // namespace
+
+struct shamx_ctx {
+    TreeBuffers tree;
+    NeighBuffers neigh;
+    DevArray<u32> alt_k, alt_v;
+    DevArray<unsigned char> scratch;
+};
+
+struct shamx_solver {
+    shamx_config cfg{};
+    u32 N = 0, G = 0, cap = 0;
+    hipStream_t stream = nullptr;
+    f64 Rkern = 0;
+    // merged arrays (local particles then ghosts), capacity cap
+    DevArray<f64> x, y, z, h, vx, vy, vz, u, omega, P, cs;
+    // local arrays
+    DevArray<f64> ax, ay, az, du, ax_old, ay_old, az_old, du_old, eps, h_old, vsig, dtc;
+    DevArray<u32> ghost_src, gwork;
+    DevArray<u8> ghost_oid;
+    DevArray<unsigned char> gscratch;
+    DevArray<u64> red;
+    TreeBuffers tree;
+    NeighBuffers nb;
+    f64 t = 0, dt = 0, cfl_mult = 1e-2; // Solver.hpp:124-147
+    shamx_step_log last{};
+    PhaseTimer timer;
+
+    DevArray<f64> *field(int id) {
+        DevArray<f64> *f[12] = {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az, &h, &u, &du};
+        need(id >= 0 && id < 12, "field id must be in [0, 11]");
+        return f[id];
+    }
+
+    void alloc(u32 n) {
+        N   = n;
+        cap = n + n / 3 + 1024;
+        for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega, &P, &cs}) a->resize(cap);
+        for (auto *a : {&ax, &ay, &az, &du, &ax_old, &ay_old, &az_old, &du_old, &eps, &h_old,
+                        &vsig, &dtc})
+            a->resize(std::max<u32>(n, 1));
+        red.resize(8);
+        for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega, &ax, &ay, &az, &du})
+            fill_f64(stream, a->data(), 0., a->capacity());
+        SHAMX_HIP(hipStreamSynchronize(stream));
+    }
+
+    void grow(u32 need_cap) {
+        u32 c = need_cap + need_cap / 8;
+        for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega}) a->resize_keep(c, stream);
+        P.resize(c);
+        cs.resize(c);
+        cap = c;
+    }
+
+    f64 read_key(int slot) {
+        u64 k;
+        SHAMX_HIP(hipMemcpyAsync(&k, red.data() + slot, 8, hipMemcpyDeviceToHost, stream));
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        return ordered_to_f64(k);
+    }
+
+    Csr csr() const { return Csr{nb.cnt.data(), nb.offset.data(), nb.ids.data()}; }
+
+    // Solver.cpp:984-1046 + BasicSPHGhosts: images of the local particles after them
+    void ghosts() {
+        if (!cfg.periodic) {
+            G = 0;
+            return;
+        }
+        fill_u64(stream, red.data(), ORDERED_NEG_INF, 1);
+        max_key_f64(stream, N, h.data(), red.data());
+        f64 hmax      = read_key(0);
+        f64 int_range = hmax * cfg.htol_up_coarse * Rkern;
+        f64 box6[6]   = {cfg.box_min[0], cfg.box_min[1], cfg.box_min[2],
+                         cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
+        u32 g = periodic_ghosts(stream, N, x.data(), y.data(), z.data(), h.data(), box6,
+                                int_range, ghost_src, ghost_oid, gwork, gscratch, cap - N);
+        if (g > cap - N) {
+            grow(N + g);
+            g = periodic_ghosts(stream, N, x.data(), y.data(), z.data(), h.data(), box6,
+                                int_range, ghost_src, ghost_oid, gwork, gscratch, cap - N);
+        }
+        G = g;
+    }
+
+    void prestep() { // Solver.cpp:1060-1304
+        u32 hstep = 0, hmax = (u32) cfg.h_max_subcycles;
+        last.h_iterations = 0;
+        for (; hstep < hmax; hstep++) {
+            int b = timer.begin(stream);
+            ghosts();
+            timer.end(stream, PH_GHOST, b);
+            u32 Nm = N + G;
+            b      = timer.begin(stream);
+            tree_stage_codes(stream, tree, Nm, x.data(), y.data(), z.data());
+            timer.end(stream, PH_TREE, b);
+            b = timer.begin(stream);
+            tree_stage_sort(stream, tree);
+            timer.end(stream, PH_SORT, b);
+            b = timer.begin(stream);
+            tree_stage_finish(stream, tree, x.data(), y.data(), z.data(), h.data(),
+                              (u32) cfg.tree_reduction_level, cfg.htol_up_coarse);
+            timer.end(stream, PH_TREE, b);
+            last.n_sorts++;
+            b = timer.begin(stream);
+            neigh_build(stream, nb, tree, N, x.data(), y.data(), z.data(), h.data(),
+                        cfg.htol_up_coarse, Rkern);
+            timer.end(stream, PH_NEIGH, b);
+            need(cfg.gpart_mass != 0, "invalid gpart_mass 0, this configuration can not converge");
+            b = timer.begin(stream);
+            fill_f64(stream, eps.data(), 100., N);
+            SHAMX_HIP(hipMemcpyAsync(h_old.data(), h.data(), N * sizeof(f64),
+                                     hipMemcpyDeviceToDevice, stream));
+            f64 max_eps = std::numeric_limits<f64>::max(), min_eps = 0;
+            for (int it = 0; it < cfg.h_iter_per_subcycles; it++) {
+                fill_u64(stream, red.data() + 1, ORDERED_NEG_INF, 1);
+                fill_u64(stream, red.data() + 2, ORDERED_POS_INF, 1);
+                h_iterate(stream, cfg.kernel, N, csr(), x.data(), y.data(), z.data(),
+                          h_old.data(), h.data(), eps.data(), cfg.gpart_mass, cfg.htol_up_coarse,
+                          cfg.htol_up_fine, red.data() + 1, cfg.mixed_precision);
+                u64 k[2];
+                SHAMX_HIP(hipMemcpyAsync(k, red.data() + 1, 16, hipMemcpyDeviceToHost, stream));
+                SHAMX_HIP(hipStreamSynchronize(stream));
+                max_eps = ordered_to_f64(k[0]);
+                min_eps = ordered_to_f64(k[1]);
+                last.h_iterations++;
+                if (max_eps < cfg.epsilon_h) break;
+            }
+            timer.end(stream, PH_HITER, b);
+            bool converged = (max_eps < cfg.epsilon_h) && !(min_eps < 0);
+            if (!converged) continue;
+            break;
+        }
+        last.subcycles = hstep + 1;
+        if (hstep == hmax) throw std::runtime_error("the h iterator is not converged");
+        int b = timer.begin(stream);
+        compute_omega(stream, cfg.kernel, N, csr(), x.data(), y.data(), z.data(), h.data(),
+                      omega.data(), cfg.gpart_mass);
+        timer.end(stream, PH_OMEGA, b);
+    }
+
+    void evolve_once() {
+        need(N > 0, "evolve_once on an empty particle set");
+        timer.reset();
+        last = shamx_step_log{};
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        auto t0   = std::chrono::steady_clock::now();
+        f64 t_cur = t, dtc_ = dt;
+        f64 box6[6] = {cfg.box_min[0], cfg.box_min[1], cfg.box_min[2],
+                       cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
+        int b = timer.begin(stream);
+        predictor(stream, N, dtc_, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
+                  ax.data(), ay.data(), az.data(), u.data(), du.data(), cfg.periodic, nullptr_box());
+        timer.end(stream, PH_CORR, b);
+        (void) box6;
+        prestep();
+
+        f64 next_cfl = 0;
+        u32 corr_iter = 0;
+        bool rerun    = false;
+        do {
+            if (corr_iter == 50)
+                throw std::runtime_error("the corrector has made over 50 loops, either their is "
+                                         "a bug, either you are using a dt that is too large");
+            b = timer.begin(stream);
+            FieldPtrs fp{{h.data(), u.data(), vx.data(), vy.data(), vz.data(), omega.data()}, 6};
+            gather_ghosts(stream, N, G, ghost_src.data(), fp);
+            compute_eos(stream, cfg.kernel, N + G, h.data(), u.data(), x.data(), y.data(),
+                        z.data(), P.data(), cs.data(), cfg.gpart_mass,
+                        EosArgs{cfg.eos_kind, cfg.gamma, cfg.eos_cs, cfg.eos_cs0, cfg.eos_q,
+                                cfg.eos_r0});
+            timer.end(stream, PH_EOS, b);
+            b = timer.begin(stream);
+            fill_u64(stream, red.data() + 3, ORDERED_POS_INF, 1);
+            fill_u64(stream, red.data() + 7, 0, 1);
+            ForceArgs fa{cfg.gpart_mass,
+                         cfg.alpha_u,
+                         cfg.alpha_AV,
+                         cfg.beta_AV,
+                         cfg.cfl_cour * cfl_mult,
+                         cfg.cfl_force * cfl_mult,
+                         cfg.mixed_precision};
+            compute_force(stream, cfg.kernel, N, csr(), x.data(), y.data(), z.data(), vx.data(),
+                          vy.data(), vz.data(), h.data(), omega.data(), u.data(), P.data(),
+                          cs.data(), ax.data(), ay.data(), az.data(), du.data(), ax_old.data(),
+                          ay_old.data(), az_old.data(), du_old.data(), vsig.data(), dtc.data(),
+                          fa, red.data() + 3, red.data() + 7);
+            timer.end(stream, PH_FORCE, b);
+            last.n_force_evals++;
+            b = timer.begin(stream);
+            fill_u64(stream, red.data() + 4, ORDERED_NEG_INF, 1);
+            fill_u64(stream, red.data() + 5, 0, 2);
+            corrector(stream, N, dtc_ / 2, vx.data(), vy.data(), vz.data(), ax.data(), ay.data(),
+                      az.data(), ax_old.data(), ay_old.data(), az_old.data(), u.data(), du.data(),
+                      du_old.data(), cfg.gpart_mass, red.data() + 4);
+            timer.end(stream, PH_CORR, b);
+            u64 k[5];
+            SHAMX_HIP(hipMemcpyAsync(k, red.data() + 3, 40, hipMemcpyDeviceToHost, stream));
+            SHAMX_HIP(hipStreamSynchronize(stream));
+            f64 dt_min         = ordered_to_f64(k[0]);
+            last.n_interacting = k[4];
+            f64 max_e2 = ordered_to_f64(k[1]);
+            f64 sum_vsq, energy;
+            std::memcpy(&sum_vsq, &k[2], 8);
+            std::memcpy(&energy, &k[3], 8);
+            f64 rank_veps_v = std::sqrt(max_e2);
+            f64 vmean       = std::sqrt(sum_vsq / f64(N));
+            f64 eps_v       = rank_veps_v / vmean;
+            if (vmean <= 0) eps_v = 0;
+            last.eps_v  = eps_v;
+            last.energy = energy;
+            if (eps_v > 1e-2) {
+                rerun = true;
+                cfl_mult /= 2;
+            } else {
+                rerun    = false;
+                next_cfl = dt_min;
+            }
+            corr_iter++;
+        } while (rerun);
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        auto t1 = std::chrono::steady_clock::now();
+
+        f64 ph[PH_N] = {0};
+        timer.collect(ph);
+        last.t_total  = std::chrono::duration<f64>(t1 - t0).count();
+        last.rate     = f64(N) / last.t_total;
+        last.t_ghost  = ph[PH_GHOST];
+        last.t_tree   = ph[PH_TREE];
+        last.t_sort   = ph[PH_SORT];
+        last.t_neigh  = ph[PH_NEIGH];
+        last.t_hiter  = ph[PH_HITER];
+        last.t_omega  = ph[PH_OMEGA];
+        last.t_eos    = ph[PH_EOS];
+        last.t_force  = ph[PH_FORCE];
+        last.t_corr   = ph[PH_CORR];
+        last.n_local  = N;
+        last.n_merged = N + G;
+        last.n_leaf   = tree.L;
+        last.n_pairs  = nb.n_pairs;
+        last.corrector_iterations = corr_iter;
+        last.next_dt  = next_cfl;
+        last.time     = t_cur;
+        last.dt       = dtc_;
+        // Solver.cpp:3235-3263
+        dt        = next_cfl;
+        t         = t_cur + dtc_;
+        f64 stiff = cfg.cfl_multiplier_stiffness;
+        cfl_mult  = (cfl_mult * stiff + 1.) / (stiff + 1.);
+    }
+
+    // periodic box for the predictor wrap, kept in device memory
+    DevArray<f64> box_dev;
+    const f64 *nullptr_box() {
+        if (!cfg.periodic) return nullptr;
+        if (box_dev.size() != 6) {
+            box_dev.resize(6);
+            f64 b[6] = {cfg.box_min[0], cfg.box_min[1], cfg.box_min[2],
+                        cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
+            SHAMX_HIP(hipMemcpy(box_dev.data(), b, sizeof(b), hipMemcpyHostToDevice));
+        }
+        return box_dev.data();
+    }
+
+    ~shamx_solver() {
+        if (stream) (void) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+    template<class T>
+    void d2h(hipStream_t s, T *dst, const T *src, size_t n) {
+        if (n) SHAMX_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    template<class T>
+    void d2d(hipStream_t s, T *dst, const T *src, size_t n) {
+        if (dst && n) SHAMX_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    // node records -> interleaved aabb (3 per cell) and rint
+    void unpack_nodes(const std::vector<f64> &nodes, u32 cells, f64 *amin, f64 *amax, f64 *rint) {
+        for (u32 c = 0; c < cells; c++) {
+            const f64 *r = nodes.data() + (size_t) c * NODE_F64;
+            if (amin)
+                for (int k = 0; k < 3; k++) amin[3 * c + k] = r[k];
+            if (amax)
+                for (int k = 0; k < 3; k++) amax[3 * c + k] = r[3 + k];
+            if (rint) rint[c] = r[6];
+        }
+    }
+} // namespace
+
+extern "C" {
+
+const char *shamx_last_error(void) { return g_last_error.c_str(); }
+
+int shamx_get_device_count(int *count) {
+    return guarded([&] { SHAMX_HIP(hipGetDeviceCount(count)); });
+}
+
+int shamx_ctx_create(shamx_ctx **out) {
+    return guarded([&] {
+        need(out != nullptr, "out is NULL");
+        *out = new shamx_ctx();
+    });
+}
+int shamx_ctx_destroy(shamx_ctx *ctx) {
+    return guarded([&] { delete ctx; });
+}
+
+int shamx_bbox(shamx_ctx *ctx, void *stream, uint32_t n, const double *x, const double *y,
+               const double *z, double *bbox6) {
+    return guarded([&] {
+        need(ctx && n > 0, "shamx_bbox: null context or n == 0");
+        ctx->scratch.resize(bbox_scratch_bytes(n));
+        bbox_nextafter((hipStream_t) stream, n, x, y, z, bbox6, ctx->scratch.data());
+    });
+}
+
+int shamx_morton(shamx_ctx *ctx, void *stream, uint32_t n, const double *x, const double *y,
+                 const double *z, const double *bbox6, uint32_t *codes) {
+    return guarded([&] {
+        need(ctx != nullptr, "shamx_morton: null context");
+        morton_codes((hipStream_t) stream, n, x, y, z, bbox6, codes);
+    });
+}
+
+int shamx_sort(shamx_ctx *ctx, void *stream, uint32_t n, uint32_t key_bits, uint32_t *keys,
+               uint32_t *perm) {
+    return guarded([&] {
+        need(ctx != nullptr, "shamx_sort: null context");
+        need(key_bits >= 1 && key_bits <= 32, "shamx_sort: key_bits must be in [1, 32]");
+        ctx->alt_k.resize(std::max<u32>(n, 1));
+        ctx->alt_v.resize(std::max<u32>(n, 1));
+        ctx->scratch.resize(radix_sort_scratch_bytes(n));
+        radix_sort_pairs((hipStream_t) stream, n, key_bits, keys, perm, ctx->alt_k.data(),
+                         ctx->alt_v.data(), ctx->scratch.data());
+    });
+}
+
+int shamx_tree_build(shamx_ctx *ctx, void *stream, uint32_t n, const double *x, const double *y,
+                     const double *z, const double *h, uint32_t reduction_level, double htol) {
+    return guarded([&] {
+        need(ctx != nullptr, "shamx_tree_build: null context");
+        tree_build((hipStream_t) stream, ctx->tree, n, x, y, z, h, reduction_level, htol);
+    });
+}
+
+int shamx_tree_sizes(shamx_ctx *ctx, uint32_t *out4) {
+    return guarded([&] {
+        out4[0] = ctx->tree.n_obj;
+        out4[1] = ctx->tree.L;
+        out4[2] = ctx->tree.I;
+        out4[3] = 0;
+    });
+}
+
+int shamx_tree_get(shamx_ctx *ctx, void *stream, uint32_t *sorted, uint32_t *sort_index,
+                   uint32_t *reduc_map, uint32_t *lchild, uint32_t *rchild, uint8_t *lflag,
+                   uint8_t *rflag, uint32_t *endrange, double *aabb_min, double *aabb_max,
+                   double *rint) {
+    return guarded([&] {
+        hipStream_t s = (hipStream_t) stream;
+        TreeBuffers &t = ctx->tree;
+        d2d(s, sorted, t.sorted.data(), t.n_obj);
+        d2d(s, sort_index, t.sort_index.data(), t.n_obj);
+        d2d(s, reduc_map, t.reduc_map.data(), (size_t) t.L + 2);
+        d2d(s, lchild, t.lchild.data(), t.I);
+        d2d(s, rchild, t.rchild.data(), t.I);
+        d2d(s, lflag, t.lflag.data(), t.I);
+        d2d(s, rflag, t.rflag.data(), t.I);
+        d2d(s, endrange, t.endrange.data(), t.I);
+        if (aabb_min || aabb_max || rint) {
+            u32 cells = t.I + t.L;
+            std::vector<f64> nodes((size_t) cells * NODE_F64), a(3 * (size_t) cells),
+                bb(3 * (size_t) cells), r(cells);
+            d2h(s, nodes.data(), t.nodes.data(), nodes.size());
+            SHAMX_HIP(hipStreamSynchronize(s));
+            unpack_nodes(nodes, cells, a.data(), bb.data(), r.data());
+            if (aabb_min)
+                SHAMX_HIP(hipMemcpyAsync(aabb_min, a.data(), a.size() * 8, hipMemcpyHostToDevice, s));
+            if (aabb_max)
+                SHAMX_HIP(hipMemcpyAsync(aabb_max, bb.data(), bb.size() * 8, hipMemcpyHostToDevice, s));
+            if (rint)
+                SHAMX_HIP(hipMemcpyAsync(rint, r.data(), r.size() * 8, hipMemcpyHostToDevice, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+        }
+    });
+}
+
+int shamx_neigh_build(shamx_ctx *ctx, void *stream, uint32_t n_local, const double *x,
+                      const double *y, const double *z, const double *h, double htol,
+                      double Rkern) {
+    return guarded([&] {
+        need(ctx && ctx->tree.L > 0, "shamx_neigh_build: build the tree first");
+        need(n_local <= ctx->tree.n_obj, "shamx_neigh_build: n_local > tree object count");
+        neigh_build((hipStream_t) stream, ctx->neigh, ctx->tree, n_local, x, y, z, h, htol, Rkern);
+    });
+}
+
+int shamx_neigh_sizes(shamx_ctx *ctx, uint32_t *out4) {
+    return guarded([&] {
+        out4[0] = ctx->neigh.n_local;
+        out4[1] = (u32) (ctx->neigh.n_pairs & 0xffffffffu);
+        out4[2] = (u32) (ctx->neigh.n_pairs >> 32);
+        out4[3] = (u32) ctx->neigh.n_leaf_pairs;
+    });
+}
+
+int shamx_neigh_get(shamx_ctx *ctx, void *stream, uint32_t *cnt, uint32_t *offset, uint32_t *ids,
+                    uint32_t *owner, uint32_t *leaf_cnt, uint32_t *leaf_offset,
+                    uint32_t *leaf_ids) {
+    return guarded([&] {
+        hipStream_t s   = (hipStream_t) stream;
+        NeighBuffers &n = ctx->neigh;
+        d2d(s, cnt, n.cnt.data(), n.n_local);
+        d2d(s, offset, n.offset.data(), n.n_local);
+        d2d(s, ids, n.ids.data(), n.n_pairs);
+        d2d(s, owner, n.owner.data(), n.n_local);
+        d2d(s, leaf_cnt, n.leaf_cnt.data(), ctx->tree.L);
+        d2d(s, leaf_offset, n.leaf_offset.data(), ctx->tree.L);
+        d2d(s, leaf_ids, n.leaf_ids.data(), n.n_leaf_pairs);
+    });
+}
+
+int shamx_h_iterate(void *stream, int kernel, uint32_t n, const uint32_t *cnt,
+                    const uint32_t *offset, const uint32_t *ids, const double *x,
+                    const double *y, const double *z, const double *h_old, double *h_new,
+                    double *eps, double gpart_mass, double h_evol_max, double h_evol_iter_max,
+                    int mixed, double *eps_minmax_host) {
+    return guarded([&] {
+        hipStream_t s = (hipStream_t) stream;
+        u64 *sl       = slots();
+        fill_u64(s, sl, ORDERED_NEG_INF, 1);
+        fill_u64(s, sl + 1, ORDERED_POS_INF, 1);
+        h_iterate(s, kernel, n, Csr{cnt, offset, ids}, x, y, z, h_old, h_new, eps, gpart_mass,
+                  h_evol_max, h_evol_iter_max, sl, mixed);
+        if (eps_minmax_host) {
+            u64 k[2];
+            SHAMX_HIP(hipMemcpyAsync(k, sl, 16, hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            eps_minmax_host[0] = ordered_to_f64(k[0]);
+            eps_minmax_host[1] = ordered_to_f64(k[1]);
+        }
+    });
+}
+
+int shamx_omega(void *stream, int kernel, uint32_t n, const uint32_t *cnt, const uint32_t *offset,
+                const uint32_t *ids, const double *x, const double *y, const double *z,
+                const double *h, double *omega, double gpart_mass) {
+    return guarded([&] {
+        compute_omega((hipStream_t) stream, kernel, n, Csr{cnt, offset, ids}, x, y, z, h, omega,
+                      gpart_mass);
+    });
+}
+
+int shamx_eos(void *stream, int kernel, uint32_t nm, int eos_kind, double gamma, double cs,
+              double cs0, double q, double r0, const double *h, const double *u, const double *x,
+              const double *y, const double *z, double *P, double *soundspeed,
+              double gpart_mass) {
+    return guarded([&] {
+        need(eos_kind >= 0 && eos_kind <= 2, "shamx_eos: unknown eos kind");
+        compute_eos((hipStream_t) stream, kernel, nm, h, u, x, y, z, P, soundspeed, gpart_mass,
+                    EosArgs{eos_kind, gamma, cs, cs0, q, r0});
+    });
+}
+
+int shamx_force(void *stream, int kernel, uint32_t n, const uint32_t *cnt, const uint32_t *offset,
+                const uint32_t *ids, const double *x, const double *y, const double *z,
+                const double *vx, const double *vy, const double *vz, const double *h,
+                const double *omega, const double *u, const double *P, const double *cs,
+                double *ax, double *ay, double *az, double *du, double *ax_old, double *ay_old,
+                double *az_old, double *du_old, double *vsig, double *dt_cfl, double gpart_mass,
+                double alpha_u, double alpha_AV, double beta_AV, double C_cour, double C_force,
+                int mixed, double *dt_min_host) {
+    return guarded([&] {
+        hipStream_t s = (hipStream_t) stream;
+        u64 *sl       = slots();
+        fill_u64(s, sl, ORDERED_POS_INF, 1);
+        compute_force(s, kernel, n, Csr{cnt, offset, ids}, x, y, z, vx, vy, vz, h, omega, u, P,
+                      cs, ax, ay, az, du, ax_old, ay_old, az_old, du_old, vsig, dt_cfl,
+                      ForceArgs{gpart_mass, alpha_u, alpha_AV, beta_AV, C_cour, C_force, mixed},
+                      sl, nullptr);
+        if (dt_min_host) {
+            u64 k;
+            SHAMX_HIP(hipMemcpyAsync(&k, sl, 8, hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            *dt_min_host = ordered_to_f64(k);
+        }
+    });
+}
+
+int shamx_predictor(void *stream, uint32_t n, double dt, double *x, double *y, double *z,
+                    double *vx, double *vy, double *vz, const double *ax, const double *ay,
+                    const double *az, double *u, const double *du, int periodic,
+                    const double *box6_host) {
+    return guarded([&] {
+        hipStream_t s = (hipStream_t) stream;
+        DevArray<f64> box(6);
+        if (periodic) {
+            need(box6_host != nullptr, "shamx_predictor: periodic needs box6");
+            SHAMX_HIP(hipMemcpyAsync(box.data(), box6_host, 48, hipMemcpyHostToDevice, s));
+        }
+        predictor(s, n, dt, x, y, z, vx, vy, vz, ax, ay, az, u, du, periodic, box.data());
+        SHAMX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int shamx_corrector(void *stream, uint32_t n, double hdt, double *vx, double *vy, double *vz,
+                    const double *ax, const double *ay, const double *az, const double *ax_old,
+                    const double *ay_old, const double *az_old, double *u, const double *du,
+                    const double *du_old, double gpart_mass, double *out3_host) {
+    return guarded([&] {
+        hipStream_t s = (hipStream_t) stream;
+        u64 *sl       = slots();
+        fill_u64(s, sl, ORDERED_NEG_INF, 1);
+        fill_u64(s, sl + 1, 0, 2);
+        corrector(s, n, hdt, vx, vy, vz, ax, ay, az, ax_old, ay_old, az_old, u, du, du_old,
+                  gpart_mass, sl);
+        u64 k[3];
+        SHAMX_HIP(hipMemcpyAsync(k, sl, 24, hipMemcpyDeviceToHost, s));
+        SHAMX_HIP(hipStreamSynchronize(s));
+        if (out3_host) {
+            out3_host[0] = ordered_to_f64(k[0]);
+            std::memcpy(&out3_host[1], &k[1], 8);
+            std::memcpy(&out3_host[2], &k[2], 8);
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------- solver
+int shamx_solver_create(const shamx_config *cfg, uint32_t n, shamx_solver **out) {
+    return guarded([&] {
+        need(cfg && out, "shamx_solver_create: null argument");
+        need(cfg->kernel >= 0 && cfg->kernel <= 5, "unknown SPH kernel");
+        need(cfg->eos_kind >= 0 && cfg->eos_kind <= 2, "unknown equation of state");
+        need(cfg->tree_reduction_level >= 0, "tree_reduction_level must be >= 0");
+        need(cfg->htol_up_coarse >= cfg->htol_up_fine,
+             "htol_up_coarse_cycle must be larger or equal to htol_up_fine_cycle");
+        if (cfg->periodic)
+            for (int k = 0; k < 3; k++)
+                need(cfg->box_max[k] > cfg->box_min[k], "periodic box must have positive size");
+        auto s = std::make_unique<shamx_solver>();
+        s->cfg = *cfg;
+        SHAMX_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->Rkern = kernel_rkern(cfg->kernel);
+        s->alloc(n);
+        *out = s.release();
+    });
+}
+
+int shamx_solver_destroy(shamx_solver *s) {
+    return guarded([&] { delete s; });
+}
+
+int shamx_solver_set_field(shamx_solver *s, int field, const double *src, int on_device) {
+    return guarded([&] {
+        auto *f = s->field(field);
+        SHAMX_HIP(hipMemcpyAsync(f->data(), src, (size_t) s->N * sizeof(f64),
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                 s->stream));
+        SHAMX_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int shamx_solver_get_field(shamx_solver *s, int field, double *dst, int on_device) {
+    return guarded([&] {
+        auto *f = s->field(field);
+        SHAMX_HIP(hipMemcpyAsync(dst, f->data(), (size_t) s->N * sizeof(f64),
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                 s->stream));
+        SHAMX_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int shamx_solver_get_time(shamx_solver *s, double *tdc3) {
+    return guarded([&] {
+        tdc3[0] = s->t;
+        tdc3[1] = s->dt;
+        tdc3[2] = s->cfl_mult;
+    });
+}
+
+int shamx_solver_set_time(shamx_solver *s, const double *tdc3) {
+    return guarded([&] {
+        s->t        = tdc3[0];
+        s->dt       = tdc3[1];
+        s->cfl_mult = tdc3[2];
+    });
+}
+
+int shamx_solver_evolve_once(shamx_solver *s, shamx_step_log *log) {
+    return guarded([&] {
+        s->evolve_once();
+        if (log) *log = s->last;
+    });
+}
+
+int shamx_solver_stream(shamx_solver *s, void **stream) {
+    return guarded([&] { *stream = (void *) s->stream; });
+}
+
+int shamx_solver_sizes(shamx_solver *s, uint64_t *out6) {
+    return guarded([&] {
+        out6[0] = s->N + s->G;
+        out6[1] = s->G;
+        out6[2] = s->tree.L;
+        out6[3] = s->tree.I;
+        out6[4] = s->nb.n_pairs;
+        out6[5] = s->nb.n_leaf_pairs;
+    });
+}
+
+int shamx_solver_get_f64(shamx_solver *s, int what, double *dst) {
+    return guarded([&] {
+        hipStream_t st = s->stream;
+        u32 Nm = s->N + s->G, cells = s->tree.I + s->tree.L;
+        DevArray<f64> *m[11] = {&s->x, &s->y, &s->z, &s->h, &s->vx, &s->vy, &s->vz,
+                                &s->u, &s->omega, &s->P, &s->cs};
+        if (what >= 0 && what <= 10) d2h(st, dst, m[what]->data(), Nm);
+        else if (what == 11) d2h(st, dst, s->omega.data(), s->N);
+        else if (what == 12) d2h(st, dst, s->vsig.data(), s->N);
+        else if (what == 13) d2h(st, dst, s->tree.bbox6.data(), 6);
+        else if (what >= 14 && what <= 16) {
+            std::vector<f64> nodes((size_t) cells * NODE_F64);
+            d2h(st, nodes.data(), s->tree.nodes.data(), nodes.size());
+            SHAMX_HIP(hipStreamSynchronize(st));
+            unpack_nodes(nodes, cells, what == 15 ? dst : nullptr, what == 16 ? dst : nullptr,
+                         what == 14 ? dst : nullptr);
+        } else throw std::invalid_argument("shamx_solver_get_f64: unknown id");
+        SHAMX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int shamx_solver_get_u32(shamx_solver *s, int what, uint32_t *dst) {
+    return guarded([&] {
+        hipStream_t st = s->stream;
+        u32 Nm = s->N + s->G, L = s->tree.L, I = s->tree.I;
+        switch (what) {
+        case 0: d2h(st, dst, s->ghost_src.data(), s->G); break;
+        case 1: d2h(st, dst, s->tree.sorted.data(), Nm); break;
+        case 2: d2h(st, dst, s->tree.sort_index.data(), Nm); break;
+        case 3: d2h(st, dst, s->tree.reduc_map.data(), (size_t) L + 2); break;
+        case 4: d2h(st, dst, s->nb.owner.data(), s->N); break;
+        case 5: d2h(st, dst, s->nb.cnt.data(), s->N); break;
+        case 6: d2h(st, dst, s->nb.offset.data(), s->N); break;
+        case 7: d2h(st, dst, s->nb.ids.data(), s->nb.n_pairs); break;
+        case 8: d2h(st, dst, s->nb.leaf_cnt.data(), L); break;
+        case 9: d2h(st, dst, s->nb.leaf_offset.data(), L); break;
+        case 10: d2h(st, dst, s->nb.leaf_ids.data(), s->nb.n_leaf_pairs); break;
+        case 11: d2h(st, dst, s->tree.lchild.data(), I); break;
+        case 12: d2h(st, dst, s->tree.rchild.data(), I); break;
+        case 13: d2h(st, dst, s->tree.endrange.data(), I); break;
+        case 14: d2h(st, dst, s->tree.codes.data(), Nm); break;
+        default: throw std::invalid_argument("shamx_solver_get_u32: unknown id");
+        }
+        SHAMX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int shamx_solver_get_u8(shamx_solver *s, int what, uint8_t *dst) {
+    return guarded([&] {
+        hipStream_t st = s->stream;
+        if (what == 0) d2h(st, dst, s->tree.lflag.data(), s->tree.I);
+        else if (what == 1) d2h(st, dst, s->tree.rflag.data(), s->tree.I);
+        else throw std::invalid_argument("shamx_solver_get_u8: unknown id");
+        SHAMX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,138 @@
+// SPDX-License-Identifier: CeCILL-2.1
+//
+// launch.hpp -- host-side launchers of the HIP kernels (implemented in prims.hip, tree.hip,
+// neigh.hip, sph.hip).  All launches are asynchronous on the given stream; functions that must
+// return a count to the host say so and synchronise the stream.
+#pragma once
+
+#include "common.hpp"
+#include "devbuf.hpp"
+
+namespace shamx {
+
+    // ------------------------------------------------------------------ prims.hip
+    void fill_u32(hipStream_t s, u32 *p, u32 v, size_t n);
+    void fill_u64(hipStream_t s, u64 *p, u64 v, size_t n);
+    void fill_f64(hipStream_t s, f64 *p, f64 v, size_t n);
+    // exclusive prefix sum of n u32 values; writes the grand total to *total (device)
+    size_t scan_scratch_bytes(u32 n);
+    void exclusive_scan_u32(hipStream_t s, const u32 *in, u32 *out, u32 n, u32 *total,
+                            void *scratch);
+
+    // ------------------------------------------------------------------ tree.hip
+    // bounding box of n positions widened outward by one ulp (BuildTrees.cpp:38-55);
+    // bbox6 = {xmin, ymin, zmin, xmax, ymax, zmax} in device memory
+    size_t bbox_scratch_bytes(u32 n);
+    void bbox_nextafter(hipStream_t s, u32 n, const f64 *x, const f64 *y, const f64 *z,
+                        f64 *bbox6, void *scratch);
+    // 30-bit Morton codes (MortonCodeSet.cpp:87-99)
+    void morton_codes(hipStream_t s, u32 n, const f64 *x, const f64 *y, const f64 *z,
+                      const f64 *bbox6, u32 *codes);
+    // stable LSD radix sort of (key, index) pairs; sorts keys[0..n) and returns the permutation
+    // in vals (vals are generated as iota in the first pass).  key_bits <= 32.
+    size_t radix_sort_scratch_bytes(u32 n);
+    void radix_sort_pairs(hipStream_t s, u32 n, u32 key_bits, u32 *keys, u32 *vals,
+                          u32 *keys_alt, u32 *vals_alt, void *scratch);
+
+    // Tree state of a compressed-leaf BVH built on the merged particle set
+    struct TreeBuffers {
+        u32 n_obj = 0, L = 0, I = 0;
+        DevArray<f64> bbox6;           // 6
+        DevArray<u32> codes;           // n_obj (unsorted Morton codes)
+        DevArray<u32> sorted;          // n_obj sorted codes
+        DevArray<u32> sort_index;      // n_obj permutation (map_morton_id_to_obj_id)
+        DevArray<u32> keys_alt, vals_alt;
+        DevArray<u32> split_a, split_b;
+        DevArray<u32> reduc_map;       // L + 2
+        DevArray<u32> leaf_codes;      // L
+        DevArray<u32> lchild, rchild, endrange; // I
+        DevArray<u8> lflag, rflag;     // I
+        DevArray<u32> parent;          // I + L
+        DevArray<u32> visits;          // I
+        DevArray<f64> nodes;           // (I + L) * NODE_F64 records (tree_layout.hpp)
+        DevArray<u32> counters;        // small device scalars
+        DevArray<unsigned char> scratch;
+    };
+    // full CompressedLeafBVH::rebuild_from_positions + rint field (BuildTrees.cpp:27-65,
+    // CompressedLeafBVH.cpp:31-95, Solver.cpp:1322-1356).  Synchronises once to read L.
+    void tree_build(hipStream_t s, TreeBuffers &t, u32 n, const f64 *x, const f64 *y,
+                    const f64 *z, const f64 *h, u32 reduction_level, f64 htol);
+    // the same build in three stages (for per-stage timing): bbox + Morton codes, radix sort,
+    // then reduction + Karras + boxes (synchronises once to read the leaf count)
+    void tree_stage_codes(hipStream_t s, TreeBuffers &t, u32 n, const f64 *x, const f64 *y,
+                          const f64 *z);
+    void tree_stage_sort(hipStream_t s, TreeBuffers &t);
+    void tree_stage_finish(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
+                           const f64 *z, const f64 *h, u32 reduction_level, f64 htol);
+
+    // ------------------------------------------------------------------ neigh.hip
+    struct NeighBuffers {
+        u32 n_local = 0;
+        u64 n_pairs = 0, n_leaf_pairs = 0;
+        DevArray<u32> leaf_cnt, leaf_offset, leaf_ids; // stage 1 (per leaf)
+        DevArray<u32> owner;                            // per local particle
+        DevArray<u32> cnt, offset, ids;                 // stage 2 CSR (per local particle)
+        DevArray<u32> counters;
+        DevArray<unsigned char> scratch;
+    };
+    // NeighbourCache::start_neighbors_cache_2stages (NeighbourCache.cpp:222-604).
+    // Synchronises twice to size the leaf list and the particle list.
+    void neigh_build(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
+                     const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
+                     f64 Rkern);
+
+    // ------------------------------------------------------------------ sph.hip
+    struct Csr {
+        const u32 *cnt, *offset, *ids;
+    };
+    // IterateSmoothingLengthDensity (one Newton sweep); eps_minmax (device, 2 ordered keys)
+    // receives max and min of eps over all n particles.
+    void h_iterate(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
+                   const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
+                   f64 h_evol_max, f64 h_evol_iter_max, u64 *eps_minmax, int mixed);
+    void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
+                       const f64 *z, const f64 *h, f64 *omega, f64 mass);
+    struct EosArgs {
+        int kind;
+        f64 gamma, cs, cs0, q, r0;
+    };
+    void compute_eos(hipStream_t s, int kernel, u32 nm, const f64 *h, const f64 *u,
+                     const f64 *x, const f64 *y, const f64 *z, f64 *P, f64 *cs, f64 mass,
+                     EosArgs e);
+    struct ForceArgs {
+        f64 mass, alpha_u, alpha_AV, beta_AV;
+        f64 C_cour, C_force; // CFL factors (cfl * multiplier)
+        int mixed;           // 1: FP32 pair sums with Kahan compensation
+    };
+    // UpdateDerivs constant AV (+ a_old save, vsig, per-particle CFL dt folded into dt_min key)
+    void compute_force(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
+                       const f64 *z, const f64 *vx, const f64 *vy, const f64 *vz, const f64 *h,
+                       const f64 *omega, const f64 *u, const f64 *P, const f64 *cs, f64 *ax,
+                       f64 *ay, f64 *az, f64 *du, f64 *ax_old, f64 *ay_old, f64 *az_old,
+                       f64 *du_old, f64 *vsig, f64 *dt_cfl, ForceArgs fa, u64 *dt_min_key,
+                       u64 *pair_count);
+    // leapfrog predictor (Solver.cpp:390-524) fused with the periodic wrap (integrators.cpp:206)
+    void predictor(hipStream_t s, u32 n, f64 dt, f64 *x, f64 *y, f64 *z, f64 *vx, f64 *vy,
+                   f64 *vz, const f64 *ax, const f64 *ay, const f64 *az, f64 *u, const f64 *du,
+                   int periodic, const f64 *box6);
+    // leapfrog corrector (integrators.cpp:87-119) + reductions for eps_v and energy:
+    // red[0] ordered key of max |dv|^2, red[1] sum |v|^2 (f64 bits), red[2] energy (f64 bits)
+    void corrector(hipStream_t s, u32 n, f64 hdt, f64 *vx, f64 *vy, f64 *vz, const f64 *ax,
+                   const f64 *ay, const f64 *az, const f64 *ax_old, const f64 *ay_old,
+                   const f64 *az_old, f64 *u, const f64 *du, const f64 *du_old, f64 mass,
+                   u64 *red);
+    // max over n values as an ordered key (device)
+    void max_key_f64(hipStream_t s, u32 n, const f64 *v, u64 *key);
+    // periodic image selection (BasicSPHGhosts.cpp:260-579): returns G (synchronises) and
+    // fills src ids + appends positions/h of the images after the n local entries.
+    u32 periodic_ghosts(hipStream_t s, u32 n, f64 *x, f64 *y, f64 *z, f64 *h, const f64 *box6,
+                        f64 int_range, DevArray<u32> &ghost_src, DevArray<u8> &ghost_off_id,
+                        DevArray<u32> &work, DevArray<unsigned char> &scratch, u32 cap);
+    // ghost field refresh: dst[n + g] = src[ghost_src[g]] (+ offset for positions)
+    struct FieldPtrs {
+        f64 *p[8];
+        int n;
+    };
+    void gather_ghosts(hipStream_t s, u32 n, u32 G, const u32 *ghost_src, FieldPtrs fields);
+
+} // namespace shamx

@@ -1,0 +1,548 @@
+// SPDX-License-Identifier: CeCILL-2.1
+//
+// tree.hip -- compressed-leaf BVH build on the merged (local + ghost) particle set:
+//   bbox with one-ulp outward widening   (BuildTrees.cpp:38-55)
+//   30-bit Morton codes                  (MortonCodeSet.cpp:62-118, morton.hpp:28-230)
+//   stable onesweep LSD radix sort        (replaces bitonicSort_updated_usm.cpp:284-396)
+//   leaf reduction                        (reduction_alg.cpp:62-585, MortonReducedSet.cpp:24-73)
+//   Karras 2012 radix tree                (KarrasRadixTree.cpp:45-180)
+//   leaf AABB + max-h field, propagated bottom-up in one pass with arrival counters
+//   (replaces the tree_depth = 32 full passes of KarrasRadixTreeAABB.cpp:32-74 and
+//    KarrasRadixTreeField.hpp:133-169; min / max are exact so the result is identical).
+//
+// Node records: every cell (I internal cells then L leaves) owns one 64-byte record
+//   [0..2] aabb min, [3..5] aabb max, [6] rint = max h * htol,
+//   [7]    internal: (left total id, right total id); leaf: (first, end) in sorted order
+// so a traversal step reads exactly one record.
+#include "launch.hpp"
+#include "tree_layout.hpp"
+
+namespace shamx {
+
+    namespace {
+        // --------------------------------------------------------------------- bbox
+        constexpr int BB_BLOCK = 256;
+        constexpr u32 BB_MAX_BLOCKS = 1024;
+
+        __device__ inline f64 next_up(f64 v) {
+            if (v != v || v == __builtin_inf()) return v;
+            if (v == 0.) return __longlong_as_double(1ll);
+            i64 b = __double_as_longlong(v);
+            return __longlong_as_double(v > 0 ? b + 1 : b - 1);
+        }
+        __device__ inline f64 next_down(f64 v) {
+            if (v != v || v == -__builtin_inf()) return v;
+            if (v == 0.) return __longlong_as_double((i64) 0x8000000000000001ull);
+            i64 b = __double_as_longlong(v);
+            return __longlong_as_double(v > 0 ? b - 1 : b + 1);
+        }
+
+        __global__ void __launch_bounds__(BB_BLOCK)
+            k_bbox_partial(u32 n, const f64 *__restrict x, const f64 *__restrict y,
+                           const f64 *__restrict z, f64 *__restrict part) {
+            f64 v[6] = {__builtin_inf(), __builtin_inf(), __builtin_inf(),
+                        -__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+            for (u32 i = blockIdx.x * BB_BLOCK + threadIdx.x; i < n; i += gridDim.x * BB_BLOCK) {
+                f64 a = x[i], b = y[i], c = z[i];
+                v[0] = fmin(v[0], a);
+                v[1] = fmin(v[1], b);
+                v[2] = fmin(v[2], c);
+                v[3] = fmax(v[3], a);
+                v[4] = fmax(v[4], b);
+                v[5] = fmax(v[5], c);
+            }
+            __shared__ f64 lds[6][BB_BLOCK / WAVE];
+            int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                v[k] = (k < 3) ? wave_reduce(v[k], [](f64 a, f64 b) { return fmin(a, b); })
+                               : wave_reduce(v[k], [](f64 a, f64 b) { return fmax(a, b); });
+                if (lane == 0) lds[k][wid] = v[k];
+            }
+            __syncthreads();
+            if (threadIdx.x < 6) {
+                int k = threadIdx.x;
+                f64 r = lds[k][0];
+                for (int w = 1; w < BB_BLOCK / WAVE; w++)
+                    r = (k < 3) ? fmin(r, lds[k][w]) : fmax(r, lds[k][w]);
+                part[blockIdx.x * 6 + k] = r;
+            }
+        }
+
+        __global__ void k_bbox_final(u32 nb, const f64 *__restrict part, f64 *__restrict bb) {
+            if (threadIdx.x < 6) {
+                int k = threadIdx.x;
+                f64 r = part[k];
+                for (u32 b = 1; b < nb; b++)
+                    r = (k < 3) ? fmin(r, part[b * 6 + k]) : fmax(r, part[b * 6 + k]);
+                bb[k] = (k < 3) ? next_down(r) : next_up(r);
+            }
+        }
+
+        // --------------------------------------------------------------------- Morton
+        __device__ inline u32 expand_bits_10(u32 v) { // bmi.hpp:42-51
+            v &= 0x3ffU;
+            v = (v | v << 16U) & 0x30000ffU;
+            v = (v | v << 8U) & 0x300f00fU;
+            v = (v | v << 4U) & 0x30c30c3U;
+            v = (v | v << 2U) & 0x9249249U;
+            return v;
+        }
+        __device__ inline u32 to_grid(f64 c, f64 lo, f64 hi) {
+            c        = fmin(fmax(c, lo), hi);        // AABB::clamp_coord
+            f64 fact = (hi - lo) / 1024.;            // CoordRangeTransform<u16_3,f64_3>
+            u16 g    = (u16) ((c - lo) / fact);      // reverse_transform, static_cast<u16>
+            return (u32) min(max(g, (u16) 0), (u16) 1023); // MortonConverter::to_morton_grid
+        }
+        __global__ void k_morton(u32 n, const f64 *__restrict x, const f64 *__restrict y,
+                                 const f64 *__restrict z, const f64 *__restrict bb,
+                                 u32 *__restrict codes) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            u32 gx = to_grid(x[i], bb[0], bb[3]);
+            u32 gy = to_grid(y[i], bb[1], bb[4]);
+            u32 gz = to_grid(z[i], bb[2], bb[5]);
+            codes[i] = expand_bits_10(gx) * 4 + expand_bits_10(gy) * 2 + expand_bits_10(gz);
+        }
+
+        // --------------------------------------------------------------------- radix sort
+        // Onesweep (one global histogram pass, then one scatter kernel per 8-bit digit with
+        // decoupled look-back across tiles).  Ranks inside a tile are computed with wave-wide
+        // peer masks built from ballots, in (wave, item, lane) order, so the sort is stable.
+        constexpr int RS_BLOCK = 256;
+        constexpr int RS_WAVES = RS_BLOCK / WAVE;
+        constexpr int RS_ITEMS = 16;
+        constexpr int RS_TILE  = RS_BLOCK * RS_ITEMS;
+        constexpr int RS_BINS  = 256;
+        constexpr u32 ST_AGG = 1u << 30, ST_PREFIX = 2u << 30, ST_VALUE = (1u << 30) - 1;
+        constexpr u64 RS_SPIN_LIMIT = 200000000ull; // 2 s of the 100 MHz constant clock
+
+        __global__ void __launch_bounds__(RS_BLOCK)
+            k_rs_hist(u32 n, const u32 *__restrict keys, u32 passes, u32 *__restrict hist) {
+            __shared__ u32 h[4][RS_BINS];
+            for (int i = threadIdx.x; i < 4 * RS_BINS; i += RS_BLOCK) (&h[0][0])[i] = 0;
+            __syncthreads();
+            for (u32 i = blockIdx.x * RS_BLOCK + threadIdx.x; i < n; i += gridDim.x * RS_BLOCK) {
+                u32 k = keys[i];
+                for (u32 p = 0; p < passes; p++) atomicAdd(&h[p][(k >> (8 * p)) & 0xff], 1u);
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < (int) passes * RS_BINS; i += RS_BLOCK) {
+                u32 v = (&h[0][0])[i];
+                if (v) atomicAdd(&hist[i], v);
+            }
+        }
+
+        __global__ void k_rs_hist_scan(u32 passes, u32 *__restrict hist) {
+            // one thread per pass: 256 bins, exclusive
+            u32 p = threadIdx.x;
+            if (p >= passes) return;
+            u32 run = 0;
+            for (int d = 0; d < RS_BINS; d++) {
+                u32 v                 = hist[p * RS_BINS + d];
+                hist[p * RS_BINS + d] = run;
+                run += v;
+            }
+        }
+
+        __device__ inline u64 lanemask_lt() {
+            u32 lane = __lane_id();
+            return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        }
+
+        __global__ void __launch_bounds__(RS_BLOCK)
+            k_rs_onesweep(u32 n, u32 shift, const u32 *__restrict keys_in,
+                          const u32 *__restrict vals_in, u32 *__restrict keys_out,
+                          u32 *__restrict vals_out, const u32 *__restrict digit_base,
+                          u32 *__restrict status, u32 *__restrict tile_counter, int first,
+                          u32 *__restrict err) {
+            __shared__ u32 s_tile;
+            __shared__ u32 wave_hist[RS_WAVES][RS_BINS];
+            __shared__ u32 tile_prefix[RS_BINS];
+            const u32 lane = __lane_id(), wid = threadIdx.x / WAVE;
+            for (int i = threadIdx.x; i < RS_WAVES * RS_BINS; i += RS_BLOCK)
+                (&wave_hist[0][0])[i] = 0;
+            if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
+            __syncthreads();
+            const u32 tile = s_tile;
+            const u64 base = (u64) tile * RS_TILE + (u64) wid * (WAVE * RS_ITEMS);
+            const u64 lt   = lanemask_lt();
+
+            u32 key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+#pragma unroll
+            for (int k = 0; k < RS_ITEMS; k++) {
+                u64 idx    = base + (u64) k * WAVE + lane;
+                bool valid = idx < n;
+                key[k]     = valid ? keys_in[idx] : 0u;
+                val[k]     = valid ? (first ? (u32) idx : vals_in[idx]) : 0u;
+                u32 d      = (key[k] >> shift) & 0xffu;
+                // peers: valid lanes holding the same digit
+                u64 peers = __ballot(valid);
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    u64 bb = __ballot((d >> b) & 1u);
+                    peers &= ((d >> b) & 1u) ? bb : ~bb;
+                }
+                u32 before = (u32) __popcll(peers & lt);
+                u32 cnt    = (u32) __popcll(peers);
+                u32 prev   = valid ? wave_hist[wid][d] : 0u;
+                rank[k]    = prev + before;
+                // the lowest peer writes the updated count after every peer has read it
+                if (valid && before == 0) wave_hist[wid][d] = prev + cnt;
+                __builtin_amdgcn_wave_barrier();
+            }
+            __syncthreads();
+
+            // per digit: exclusive scan across waves, tile aggregate, decoupled look-back
+            {
+                const u32 d = threadIdx.x; // RS_BLOCK == RS_BINS
+                u32 run     = 0;
+#pragma unroll
+                for (int w = 0; w < RS_WAVES; w++) {
+                    u32 t           = wave_hist[w][d];
+                    wave_hist[w][d] = run;
+                    run += t;
+                }
+                u32 *st = status + (u64) tile * RS_BINS + d;
+                if (tile == 0) {
+                    __hip_atomic_store(st, ST_PREFIX | run, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    tile_prefix[d] = digit_base[d];
+                } else {
+                    __hip_atomic_store(st, ST_AGG | run, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    u32 excl = 0;
+                    i64 j    = (i64) tile - 1;
+                    u64 t0   = wall_clock64();
+                    while (j >= 0) {
+                        u32 s = __hip_atomic_load(status + (u64) j * RS_BINS + d,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if ((s & ~ST_VALUE) == 0) { // predecessor not published yet
+                            // predecessors always publish; a wait this long means a bug:
+                            // flag it and leave instead of hanging the device
+                            if (wall_clock64() - t0 > RS_SPIN_LIMIT) {
+                                atomicOr(err, 1u);
+                                break;
+                            }
+                            continue;
+                        }
+                        excl += s & ST_VALUE;
+                        if ((s & ~ST_VALUE) == ST_PREFIX) break;
+                        j--;
+                    }
+                    __hip_atomic_store(st, ST_PREFIX | (excl + run), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    tile_prefix[d] = digit_base[d] + excl;
+                }
+            }
+            __syncthreads();
+
+#pragma unroll
+            for (int k = 0; k < RS_ITEMS; k++) {
+                u64 idx = base + (u64) k * WAVE + lane;
+                if (idx < n) {
+                    u32 d   = (key[k] >> shift) & 0xffu;
+                    u32 dst = tile_prefix[d] + wave_hist[wid][d] + rank[k];
+                    if (dst < n) {
+                        keys_out[dst] = key[k];
+                        vals_out[dst] = val[k];
+                    } else {
+                        atomicOr(err, 2u);
+                    }
+                }
+            }
+        }
+
+        // --------------------------------------------------------------------- reduction
+        __device__ inline i32 clz32(u32 v) { return v ? __clz(v) : 32; }
+        // karras_delta: shambackends/math.hpp:825-828
+        __device__ inline i32 kdelta(i32 x, i32 y, u32 len, const u32 *__restrict m) {
+            return ((u32) y > len - 1 || y < 0) ? -1 : clz32(m[x] ^ m[y]);
+        }
+
+        __global__ void k_split(u32 n, const u32 *__restrict m, u32 *__restrict s) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            s[i] = (i > 0) ? (m[i - 1] != m[i] ? 1u : 0u) : 1u;
+        }
+
+        // sycl_reduction_iteration (NEW_BEHAVIOR), reduction_alg.cpp:196-244
+        __global__ void k_reduc_iter(u32 n, const u32 *__restrict m, const u32 *__restrict s_in,
+                                     u32 *__restrict s_out) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            u32 before1 = i - 1;
+            while (before1 <= n - 1 && !s_in[before1]) before1--;
+            u32 before2 = before1 - 1;
+            while (before2 <= n - 1 && !s_in[before2]) before2--;
+            u32 next1 = i + 1;
+            while (next1 <= n - 1 && !s_in[next1]) next1++;
+            i32 d0  = kdelta((i32) i, (i32) next1, n, m);
+            i32 dm  = kdelta((i32) i, (i32) before1, n, m);
+            i32 dmm = kdelta((i32) before1, (i32) before2, n, m);
+            s_out[i] = (!(d0 < dm && dmm < dm) && s_in[i]) ? 1u : 0u;
+        }
+
+        __global__ void k_compact(u32 n, const u32 *__restrict flag, const u32 *__restrict off,
+                                  u32 *__restrict map) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            if (flag[i]) map[off[i]] = i;
+        }
+
+        // update_morton_buf tail (reduction_alg.cpp:274-295) + remap (:564-585)
+        __global__ void k_leaf_codes(u32 n, u32 L, u32 *__restrict map, const u32 *__restrict m,
+                                     u32 *__restrict leaf_codes) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i < L) leaf_codes[i] = m[map[i]];
+            if (i == 0) {
+                map[L]     = n;
+                map[L + 1] = 0;
+            }
+        }
+
+        // --------------------------------------------------------------------- Karras
+        __global__ void k_karras(u32 L, const u32 *__restrict morton, u32 *__restrict lchild,
+                                 u32 *__restrict rchild, u8 *__restrict lflag,
+                                 u8 *__restrict rflag, u32 *__restrict endrange,
+                                 u32 *__restrict parent, f64 *__restrict nodes) {
+            u32 I  = L - 1;
+            u32 iu = blockIdx.x * blockDim.x + threadIdx.x;
+            if (iu >= I) return;
+            i32 i      = (i32) iu;
+            auto D     = [&](i32 a, i32 b) { return kdelta(a, b, L, morton); };
+            int ddelta = D(i, i + 1) - D(i, i - 1);
+            int d      = (ddelta == 0) ? 0 : ((ddelta > 0) ? 1 : -1);
+            int delta_min = D(i, i - d);
+            int lmax      = 2;
+            while (D(i, i + lmax * d) > delta_min) lmax *= 2;
+            int l = 0, t = lmax / 2;
+            while (t > 0) {
+                if (D(i, i + (l + t) * d) > delta_min) l = l + t;
+                t = t / 2;
+            }
+            u32 j          = (u32) (i + l * d);
+            endrange[iu]   = j;
+            int delta_node = D(i, (i32) j);
+            int s          = 0;
+            float div      = 2;
+            t              = (int) ceilf((float) l / div);
+            while (true) {
+                int tmp_ = i + (s + t) * d;
+                if (D(i, tmp_) > delta_node) s = s + t;
+                if (t <= 1) break;
+                div *= 2;
+                t = (int) ceilf((float) l / div);
+            }
+            int gamma = i + s * d + min(d, 0);
+            u8 lf     = (min(iu, j) == (u32) gamma) ? 1 : 0;
+            u8 rf     = (max(iu, j) == (u32) (gamma + 1)) ? 1 : 0;
+            lchild[iu] = (u32) gamma;
+            rchild[iu] = (u32) (gamma + 1);
+            lflag[iu]  = lf;
+            rflag[iu]  = rf;
+            u32 lt     = (u32) gamma + (lf ? I : 0u);
+            u32 rt     = (u32) (gamma + 1) + (rf ? I : 0u);
+            parent[lt] = iu;
+            parent[rt] = iu;
+            node_set_children(nodes, iu, lt, rt);
+            if (iu == 0) parent[0] = NO_PARENT;
+        }
+
+        // leaf boxes + max h, then climb: the second child to arrive at a parent merges
+        __global__ void k_leaf_aabb_climb(u32 L, u32 I, const u32 *__restrict map,
+                                          const u32 *__restrict sort_index,
+                                          const f64 *__restrict x, const f64 *__restrict y,
+                                          const f64 *__restrict z, const f64 *__restrict h,
+                                          f64 htol, const u32 *__restrict parent,
+                                          u32 *__restrict visits, f64 *__restrict nodes) {
+            u32 l = blockIdx.x * blockDim.x + threadIdx.x;
+            if (l >= L) return;
+            const f64 mx = 1.7976931348623157e308;
+            f64 lo0 = mx, lo1 = mx, lo2 = mx, hi0 = -mx, hi1 = -mx, hi2 = -mx;
+            f64 hm  = -mx;
+            u32 s0 = map[l], s1 = map[l + 1];
+            for (u32 s = s0; s < s1; s++) {
+                u32 o = sort_index[s];
+                f64 a = x[o], b = y[o], c = z[o];
+                lo0 = fmin(lo0, a);
+                lo1 = fmin(lo1, b);
+                lo2 = fmin(lo2, c);
+                hi0 = fmax(hi0, a);
+                hi1 = fmax(hi1, b);
+                hi2 = fmax(hi2, c);
+                hm  = fmax(hm, h[o]);
+            }
+            u32 node = I + l;
+            f64 *rec = nodes + (u64) node * NODE_F64;
+            rec[0] = lo0;
+            rec[1] = lo1;
+            rec[2] = lo2;
+            rec[3] = hi0;
+            rec[4] = hi1;
+            rec[5] = hi2;
+            // tree field max then *= htol (Solver.cpp:1345-1352): x -> x*htol is monotone,
+            // so scaling before the max-propagation gives the identical doubles
+            rec[6] = hm * htol;
+            node_set_range(nodes, node, s0, s1);
+            if (I == 0) return;
+            while (true) {
+                u32 p = parent[node];
+                if (p >= I) break; // root (NO_PARENT) or a corrupted link: stop climbing
+                __threadfence();
+                u32 old = atomicAdd(&visits[p], 1u);
+                if (old == 0) break;
+                __threadfence();
+                u32 lt, rt;
+                node_children(nodes, p, lt, rt);
+                if (lt >= I + L || rt >= I + L) break;
+                const f64 *A = nodes + (u64) lt * NODE_F64;
+                const f64 *B = nodes + (u64) rt * NODE_F64;
+                f64 *P       = nodes + (u64) p * NODE_F64;
+#pragma unroll
+                for (int k = 0; k < 3; k++) P[k] = fmin(A[k], B[k]);
+#pragma unroll
+                for (int k = 3; k < 7; k++) P[k] = fmax(A[k], B[k]);
+                node = p;
+            }
+        }
+    } // namespace
+
+    size_t bbox_scratch_bytes(u32) { return BB_MAX_BLOCKS * 6 * sizeof(f64); }
+
+    void bbox_nextafter(hipStream_t s, u32 n, const f64 *x, const f64 *y, const f64 *z,
+                        f64 *bbox6, void *scratch) {
+        u32 nb = std::max<u32>(1, std::min<u32>(BB_MAX_BLOCKS, div_up(n, BB_BLOCK * 8)));
+        k_bbox_partial<<<nb, BB_BLOCK, 0, s>>>(n, x, y, z, (f64 *) scratch);
+        k_bbox_final<<<1, 64, 0, s>>>(nb, (const f64 *) scratch, bbox6);
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void morton_codes(hipStream_t s, u32 n, const f64 *x, const f64 *y, const f64 *z,
+                      const f64 *bbox6, u32 *codes) {
+        if (n == 0) return;
+        k_morton<<<div_up(n, 256), 256, 0, s>>>(n, x, y, z, bbox6, codes);
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    size_t radix_sort_scratch_bytes(u32 n) {
+        u32 nt = div_up(std::max<u32>(n, 1), RS_TILE);
+        return (size_t) (4 * RS_BINS + 8 + (size_t) 4 * nt * RS_BINS) * sizeof(u32);
+    }
+
+    void radix_sort_pairs(hipStream_t s, u32 n, u32 key_bits, u32 *keys, u32 *vals,
+                          u32 *keys_alt, u32 *vals_alt, void *scratch) {
+        if (n == 0) return;
+        if (key_bits == 0 || key_bits > 32) throw std::invalid_argument("radix sort key_bits");
+        u32 passes   = div_up(key_bits, 8);
+        u32 nt       = div_up(n, RS_TILE);
+        u32 *hist    = (u32 *) scratch;
+        u32 *ctr     = hist + 4 * RS_BINS;
+        u32 *err     = ctr + 4;
+        u32 *status  = ctr + 8;
+        fill_u32(s, hist, 0, 4 * RS_BINS + 8 + (size_t) passes * nt * RS_BINS);
+        u32 hb = std::max<u32>(1, std::min<u32>(512, div_up(n, RS_BLOCK * 16)));
+        k_rs_hist<<<hb, RS_BLOCK, 0, s>>>(n, keys, passes, hist);
+        k_rs_hist_scan<<<1, 64, 0, s>>>(passes, hist);
+        u32 *kin = keys, *vin = vals, *kout = keys_alt, *vout = vals_alt;
+        for (u32 p = 0; p < passes; p++) {
+            k_rs_onesweep<<<nt, RS_BLOCK, 0, s>>>(n, 8 * p, kin, vin, kout, vout,
+                                                  hist + p * RS_BINS,
+                                                  status + (u64) p * nt * RS_BINS, ctr + p,
+                                                  p == 0 ? 1 : 0, err);
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
+        if (kin != keys) {
+            SHAMX_HIP(hipMemcpyAsync(keys, kin, n * sizeof(u32), hipMemcpyDeviceToDevice, s));
+            SHAMX_HIP(hipMemcpyAsync(vals, vin, n * sizeof(u32), hipMemcpyDeviceToDevice, s));
+        }
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void tree_stage_codes(hipStream_t s, TreeBuffers &t, u32 n, const f64 *x, const f64 *y,
+                          const f64 *z) {
+        if (n == 0) throw std::invalid_argument("obj_cnt is 0, cannot build a CompressedLeafBVH");
+        t.n_obj = n;
+        size_t scratch = std::max({bbox_scratch_bytes(n), radix_sort_scratch_bytes(n),
+                                   scan_scratch_bytes(n)});
+        t.scratch.resize(scratch);
+        t.bbox6.resize(6);
+        t.counters.resize(8);
+        for (auto *a : {&t.codes, &t.sorted, &t.sort_index, &t.keys_alt, &t.vals_alt, &t.split_a,
+                        &t.split_b})
+            a->resize(n);
+        t.reduc_map.resize((size_t) n + 2);
+        fill_u32(s, t.counters.data(), 0, 8);
+        bbox_nextafter(s, n, x, y, z, t.bbox6.data(), t.scratch.data());
+        morton_codes(s, n, x, y, z, t.bbox6.data(), t.codes.data());
+        SHAMX_HIP(hipMemcpyAsync(t.sorted.data(), t.codes.data(), n * sizeof(u32),
+                                 hipMemcpyDeviceToDevice, s));
+    }
+
+    u32 *radix_sort_error_word(void *scratch) { return (u32 *) scratch + 4 * RS_BINS + 4; }
+
+    void tree_stage_sort(hipStream_t s, TreeBuffers &t) {
+        radix_sort_pairs(s, t.n_obj, 30, t.sorted.data(), t.sort_index.data(),
+                         t.keys_alt.data(), t.vals_alt.data(), t.scratch.data());
+        // keep the sort's error word before the scratch is reused by the scans
+        SHAMX_HIP(hipMemcpyAsync(t.counters.data() + 2, radix_sort_error_word(t.scratch.data()),
+                                 sizeof(u32), hipMemcpyDeviceToDevice, s));
+    }
+
+    void tree_stage_finish(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
+                           const f64 *z, const f64 *h, u32 reduction_level, f64 htol) {
+        u32 n = t.n_obj;
+        // leaf reduction
+        u32 g = div_up(n, 256);
+        k_split<<<g, 256, 0, s>>>(n, t.sorted.data(), t.split_a.data());
+        u32 *sin = t.split_a.data(), *sout = t.split_b.data();
+        for (u32 it = 1; it <= reduction_level; it++) {
+            k_reduc_iter<<<g, 256, 0, s>>>(n, t.sorted.data(), sin, sout);
+            std::swap(sin, sout);
+        }
+        // sin holds the final split table; sout is free for the offsets
+        exclusive_scan_u32(s, sin, sout, n, t.counters.data(), t.scratch.data());
+        k_compact<<<g, 256, 0, s>>>(n, sin, sout, t.reduc_map.data());
+        u32 hc[3];
+        SHAMX_HIP(hipMemcpyAsync(hc, t.counters.data(), 3 * sizeof(u32), hipMemcpyDeviceToHost, s));
+        SHAMX_HIP(hipStreamSynchronize(s));
+        if (hc[2]) throw std::runtime_error("radix sort: look-back or scatter inconsistency");
+        u32 L = hc[0];
+        if (L == 0) throw std::runtime_error("0 leaf tree cannot exists");
+        t.L = L;
+        t.I = L - 1;
+        t.leaf_codes.resize(L);
+        k_leaf_codes<<<div_up(L, 256), 256, 0, s>>>(n, L, t.reduc_map.data(), t.sorted.data(),
+                                                   t.leaf_codes.data());
+        u32 I = t.I;
+        for (auto *a : {&t.lchild, &t.rchild, &t.endrange, &t.visits}) a->resize(std::max<u32>(I, 1));
+        t.lflag.resize(std::max<u32>(I, 1));
+        t.rflag.resize(std::max<u32>(I, 1));
+        t.parent.resize((size_t) I + L);
+        t.nodes.resize((size_t) (I + L) * NODE_F64);
+        fill_u32(s, t.parent.data(), NO_PARENT, (size_t) I + L);
+        if (I > 0) {
+            fill_u32(s, t.visits.data(), 0, I);
+            k_karras<<<div_up(I, 256), 256, 0, s>>>(L, t.leaf_codes.data(), t.lchild.data(),
+                                                   t.rchild.data(), t.lflag.data(),
+                                                   t.rflag.data(), t.endrange.data(),
+                                                   t.parent.data(), t.nodes.data());
+        } else {
+            fill_u32(s, t.parent.data(), NO_PARENT, 1);
+        }
+        k_leaf_aabb_climb<<<div_up(L, 128), 128, 0, s>>>(L, I, t.reduc_map.data(),
+                                                        t.sort_index.data(), x, y, z, h, htol,
+                                                        t.parent.data(), t.visits.data(),
+                                                        t.nodes.data());
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void tree_build(hipStream_t s, TreeBuffers &t, u32 n, const f64 *x, const f64 *y,
+                    const f64 *z, const f64 *h, u32 reduction_level, f64 htol) {
+        tree_stage_codes(s, t, n, x, y, z);
+        tree_stage_sort(s, t);
+        tree_stage_finish(s, t, x, y, z, h, reduction_level, htol);
+    }
+
+} // namespace shamx

@@ -1,0 +1,407 @@
+"""Python surface of the SPH model, mirroring the reference bindings for the hot path.
+
+Reference: shammodels/sph/src/pySPHModel.cpp (get_Model_SPH :1725-1754, SPHModel_f64_3_<K>
+:761-1510, SolverConfig setters :69-400, TimestepLog :1763-1770, EvolveUntilResults
+:1692-1705) and shampylib/src/pyShamrockCtx.cpp (Context, pdata_layout_new, collect_data
+:71-128).  Particles live on the host until the first step; from then on the fields are owned
+by the GPU solver (libshamx_sph.so) and copied back on demand.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import Callable
+
+import numpy as np
+
+from .setups import KERNEL_HFACT, KERNEL_RKERN, base_config, kernel_W, lattice_cubic
+from .solver import GpuSolver
+
+_VEC_FIELDS = {"xyz": ("x", "y", "z"), "vxyz": ("vx", "vy", "vz"), "axyz": ("ax", "ay", "az")}
+_SCAL_FIELDS = {"hpart": "h", "uint": "u", "duint": "du"}
+
+
+@dataclasses.dataclass
+class TimestepLog:
+    """Solver.hpp:43-54"""
+    rank: int = 0
+    rate: float = 0.0
+    npart: int = 0
+    tcompute: float = 0.0
+
+    def rate_sum(self):
+        return self.rate
+
+    def npart_sum(self):
+        return self.npart
+
+
+@dataclasses.dataclass
+class EvolveUntilResults:
+    """Solver.hpp:56-62"""
+    reach_target_time: bool = False
+    reach_niter_max: bool = False
+    reach_max_walltime: bool = False
+    iter_count: int = 0
+
+
+class Context:
+    """ShamrockCtx subset: one patch, one GPU."""
+
+    def __init__(self):
+        self.layout_ready = False
+        self.model = None
+
+    def pdata_layout_new(self):
+        self.layout_ready = True
+
+    def init_sched(self, split_crit=0, merge_crit=0):
+        self.sched_ready = True
+
+    def collect_data(self) -> dict:
+        """pyShamrockCtx.cpp:97-128: dict of numpy arrays keyed by field name."""
+        if self.model is None:
+            return {}
+        return self.model.collect_data()
+
+
+class SolverConfig:
+    """Hot-path subset of SolverConfig (SolverConfig.hpp:432-1400)."""
+
+    def __init__(self, kernel: str):
+        self.kernel = kernel
+        self.d = base_config(kernel, (0, 0, 0), (1, 1, 1))
+        self.two_stage_search = True
+
+    # artificial viscosity (AVConfig.hpp:49-55)
+    def set_artif_viscosity_Constant(self, alpha_u, alpha_AV, beta_AV):
+        self.d.update(alpha_u=float(alpha_u), alpha_AV=float(alpha_AV), beta_AV=float(beta_AV))
+
+    def set_boundary_periodic(self):
+        self.d["periodic"] = 1
+
+    def set_boundary_free(self):
+        self.d["periodic"] = 0
+
+    def set_eos_adiabatic(self, gamma):
+        self.d.update(eos_kind=0, gamma=float(gamma))
+
+    def set_eos_isothermal(self, cs):
+        self.d.update(eos_kind=1, eos_cs=float(cs))
+
+    def set_eos_locally_isothermalLP07(self, cs0, q, r0):
+        self.d.update(eos_kind=2, eos_cs0=float(cs0), eos_q=float(q), eos_r0=float(r0))
+
+    def set_tree_reduction_level(self, level):
+        self.d["tree_reduction_level"] = int(level)
+
+    def set_two_stage_search(self, enable):
+        if not enable:
+            raise NotImplementedError("only the two-stage neighbour search is implemented")
+        self.two_stage_search = True
+
+    def set_smoothing_length_density_based(self):
+        pass
+
+    def set_cfl_cour(self, v):
+        self.d["cfl_cour"] = float(v)
+
+    def set_cfl_force(self, v):
+        self.d["cfl_force"] = float(v)
+
+    def set_cfl_mult_stiffness(self, v):
+        self.d["cfl_multiplier_stiffness"] = float(v)
+
+    def set_particle_mass(self, m):
+        self.d["gpart_mass"] = float(m)
+
+    def set_mixed_precision(self, enable: bool):
+        """Build extension: FP32 pair sums with Kahan compensation on FP64 r_ab."""
+        self.d["mixed_precision"] = int(bool(enable))
+
+    def print_status(self):
+        print(self.d)
+
+
+class _Setup:
+    """SPHSetup subset (SPHSetup.cpp:111, GeneratorLatticeCubic)."""
+
+    def __init__(self, model):
+        self.model = model
+
+    def make_generator_lattice_cubic(self, dr, box_min, box_max):
+        return ("lattice_cubic", float(dr), tuple(box_min), tuple(box_max))
+
+    def apply_setup(self, gen, **kw):
+        kind, dr, bmin, bmax = gen
+        pos = lattice_cubic(dr, bmin, bmax)
+        self.model.push_particle(pos, np.full(pos.shape[0], dr), np.zeros(pos.shape[0]))
+
+
+class SPHModel:
+    """SPHModel_f64_3_<K> (pySPHModel.cpp:761-1510), hot-path subset."""
+
+    def __init__(self, context: Context, kernel: str):
+        self.ctx = context
+        self.kernel = kernel
+        self.name = f"SPHModel_f64_3_{kernel}"
+        self.cfg = SolverConfig(kernel)
+        self.host = {k: np.zeros(0) for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az",
+                                              "h", "u", "du")}
+        self.gpu: GpuSolver | None = None
+        self.t, self.dt, self.cfl_mult = 0.0, 0.0, 1e-2
+        self.logs: list[dict] = []
+        self.step_count = 0
+        self.callbacks: list[tuple[Callable | None, Callable | None]] = []
+        context.model = self
+
+    # --- configuration -------------------------------------------------------------------
+    def gen_default_config(self) -> SolverConfig:
+        return SolverConfig(self.kernel)
+
+    def set_solver_config(self, cfg: SolverConfig):
+        self.cfg = cfg
+        self._invalidate()
+
+    def get_current_config(self) -> SolverConfig:
+        return self.cfg
+
+    def init_scheduler(self, split_crit=0, merge_crit=0):
+        pass
+
+    def init(self):
+        pass
+
+    def resize_simulation_box(self, box_min, box_max):
+        self.cfg.d["box_min"] = [float(v) for v in box_min]
+        self.cfg.d["box_max"] = [float(v) for v in box_max]
+        self._invalidate()
+
+    def get_setup(self):
+        return _Setup(self)
+
+    # --- particles -----------------------------------------------------------------------
+    def _sync_from_gpu(self):
+        if self.gpu is not None:
+            for k in self.host:
+                self.host[k] = self.gpu.get(k)
+            tm = self.gpu.time()
+            self.t, self.dt, self.cfl_mult = tm["t"], tm["dt"], tm["cfl_mult"]
+
+    def _invalidate(self):
+        self._sync_from_gpu()
+        if self.gpu is not None:
+            self.gpu.close()
+        self.gpu = None
+
+    def push_particle(self, pos, hpart, upart):
+        self._invalidate()
+        pos = np.asarray(pos, float).reshape(-1, 3)
+        n = pos.shape[0]
+        add = dict(x=pos[:, 0], y=pos[:, 1], z=pos[:, 2], h=np.asarray(hpart, float),
+                   u=np.asarray(upart, float))
+        for k in self.host:
+            self.host[k] = np.concatenate([self.host[k], add.get(k, np.zeros(n))])
+
+    def get_total_part_count(self):
+        return int(self.host["x"].size) if self.gpu is None else self.gpu.n
+
+    def _field_get(self, name):
+        self._sync_from_gpu()
+        if name in _VEC_FIELDS:
+            return np.stack([self.host[k] for k in _VEC_FIELDS[name]], axis=1)
+        return self.host[_SCAL_FIELDS[name]].copy()
+
+    def _field_set(self, name, val):
+        self._invalidate()
+        if name in _VEC_FIELDS:
+            val = np.asarray(val, float).reshape(-1, 3)
+            for i, k in enumerate(_VEC_FIELDS[name]):
+                self.host[k] = val[:, i].copy()
+        else:
+            self.host[_SCAL_FIELDS[name]] = np.asarray(val, float).copy()
+
+    def set_value_in_a_box(self, field_name, field_type, val, box_min, box_max):
+        pos = self._field_get("xyz")
+        bmin, bmax = np.asarray(box_min, float), np.asarray(box_max, float)
+        inside = np.all((pos >= bmin) & (pos < bmax), axis=1)
+        f = self._field_get(field_name)
+        f[inside] = val
+        self._field_set(field_name, f)
+
+    def set_field_value_lambda_f64(self, field_name, pos_to_val, offset=0):
+        pos = self._field_get("xyz")
+        f = self._field_get(field_name)
+        f[offset:] = [pos_to_val(tuple(p)) for p in pos[offset:]]
+        self._field_set(field_name, f)
+
+    def set_field_value_lambda_f64_3(self, field_name, pos_to_val, offset=0):
+        pos = self._field_get("xyz")
+        f = self._field_get(field_name)
+        f[offset:] = np.array([pos_to_val(tuple(p)) for p in pos[offset:]], float).reshape(-1, 3)
+        self._field_set(field_name, f)
+
+    def add_kernel_value(self, field_name, field_type, val, center, h_ker):
+        """Model.hpp:739-768: f += val * W_3d(|r - center|, h_ker)."""
+        pos = self._field_get("xyz")
+        r = np.sqrt(((pos - np.asarray(center, float)) ** 2).sum(axis=1))
+        f = self._field_get(field_name)
+        f = f + val * kernel_W(self.kernel, r, h_ker)
+        self._field_set(field_name, f)
+
+    def get_sum(self, field_name, field_type):
+        f = self._field_get(field_name)
+        return f.sum(axis=0)
+
+    def collect_data(self) -> dict:
+        return {k: self._field_get(k) for k in ("xyz", "vxyz", "axyz", "hpart", "uint", "duint")}
+
+    # --- mass / density ------------------------------------------------------------------
+    def set_particle_mass(self, m):
+        self.cfg.set_particle_mass(m)
+        self._invalidate()
+
+    def get_particle_mass(self):
+        return self.cfg.d["gpart_mass"]
+
+    def total_mass_to_part_mass(self, total_mass):
+        return total_mass / self.get_total_part_count()
+
+    def get_hfact(self):
+        return KERNEL_HFACT[self.kernel]
+
+    def rho_h(self, h):
+        """density.hpp:22-25"""
+        hf = self.get_hfact()
+        m = self.get_particle_mass()
+        return m * (hf / h) * (hf / h) * (hf / h)
+
+    # --- time ----------------------------------------------------------------------------
+    def _time(self):
+        if self.gpu is not None:
+            tm = self.gpu.time()
+            self.t, self.dt, self.cfl_mult = tm["t"], tm["dt"], tm["cfl_mult"]
+
+    def get_time(self):
+        self._time()
+        return self.t
+
+    def get_dt(self):
+        self._time()
+        return self.dt
+
+    def set_time(self, t):
+        self._time()
+        self.t = float(t)
+        if self.gpu is not None:
+            self.gpu.set_time(t=self.t)
+
+    def set_next_dt(self, dt):
+        self._time()
+        self.dt = float(dt)
+        if self.gpu is not None:
+            self.gpu.set_time(dt=self.dt)
+
+    def set_cfl_multipler(self, m):
+        self._time()
+        self.cfl_mult = float(m)
+        if self.gpu is not None:
+            self.gpu.set_time(cfl_mult=self.cfl_mult)
+
+    def set_cfl_cour(self, v):
+        self.cfg.set_cfl_cour(v)
+        self._invalidate()
+
+    def set_cfl_force(self, v):
+        self.cfg.set_cfl_force(v)
+        self._invalidate()
+
+    def change_htolerances(self, coarse, fine):
+        self.cfg.d.update(htol_up_coarse=float(coarse), htol_up_fine=float(fine))
+        self._invalidate()
+
+    # --- stepping ------------------------------------------------------------------------
+    def _ensure_gpu(self):
+        if self.gpu is None:
+            n = self.host["x"].size
+            if n == 0:
+                raise RuntimeError("no particles: add particles before stepping")
+            if self.cfg.d["gpart_mass"] == 0:
+                raise RuntimeError("invalid gpart_mass 0, this configuration can not converge.\n"
+                                   "Please set it using either model.set_particle_mass(pmass) "
+                                   "or cfg.set_particle_mass(pmass)")
+            self.gpu = GpuSolver(self.cfg.d, self.host)
+            self.gpu.set_time(self.t, self.dt, self.cfl_mult)
+        return self.gpu
+
+    def add_timestep_callback(self, step_begin=None, step_end=None):
+        self.callbacks.append((step_begin, step_end))
+
+    def evolve_once(self):
+        """Solver::evolve_once (Solver.cpp:1942-3272)"""
+        g = self._ensure_gpu()
+        for b, _ in self.callbacks:
+            if b:
+                b()
+        log = g.evolve_once()
+        for _, e in reversed(self.callbacks):
+            if e:
+                e()
+        self.logs.append(log)
+        self.step_count += 1
+        self._time()
+        return log
+
+    def evolve_once_override_time(self, t_curr, dt_input):
+        """Solver::evolve_once_time_expl (Solver.hpp:298-303)"""
+        self.set_time(t_curr)
+        self.set_next_dt(dt_input)
+        self.evolve_once()
+        return self.get_dt()
+
+    def timestep(self) -> TimestepLog:
+        log = self.evolve_once()
+        return TimestepLog(rank=0, rate=log["rate"], npart=log["n_local"],
+                           tcompute=log["t_total"])
+
+    def evolve_until(self, target_time, niter_max=-1, max_walltime=-1):
+        """Solver::evolve_until (Solver.hpp:305-433)"""
+        start = time.time()
+        it = 0
+        while self.get_time() < target_time:
+            t, dt = self.get_time(), self.get_dt()
+            if t > target_time:
+                raise ValueError("the target time is higher than the current time")
+            if t + dt > target_time:
+                self.set_next_dt(target_time - t)
+            self.evolve_once()
+            it += 1
+            if niter_max >= 0 and it >= niter_max:
+                return EvolveUntilResults(False, True, False, it)
+            if max_walltime >= 0 and time.time() - start >= max_walltime:
+                return EvolveUntilResults(False, False, True, it)
+        return EvolveUntilResults(True, False, False, it)
+
+    # --- logs ----------------------------------------------------------------------------
+    def solver_logs_last_rate(self):
+        return self.logs[-1]["rate"] if self.logs else math.nan
+
+    def solver_logs_last_obj_count(self):
+        return self.logs[-1]["n_local"] if self.logs else 0
+
+    def solver_logs_step_count(self):
+        return self.step_count
+
+    def solver_logs_reset_step_count(self):
+        self.step_count = 0
+
+
+_KERNELS = ("M4", "M6", "M8", "C2", "C4", "C6")
+
+
+def get_Model_SPH(*, context: Context, vector_type: str, sph_kernel: str) -> SPHModel:
+    """pySPHModel.cpp:1725-1754"""
+    if vector_type != "f64_3" or sph_kernel not in _KERNELS:
+        raise ValueError("unknown combination of representation and kernel")
+    return SPHModel(context, sph_kernel)

@@ -1,0 +1,136 @@
+"""Python handle on the C ABI Solver (shamx_solver_*): one patch, fields on the GPU.
+
+Mirrors shammodels::sph::Solver<f64_3, Kern> (Solver.hpp:70-434) for the hot path: field
+set/get, the time-state edges (t, dt, cfl multiplier) and evolve_once.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du"]
+
+
+def make_config(cfg: dict) -> _lib.ShamxConfig:
+    c = _lib.ShamxConfig()
+    kernel = cfg["kernel"]
+    c.kernel = _lib.KERNEL_IDS[kernel] if isinstance(kernel, str) else int(kernel)
+    for k in ("periodic", "eos_kind", "tree_reduction_level", "h_iter_per_subcycles",
+              "h_max_subcycles", "mixed_precision"):
+        setattr(c, k, int(cfg[k]))
+    for i in range(3):
+        c.box_min[i] = float(cfg["box_min"][i])
+        c.box_max[i] = float(cfg["box_max"][i])
+    for k in ("gpart_mass", "gamma", "eos_cs", "eos_cs0", "eos_q", "eos_r0", "alpha_u",
+              "alpha_AV", "beta_AV", "htol_up_coarse", "htol_up_fine", "epsilon_h", "cfl_cour",
+              "cfl_force", "cfl_multiplier_stiffness"):
+        setattr(c, k, float(cfg[k]))
+    return c
+
+
+class GpuSolver:
+    """Owns a shamx_solver; fields are kept in HBM between steps."""
+
+    def __init__(self, cfg: dict, fields: dict):
+        self.cfg = dict(cfg)
+        self.n = int(np.asarray(fields["x"]).size)
+        self._c = make_config(cfg)
+        h = C.c_void_p()
+        check(lib().shamx_solver_create(C.byref(self._c), self.n, C.byref(h)))
+        self.h = h
+        for k, v in fields.items():
+            self.set(k, v)
+
+    def set(self, name, v):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (self.n,)))
+        check(lib().shamx_solver_set_field(self.h, FIELDS.index(name), a.ctypes.data, 0))
+
+    def set_device(self, name, tensor):
+        check(lib().shamx_solver_set_field(self.h, FIELDS.index(name), tensor.data_ptr(), 1))
+
+    def get(self, name):
+        a = np.zeros(self.n)
+        check(lib().shamx_solver_get_field(self.h, FIELDS.index(name), a.ctypes.data, 0))
+        return a
+
+    def time(self):
+        a = np.zeros(3)
+        check(lib().shamx_solver_get_time(self.h, _lib.dptr(a)))
+        return dict(t=a[0], dt=a[1], cfl_mult=a[2])
+
+    def set_time(self, t=None, dt=None, cfl_mult=None):
+        cur = self.time()
+        a = np.array([cur["t"] if t is None else t, cur["dt"] if dt is None else dt,
+                      cur["cfl_mult"] if cfl_mult is None else cfl_mult])
+        check(lib().shamx_solver_set_time(self.h, _lib.dptr(a)))
+
+    def evolve_once(self) -> dict:
+        log = _lib.ShamxStepLog()
+        check(lib().shamx_solver_evolve_once(self.h, C.byref(log)))
+        return log.as_dict()
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(lib().shamx_solver_stream(self.h, C.byref(s)))
+        return s.value or 0
+
+    def sizes(self) -> dict:
+        s = np.zeros(6, np.uint64)
+        check(lib().shamx_solver_sizes(self.h, s.ctypes.data))
+        return dict(Nm=int(s[0]), G=int(s[1]), L=int(s[2]), I=int(s[3]), P=int(s[4]),
+                    PL=int(s[5]))
+
+    def get_f64(self, what, n):
+        out = np.zeros(max(n, 1))
+        check(lib().shamx_solver_get_f64(self.h, what, out.ctypes.data))
+        return out[:n]
+
+    def get_u32(self, what, n):
+        out = np.zeros(max(n, 1), np.uint32)
+        check(lib().shamx_solver_get_u32(self.h, what, out.ctypes.data))
+        return out[:n]
+
+    def get_u8(self, what, n):
+        out = np.zeros(max(n, 1), np.uint8)
+        check(lib().shamx_solver_get_u8(self.h, what, out.ctypes.data))
+        return out[:n]
+
+    def transient(self) -> dict:
+        """Intermediate products of the last step (same keys as the oracle's)."""
+        s = self.sizes()
+        Nm, G, L, I, P, PL = s["Nm"], s["G"], s["L"], s["I"], s["P"], s["PL"]
+        N = self.n
+        d = dict(sizes=s)
+        for i, k in enumerate(["mx", "my", "mz", "mh", "mvx", "mvy", "mvz", "mu", "momega", "P",
+                               "cs"]):
+            d[k] = self.get_f64(i, Nm)
+        d["omega"] = self.get_f64(11, N)
+        d["vsig"] = self.get_f64(12, N)
+        d["tree_bbox"] = self.get_f64(13, 6)
+        d["rint_tree"] = self.get_f64(14, I + L)
+        d["aabb_min"] = self.get_f64(15, 3 * (I + L)).reshape(-1, 3)
+        d["aabb_max"] = self.get_f64(16, 3 * (I + L)).reshape(-1, 3)
+        for i, k, n in [(0, "ghost_src", G), (1, "sorted_codes", Nm), (2, "sort_index", Nm),
+                        (3, "reduc_map", L + 2), (4, "owner", N), (5, "part_cnt", N),
+                        (6, "part_offset", N), (7, "part_ids", P), (8, "leaf_cnt", L),
+                        (9, "leaf_offset", L), (10, "leaf_ids", PL), (11, "lchild", I),
+                        (12, "rchild", I), (13, "endrange", I), (14, "codes", Nm)]:
+            d[k] = self.get_u32(i, n)
+        d["lflag"] = self.get_u8(0, I)
+        d["rflag"] = self.get_u8(1, I)
+        return d
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().shamx_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
