@@ -101,10 +101,23 @@ SIGNATURES = {
 _lib = None
 
 
+def _init_torch_runtime_first():
+    """PyTorch ships its own HIP runtime (libamdhip64.so) next to the system one this library
+    links (libamdhip64.so.7).  Both can live in one process and share the device address
+    space, but torch's must initialise first; initialise it before loading this library when
+    torch is importable (a process without torch just uses the system runtime)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def lib():
     """Load libshamx_sph.so; raises if it is missing (no CPU fallback exists)."""
     global _lib
     if _lib is None:
+        _init_torch_runtime_first()
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with shamrock_amd/build.py "
                               "(python -c 'import __graft_entry__ as g; g.build()')")

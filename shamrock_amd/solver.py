@@ -80,7 +80,7 @@ class GpuSolver:
 
     def sizes(self) -> dict:
         s = np.zeros(6, np.uint64)
-        check(lib().shamx_solver_sizes(self.h, s.ctypes.data))
+        check(lib().shamx_solver_sizes(self.h, s.ctypes.data_as(C.POINTER(C.c_uint64))))
         return dict(Nm=int(s[0]), G=int(s[1]), L=int(s[2]), I=int(s[3]), P=int(s[4]),
                     PL=int(s[5]))
 

@@ -253,8 +253,13 @@ def _compare_transient(g, o, exact_sort=True):
             np.testing.assert_array_equal(g[k], o[k], err_msg=k)
     for k in ("mx", "my", "mz"):
         np.testing.assert_array_equal(g[k], o[k], err_msg=k)
-    for k in ("mh", "omega", "P", "cs", "momega", "mu", "mvx", "mvy", "mvz", "vsig"):
+    for k in ("mh", "omega", "P", "cs", "momega", "vsig"):
         assert relerr(g[k], o[k]) <= RTOL, (k, relerr(g[k], o[k]))
+    # the GPU keeps v and u of the local particles in place (the corrector has updated them
+    # since the ghost refresh), the oracle keeps a pre-corrector copy: compare the images only
+    n = g["omega"].size
+    for k in ("mu", "mvx", "mvy", "mvz"):
+        assert relerr(g[k][n:], o[k][n:]) <= RTOL, (k, relerr(g[k][n:], o[k][n:]))
 
 
 @pytest.mark.parametrize("workload,steps", [("sound_wave_16", 4), ("sedov_20", 3),
