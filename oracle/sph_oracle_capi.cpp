@@ -270,6 +270,52 @@ int orc_tree_box_query(void *h, uint32_t n, const double *x, const double *y, co
     });
 }
 
+// ---- neighbour cache on arbitrary positions ----------------------------------------------
+// builds the u32 CLBVH on all n objects (bbox widened by nextafter), the rint field
+// (max h * htol) and the two-stage cache of the first n_local objects; out_sizes =
+// {L, I, P, PL}; the handle keeps the result for orc_neigh_get
+struct NeighHandle {
+    CLBVH<u32> t;
+    std::vector<double> rint;
+    NeighOut o;
+};
+} // extern "C"
+extern "C" {
+void *orc_neigh_build(uint32_t n, const double *x, const double *y, const double *z,
+                      const double *h, uint32_t n_local, uint32_t level, double htol,
+                      double Rkern, uint64_t *out_sizes) {
+    auto *nh = new NeighHandle();
+    int rc   = guard([&] {
+        vec3 lo, hi;
+        bbox_nextafter(x, y, z, n, lo, hi);
+        nh->t.build_no_aabb(x, y, z, n, lo, hi, level);
+        nh->t.compute_aabb(x, y, z);
+        nh->rint = nh->t.field_max(h);
+        for (auto &v : nh->rint) v *= htol;
+        nh->o = neigh_cache_2stages(nh->t, nh->rint, x, y, z, h, n_local, htol, Rkern);
+        out_sizes[0] = nh->t.L;
+        out_sizes[1] = nh->t.I;
+        out_sizes[2] = nh->o.part_cache.ids.size();
+        out_sizes[3] = nh->o.leaf_cache.ids.size();
+    });
+    if (rc) {
+        delete nh;
+        return nullptr;
+    }
+    return nh;
+}
+void orc_neigh_get(void *hnd, uint32_t *cnt, uint32_t *offset, uint32_t *ids, uint32_t *owner,
+                   uint32_t *leaf_cnt, uint32_t *leaf_ids) {
+    auto *nh = (NeighHandle *) hnd;
+    std::copy(nh->o.part_cache.cnt.begin(), nh->o.part_cache.cnt.end(), cnt);
+    std::copy(nh->o.part_cache.offset.begin(), nh->o.part_cache.offset.end(), offset);
+    std::copy(nh->o.part_cache.ids.begin(), nh->o.part_cache.ids.end(), ids);
+    std::copy(nh->o.owner.begin(), nh->o.owner.end(), owner);
+    std::copy(nh->o.leaf_cache.cnt.begin(), nh->o.leaf_cache.cnt.end(), leaf_cnt);
+    std::copy(nh->o.leaf_cache.ids.begin(), nh->o.leaf_cache.ids.end(), leaf_ids);
+}
+void orc_neigh_free(void *hnd) { delete (NeighHandle *) hnd; }
+
 // ---- h iteration on an explicit cache ----------------------------------------------------
 int orc_iterate_h(int kernel, uint32_t n, const uint32_t *cnt, const uint32_t *offset,
                   const uint32_t *ids, uint32_t nids, const double *x, const double *y,

@@ -72,6 +72,10 @@ namespace shamx {
         DevArray<u32> leaf_cnt, leaf_offset, leaf_ids; // stage 1 (per leaf)
         DevArray<u32> owner;                            // per local particle
         DevArray<u32> cnt, offset, ids;                 // stage 2 CSR (per local particle)
+        DevArray<double4> ps;                           // (x, y, z, h) in sorted order
+        DevArray<u32> inv;                              // sorted position of each object
+        DevArray<u32> mism;                             // owner leaf != own leaf
+        u32 n_mismatch = 0;
         DevArray<u32> counters;
         DevArray<unsigned char> scratch;
     };

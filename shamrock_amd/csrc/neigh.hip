@@ -109,7 +109,9 @@ namespace shamx {
 
         // stage 2 count / fill
         template<bool FILL>
-        __global__ void k_part_neigh(u32 n, u32 I, const f64 *__restrict nodes,
+        __global__ void k_part_neigh(u32 n, const u32 *__restrict list,
+                                     const u32 *__restrict list_len, u32 I,
+                                     const f64 *__restrict nodes,
                                      const u32 *__restrict sort_index,
                                      const u32 *__restrict owner,
                                      const u32 *__restrict leaf_cnt,
@@ -119,8 +121,9 @@ namespace shamx {
                                      const f64 *__restrict h, f64 htol, f64 Rker2,
                                      u32 *__restrict cnt, const u32 *__restrict offset,
                                      u32 *__restrict ids) {
-            u32 a = blockIdx.x * blockDim.x + threadIdx.x;
-            if (a >= n) return;
+            u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+            if (t >= (list ? *list_len : n)) return;
+            u32 a      = list ? list[t] : t;
             f64 rint_a = h[a] * htol;
             f64 xa = x[a], ya = y[a], za = z[a];
             f64 ca = rint_a * rint_a * Rker2;
@@ -143,6 +146,139 @@ namespace shamx {
                 }
             }
             if (!FILL) cnt[a] = c;
+        }
+
+        // sorted copy of positions and h (AoS, 32 B per object) + inverse permutation
+        __global__ void k_sorted_copy(u32 nm, const u32 *__restrict sort_index,
+                                      const f64 *__restrict x, const f64 *__restrict y,
+                                      const f64 *__restrict z, const f64 *__restrict h,
+                                      double4 *__restrict ps, u32 *__restrict inv) {
+            u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+            if (s >= nm) return;
+            u32 b  = sort_index[s];
+            ps[s]  = make_double4(x[b], y[b], z[b], h[b]);
+            inv[b] = s;
+        }
+
+        // particles whose owner leaf is not the leaf holding them go to the per-particle path
+        __global__ void k_mismatch(u32 n, u32 I, const f64 *__restrict nodes,
+                                   const u32 *__restrict owner, const u32 *__restrict inv,
+                                   u32 *__restrict list, u32 *__restrict list_len) {
+            u32 a = blockIdx.x * blockDim.x + threadIdx.x;
+            if (a >= n) return;
+            u32 s0, s1;
+            node_children(nodes, I + owner[a], s0, s1);
+            u32 sa = inv[a];
+            if (!(s0 <= sa && sa < s1)) list[atomicAdd(list_len, 1u)] = a;
+        }
+
+        __device__ inline f64 wave_bcast(f64 v, int lane) {
+            i64 b  = __double_as_longlong(v);
+            int lo = __builtin_amdgcn_readlane((int) (b & 0xffffffff), lane);
+            int hi = __builtin_amdgcn_readlane((int) (b >> 32), lane);
+            return __longlong_as_double(((i64) hi << 32) | (u32) lo);
+        }
+
+        constexpr int GRP_WAVES = 4;
+        constexpr int GRP_CAP   = 512; // candidate window per wave (LDS)
+
+        // stage 2, one wave per leaf: the members are the leaf's local particles whose owner
+        // is this leaf; the candidates (particles of the neighbour leaves, in list order then
+        // sorted order) are streamed through an LDS window 64 at a time, each lane holding one
+        // candidate; for each member the keep-mask is a ballot, so the compacted output keeps
+        // candidate order.
+        template<bool FILL>
+        __global__ void __launch_bounds__(GRP_WAVES * WAVE)
+            k_group_neigh(u32 n, u32 L, u32 I, const f64 *__restrict nodes,
+                          const u32 *__restrict sort_index, const double4 *__restrict ps,
+                          const u32 *__restrict owner, const u32 *__restrict leaf_cnt,
+                          const u32 *__restrict leaf_offset, const u32 *__restrict leaf_ids,
+                          f64 htol, f64 Rker2, u32 *__restrict cnt, const u32 *__restrict offset,
+                          u32 *__restrict ids) {
+            __shared__ u32 win[GRP_WAVES][GRP_CAP];
+            const int lane = __lane_id(), wid = threadIdx.x / WAVE;
+            const u32 l    = blockIdx.x * GRP_WAVES + wid;
+            if (l >= L) return;
+            u32 *w = win[wid];
+            u32 m0, m1;
+            node_children(nodes, I + l, m0, m1);
+            const u64 lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            const u32 k0 = leaf_offset[l], k1 = k0 + leaf_cnt[l];
+            for (u32 mb = m0; mb < m1; mb += WAVE) { // member batches of 64
+                u32 sm     = mb + lane;
+                bool mem   = false;
+                u32 a      = 0;
+                f64 xa = 0, ya = 0, za = 0, ca = 0;
+                if (sm < m1) {
+                    a = sort_index[sm];
+                    if (a < n && owner[a] == l) {
+                        mem       = true;
+                        double4 p = ps[sm];
+                        xa = p.x;
+                        ya = p.y;
+                        za = p.z;
+                        f64 ra = p.w * htol;
+                        ca     = ra * ra * Rker2;
+                    }
+                }
+                u64 members = __ballot(mem);
+                if (!members) continue;
+                u32 cursor = (FILL && mem) ? offset[a] : 0u;
+                u32 count  = 0;
+                u32 k = k0, part = 0;
+                while (k < k1) {
+                    // fill the candidate window with whole (or split) leaf ranges
+                    u32 wc = 0;
+                    while (k < k1 && wc < (u32) GRP_CAP) {
+                        u32 r0, r1;
+                        node_children(nodes, leaf_ids[k], r0, r1);
+                        r0 += part;
+                        u32 take = min(r1 - r0, (u32) GRP_CAP - wc);
+                        for (u32 j = lane; j < take; j += WAVE) w[wc + j] = r0 + j;
+                        wc += take;
+                        if (r0 + take == r1) {
+                            k++;
+                            part = 0;
+                        } else {
+                            part += take;
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    for (u32 c0 = 0; c0 < wc; c0 += WAVE) {
+                        u32 c      = c0 + lane;
+                        bool valid = c < wc;
+                        u32 sb     = valid ? w[c] : 0u;
+                        double4 q  = ps[valid ? sb : 0u];
+                        f64 rb     = q.w * htol;
+                        f64 cb     = rb * rb * Rker2;
+                        u32 b      = FILL && valid ? sort_index[sb] : 0u;
+                        u64 mm     = members;
+                        while (mm) {
+                            int m  = __builtin_ctzll(mm);
+                            mm &= mm - 1;
+                            f64 dx = wave_bcast(xa, m) - q.x;
+                            f64 dy = wave_bcast(ya, m) - q.y;
+                            f64 dz = wave_bcast(za, m) - q.z;
+                            f64 rab2 = dx * dx + dy * dy + dz * dz;
+                            bool keep = valid && !(rab2 > wave_bcast(ca, m) && rab2 > cb);
+                            u64 km    = __ballot(keep);
+                            u32 kc    = (u32) __popcll(km);
+                            if (FILL) {
+                                u32 base = (u32) __builtin_amdgcn_readlane((int) cursor, m);
+                                if (keep) ids[base + (u32) __popcll(km & lt)] = b;
+                            }
+                            if (lane == m) {
+                                cursor += kc;
+                                count += kc;
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (!FILL && mem) cnt[a] = count;
+            }
         }
     } // namespace
 
@@ -183,12 +319,28 @@ namespace shamx {
         // owner leaf
         k_owner<<<div_up(n_local, 128), 128, 0, s>>>(n_local, I, t.nodes.data(), x, y, z, L,
                                                      nb.owner.data(), err);
-        // stage 2
+        // sorted copy, inverse permutation, owner != own-leaf list
+        u32 nm = t.n_obj;
+        nb.ps.resize(nm);
+        nb.inv.resize(nm);
+        nb.mism.resize(std::max<u32>(n_local, 1));
+        k_sorted_copy<<<div_up(nm, 256), 256, 0, s>>>(nm, t.sort_index.data(), x, y, z, h,
+                                                      nb.ps.data(), nb.inv.data());
+        u32 *mism_len = nb.counters.data() + 5;
+        k_mismatch<<<div_up(n_local, 256), 256, 0, s>>>(n_local, I, t.nodes.data(),
+                                                        nb.owner.data(), nb.inv.data(),
+                                                        nb.mism.data(), mism_len);
+        // stage 2 count
         f64 Rker2 = Rkern * Rkern;
+        u32 gblocks = div_up(L, GRP_WAVES);
+        k_group_neigh<false><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+            n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(), nb.owner.data(),
+            nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), htol, Rker2,
+            nb.cnt.data(), nullptr, nullptr);
         k_part_neigh<false><<<div_up(n_local, 128), 128, 0, s>>>(
-            n_local, I, t.nodes.data(), t.sort_index.data(), nb.owner.data(), nb.leaf_cnt.data(),
-            nb.leaf_offset.data(), nb.leaf_ids.data(), x, y, z, h, htol, Rker2, nb.cnt.data(),
-            nullptr, nullptr);
+            n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
+            nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x, y,
+            z, h, htol, Rker2, nb.cnt.data(), nullptr, nullptr);
         exclusive_scan_u32(s, nb.cnt.data(), nb.offset.data(), n_local, nb.counters.data() + 1,
                            nb.scratch.data());
         SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32), hipMemcpyDeviceToHost, s));
@@ -196,12 +348,17 @@ namespace shamx {
         if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
         if (hc[4] & 2u)
             throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
-        nb.n_pairs = hc[1];
+        nb.n_pairs    = hc[1];
+        nb.n_mismatch = hc[5];
         nb.ids.resize(std::max<u64>(nb.n_pairs, 1));
-        k_part_neigh<true><<<div_up(n_local, 128), 128, 0, s>>>(
-            n_local, I, t.nodes.data(), t.sort_index.data(), nb.owner.data(), nb.leaf_cnt.data(),
-            nb.leaf_offset.data(), nb.leaf_ids.data(), x, y, z, h, htol, Rker2, nullptr,
+        k_group_neigh<true><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+            n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(), nb.owner.data(),
+            nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), htol, Rker2, nullptr,
             nb.offset.data(), nb.ids.data());
+        k_part_neigh<true><<<div_up(n_local, 128), 128, 0, s>>>(
+            n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
+            nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x, y,
+            z, h, htol, Rker2, nullptr, nb.offset.data(), nb.ids.data());
         SHAMX_HIP(hipGetLastError());
     }
 

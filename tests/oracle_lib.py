@@ -95,6 +95,11 @@ def lib():
                                     C.c_double]
         L.orc_periodic_ghosts.argtypes = [C.c_uint32, _f64p, _f64p, _f64p, _f64p, C.c_double,
                                           C.c_uint32, _u32p, _f64p, C.POINTER(C.c_uint32)]
+        L.orc_neigh_build.restype = C.c_void_p
+        L.orc_neigh_build.argtypes = [C.c_uint32, _f64p, _f64p, _f64p, _f64p, C.c_uint32,
+                                      C.c_uint32, C.c_double, C.c_double, _u64p]
+        L.orc_neigh_get.argtypes = [C.c_void_p, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p]
+        L.orc_neigh_free.argtypes = [C.c_void_p]
         L.orc_solver_new.restype = C.c_void_p
         L.orc_solver_new.argtypes = [C.POINTER(OrcConfig), C.c_uint32]
         L.orc_solver_free.argtypes = [C.c_void_p]
@@ -260,6 +265,28 @@ def periodic_ghosts(pos, box6, int_range):
                                    C.byref(cnt)))
     n = cnt.value
     return src[:n], off[: 3 * n].reshape(-1, 3)
+
+
+def neigh_cache(pos, h, n_local, level=3, htol=1.1, Rkern=2.0):
+    """Two-stage neighbour cache (NeighbourCache.cpp:222-604) on arbitrary positions."""
+    x, y, z = soa(pos)
+    h = _f(h)
+    sz = np.zeros(4, np.uint64)
+    hd = lib().orc_neigh_build(x.size, x, y, z, h, n_local, level, htol, Rkern, sz)
+    if not hd:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+    L, I, P, PL = (int(v) for v in sz)
+    out = dict(cnt=np.zeros(max(n_local, 1), np.uint32), offset=np.zeros(max(n_local, 1), np.uint32),
+               ids=np.zeros(max(P, 1), np.uint32), owner=np.zeros(max(n_local, 1), np.uint32),
+               leaf_cnt=np.zeros(max(L, 1), np.uint32), leaf_ids=np.zeros(max(PL, 1), np.uint32))
+    lib().orc_neigh_get(hd, out["cnt"], out["offset"], out["ids"], out["owner"], out["leaf_cnt"],
+                        out["leaf_ids"])
+    lib().orc_neigh_free(hd)
+    for k, n in (("cnt", n_local), ("offset", n_local), ("owner", n_local), ("ids", P),
+                 ("leaf_cnt", L), ("leaf_ids", PL)):
+        out[k] = out[k][:n]
+    out.update(L=L, I=I, P=P, PL=PL)
+    return out
 
 
 # ---------------------------------------------------------------- full solver

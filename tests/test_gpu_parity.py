@@ -189,6 +189,65 @@ def test_tree_build_bitexact(ctx, case):
     np.testing.assert_array_equal(g["rint"], t.field_max(h) * 1.1)
 
 
+def _neigh_gpu(ctx, pos, h, n_local, htol, Rkern):
+    g = _tree_gpu(ctx, pos, h, 3, htol)
+    check(lib().shamx_neigh_build(ctx, stream(), n_local, g["x"].data_ptr(), g["y"].data_ptr(),
+                                  g["z"].data_ptr(), g["h"].data_ptr(), htol, Rkern))
+    s = np.zeros(4, np.uint32)
+    check(lib().shamx_neigh_sizes(ctx, s.ctypes.data_as(C.POINTER(C.c_uint32))))
+    P = int(s[1]) + (int(s[2]) << 32)
+    PL = int(s[3])
+    L = g["L"]
+    t = {k: torch.zeros(max(m, 1), dtype=torch.int32, device=DEV)
+         for k, m in (("cnt", n_local), ("offset", n_local), ("ids", P), ("owner", n_local),
+                      ("leaf_cnt", L), ("leaf_offset", L), ("leaf_ids", PL))}
+    check(lib().shamx_neigh_get(ctx, stream(), *[t[k].data_ptr() for k in
+                                                 ("cnt", "offset", "ids", "owner", "leaf_cnt",
+                                                  "leaf_offset", "leaf_ids")]))
+    torch.cuda.synchronize()
+    out = {k: v.cpu().numpy().view(np.uint32) for k, v in t.items()}
+    out["ids"], out["leaf_ids"] = out["ids"][:P], out["leaf_ids"][:PL]
+    return out, g
+
+
+@pytest.mark.parametrize("case", ["jittered", "clustered", "two_scales"])
+def test_neighbour_cache_sets_match_oracle(ctx, case):
+    """Per-particle neighbour sets and counts, owner leaves and leaf lists vs the oracle."""
+    say(f"[neigh] case={case}")
+    rng = np.random.default_rng(21)
+    if case == "jittered":
+        pos = setups.lattice_cubic(1 / 20, (0, 0, 0), (1, 1, 1))
+        pos = pos + rng.uniform(-0.45, 0.45, pos.shape) / 20
+        h = np.full(pos.shape[0], 1.2 / 20)
+    elif case == "clustered":  # Plummer-like core: leaves of hundreds of particles
+        from scipy.spatial import cKDTree
+
+        n = 40_000
+        r = rng.pareto(1.5, n)[:, None] * 2e-3
+        d = rng.normal(size=(n, 3))
+        pos = r * d / np.linalg.norm(d, axis=1, keepdims=True)
+        h = cKDTree(pos).query(pos, 40)[0][:, -1] / 2  # ~60 neighbours per particle
+    else:  # two smoothing-length scales side by side
+        pos = rng.uniform(0, 1, (20_000, 3))
+        h = np.where(pos[:, 0] < 0.5, 0.02, 0.06)
+    n_local = int(pos.shape[0] * 0.8)
+    go, g = _neigh_gpu(ctx, pos, h, n_local, 1.1, 2.0)
+    o = orc.neigh_cache(pos, h, n_local)
+    assert g["L"] == o["L"]
+    np.testing.assert_array_equal(go["owner"], o["owner"])
+    np.testing.assert_array_equal(go["leaf_cnt"], o["leaf_cnt"])
+    np.testing.assert_array_equal(go["leaf_ids"], o["leaf_ids"])
+    np.testing.assert_array_equal(go["cnt"], o["cnt"])
+    np.testing.assert_array_equal(go["offset"], o["offset"])
+    for a in range(n_local):
+        ga = go["ids"][go["offset"][a]: go["offset"][a] + go["cnt"][a]]
+        oa = o["ids"][o["offset"][a]: o["offset"][a] + o["cnt"][a]]
+        if case == "jittered":  # unique Morton codes: same order, not only same set
+            np.testing.assert_array_equal(ga, oa)
+        else:
+            np.testing.assert_array_equal(np.sort(ga), np.sort(oa))
+
+
 def test_h_iterate_golden_on_gpu():
     """IterateSmoothingLengthDensityTests.cpp:250-366 sequences, through shamx_h_iterate."""
     import json
