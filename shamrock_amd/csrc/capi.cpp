@@ -53,13 +53,15 @@ namespace {
     // small device scratch for the phase entries that do not take a context
     struct Slots {
         DevArray<u64> v;
+        DevArray<u64> part; // per-block partials of the reductions
         Slots() { v.resize(8); }
     };
     thread_local std::unique_ptr<Slots> g_slots;
-    u64 *slots() {
+    Slots &slot_state() {
         if (!g_slots) g_slots = std::make_unique<Slots>();
-        return g_slots->v.data();
+        return *g_slots;
     }
+    u64 *slots() { return slot_state().v.data(); }
 
     // phase timing with HIP events, read back once the step has synchronised
     struct PhaseTimer {
@@ -122,7 +124,8 @@ struct shamx_solver {
     DevArray<u32> ghost_src, gwork;
     DevArray<u8> ghost_oid;
     DevArray<unsigned char> gscratch;
-    DevArray<u64> red;
+    DevArray<u64> red, part;
+    DevArray<unsigned char> frec; // force records, sorted order
     TreeBuffers tree;
     NeighBuffers nb;
     f64 t = 0, dt = 0, cfl_mult = 1e-2; // Solver.hpp:124-147
@@ -163,7 +166,11 @@ struct shamx_solver {
         return ordered_to_f64(k);
     }
 
-    Csr csr() const { return Csr{nb.cnt.data(), nb.offset.data(), nb.ids.data()}; }
+    // sorted-layout CSR: local particles in sorted order, records and rows by sorted position
+    Csr csr() const {
+        return Csr{nb.cnt.data(), nb.offset.data(), nb.ids.data(), nb.lpos.data(),
+                   tree.sort_index.data()};
+    }
 
     // Solver.cpp:984-1046 + BasicSPHGhosts: images of the local particles after them
     void ghosts() {
@@ -208,7 +215,7 @@ struct shamx_solver {
             last.n_sorts++;
             b = timer.begin(stream);
             neigh_build(stream, nb, tree, N, x.data(), y.data(), z.data(), h.data(),
-                        cfg.htol_up_coarse, Rkern);
+                        cfg.htol_up_coarse, Rkern, true);
             timer.end(stream, PH_NEIGH, b);
             need(cfg.gpart_mass != 0, "invalid gpart_mass 0, this configuration can not converge");
             b = timer.begin(stream);
@@ -217,11 +224,10 @@ struct shamx_solver {
                                      hipMemcpyDeviceToDevice, stream));
             f64 max_eps = std::numeric_limits<f64>::max(), min_eps = 0;
             for (int it = 0; it < cfg.h_iter_per_subcycles; it++) {
-                fill_u64(stream, red.data() + 1, ORDERED_NEG_INF, 1);
-                fill_u64(stream, red.data() + 2, ORDERED_POS_INF, 1);
-                h_iterate(stream, cfg.kernel, N, csr(), x.data(), y.data(), z.data(),
-                          h_old.data(), h.data(), eps.data(), cfg.gpart_mass, cfg.htol_up_coarse,
-                          cfg.htol_up_fine, red.data() + 1, cfg.mixed_precision);
+                h_iterate(stream, cfg.kernel, N, csr(), nb.ps.data(), x.data(), y.data(),
+                          z.data(), h_old.data(), h.data(), eps.data(), cfg.gpart_mass,
+                          cfg.htol_up_coarse, cfg.htol_up_fine, red.data() + 1,
+                          cfg.mixed_precision, part);
                 u64 k[2];
                 SHAMX_HIP(hipMemcpyAsync(k, red.data() + 1, 16, hipMemcpyDeviceToHost, stream));
                 SHAMX_HIP(hipStreamSynchronize(stream));
@@ -238,8 +244,8 @@ struct shamx_solver {
         last.subcycles = hstep + 1;
         if (hstep == hmax) throw std::runtime_error("the h iterator is not converged");
         int b = timer.begin(stream);
-        compute_omega(stream, cfg.kernel, N, csr(), x.data(), y.data(), z.data(), h.data(),
-                      omega.data(), cfg.gpart_mass);
+        compute_omega(stream, cfg.kernel, N, csr(), nb.ps.data(), x.data(), y.data(), z.data(),
+                      h.data(), omega.data(), cfg.gpart_mass);
         timer.end(stream, PH_OMEGA, b);
     }
 
@@ -269,14 +275,16 @@ struct shamx_solver {
             b = timer.begin(stream);
             FieldPtrs fp{{h.data(), u.data(), vx.data(), vy.data(), vz.data(), omega.data()}, 6};
             gather_ghosts(stream, N, G, ghost_src.data(), fp);
-            compute_eos(stream, cfg.kernel, N + G, h.data(), u.data(), x.data(), y.data(),
-                        z.data(), P.data(), cs.data(), cfg.gpart_mass,
-                        EosArgs{cfg.eos_kind, cfg.gamma, cfg.eos_cs, cfg.eos_cs0, cfg.eos_q,
-                                cfg.eos_r0});
+            // EOS of the merged set fused with the sorted force records
+            frec.resize(force_record_bytes(cfg.mixed_precision) * (size_t) (N + G));
+            pack_force(stream, cfg.kernel, N + G, tree.sort_index.data(), x.data(), y.data(),
+                       z.data(), h.data(), vx.data(), vy.data(), vz.data(), u.data(),
+                       omega.data(), P.data(), cs.data(), frec.data(), cfg.gpart_mass,
+                       EosArgs{cfg.eos_kind, cfg.gamma, cfg.eos_cs, cfg.eos_cs0, cfg.eos_q,
+                               cfg.eos_r0},
+                       cfg.mixed_precision);
             timer.end(stream, PH_EOS, b);
             b = timer.begin(stream);
-            fill_u64(stream, red.data() + 3, ORDERED_POS_INF, 1);
-            fill_u64(stream, red.data() + 7, 0, 1);
             ForceArgs fa{cfg.gpart_mass,
                          cfg.alpha_u,
                          cfg.alpha_AV,
@@ -284,19 +292,17 @@ struct shamx_solver {
                          cfg.cfl_cour * cfl_mult,
                          cfg.cfl_force * cfl_mult,
                          cfg.mixed_precision};
-            compute_force(stream, cfg.kernel, N, csr(), x.data(), y.data(), z.data(), vx.data(),
-                          vy.data(), vz.data(), h.data(), omega.data(), u.data(), P.data(),
-                          cs.data(), ax.data(), ay.data(), az.data(), du.data(), ax_old.data(),
-                          ay_old.data(), az_old.data(), du_old.data(), vsig.data(), dtc.data(),
-                          fa, red.data() + 3, red.data() + 7);
+            compute_force(stream, cfg.kernel, N, csr(), frec.data(), x.data(), y.data(),
+                          z.data(), vx.data(), vy.data(), vz.data(), h.data(), omega.data(),
+                          u.data(), P.data(), cs.data(), ax.data(), ay.data(), az.data(),
+                          du.data(), ax_old.data(), ay_old.data(), az_old.data(), du_old.data(),
+                          vsig.data(), dtc.data(), fa, red.data() + 3, red.data() + 7, part);
             timer.end(stream, PH_FORCE, b);
             last.n_force_evals++;
             b = timer.begin(stream);
-            fill_u64(stream, red.data() + 4, ORDERED_NEG_INF, 1);
-            fill_u64(stream, red.data() + 5, 0, 2);
             corrector(stream, N, dtc_ / 2, vx.data(), vy.data(), vz.data(), ax.data(), ay.data(),
                       az.data(), ax_old.data(), ay_old.data(), az_old.data(), u.data(), du.data(),
-                      du_old.data(), cfg.gpart_mass, red.data() + 4);
+                      du_old.data(), cfg.gpart_mass, red.data() + 4, part);
             timer.end(stream, PH_CORR, b);
             u64 k[5];
             SHAMX_HIP(hipMemcpyAsync(k, red.data() + 3, 40, hipMemcpyDeviceToHost, stream));
@@ -364,6 +370,32 @@ struct shamx_solver {
             SHAMX_HIP(hipMemcpy(box_dev.data(), b, sizeof(b), hipMemcpyHostToDevice));
         }
         return box_dev.data();
+    }
+
+    // the neighbour cache in the reference layout (row = particle id, entries = particle ids),
+    // rebuilt on the host from the sorted layout: 5 counts, 6 offsets, 7 ids
+    void reference_layout_csr(int what, u32 *dst) {
+        u32 Nm = N + G;
+        std::vector<u32> cnt_s(Nm), off_s(Nm), ids_s(nb.n_pairs), sidx(Nm), inv(Nm);
+        SHAMX_HIP(hipMemcpyAsync(cnt_s.data(), nb.cnt.data(), Nm * 4, hipMemcpyDeviceToHost, stream));
+        SHAMX_HIP(hipMemcpyAsync(off_s.data(), nb.offset.data(), Nm * 4, hipMemcpyDeviceToHost,
+                                 stream));
+        SHAMX_HIP(hipMemcpyAsync(sidx.data(), tree.sort_index.data(), Nm * 4,
+                                 hipMemcpyDeviceToHost, stream));
+        if (what == 7 && nb.n_pairs)
+            SHAMX_HIP(hipMemcpyAsync(ids_s.data(), nb.ids.data(), nb.n_pairs * 4,
+                                     hipMemcpyDeviceToHost, stream));
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        for (u32 k = 0; k < Nm; k++) inv[sidx[k]] = k;
+        u64 acc = 0;
+        for (u32 a = 0; a < N; a++) {
+            u32 row = inv[a], c = cnt_s[row];
+            if (what == 5) dst[a] = c;
+            if (what == 6) dst[a] = (u32) acc;
+            if (what == 7)
+                for (u32 k = 0; k < c; k++) dst[acc + k] = sidx[ids_s[off_s[row] + k]];
+            acc += c;
+        }
     }
 
     ~shamx_solver() {
@@ -497,7 +529,8 @@ int shamx_neigh_build(shamx_ctx *ctx, void *stream, uint32_t n_local, const doub
     return guarded([&] {
         need(ctx && ctx->tree.L > 0, "shamx_neigh_build: build the tree first");
         need(n_local <= ctx->tree.n_obj, "shamx_neigh_build: n_local > tree object count");
-        neigh_build((hipStream_t) stream, ctx->neigh, ctx->tree, n_local, x, y, z, h, htol, Rkern);
+        neigh_build((hipStream_t) stream, ctx->neigh, ctx->tree, n_local, x, y, z, h, htol, Rkern,
+                    false);
     });
 }
 
@@ -534,10 +567,8 @@ int shamx_h_iterate(void *stream, int kernel, uint32_t n, const uint32_t *cnt,
     return guarded([&] {
         hipStream_t s = (hipStream_t) stream;
         u64 *sl       = slots();
-        fill_u64(s, sl, ORDERED_NEG_INF, 1);
-        fill_u64(s, sl + 1, ORDERED_POS_INF, 1);
-        h_iterate(s, kernel, n, Csr{cnt, offset, ids}, x, y, z, h_old, h_new, eps, gpart_mass,
-                  h_evol_max, h_evol_iter_max, sl, mixed);
+        h_iterate(s, kernel, n, Csr{cnt, offset, ids}, nullptr, x, y, z, h_old, h_new, eps,
+                  gpart_mass, h_evol_max, h_evol_iter_max, sl, mixed, slot_state().part);
         if (eps_minmax_host) {
             u64 k[2];
             SHAMX_HIP(hipMemcpyAsync(k, sl, 16, hipMemcpyDeviceToHost, s));
@@ -552,8 +583,8 @@ int shamx_omega(void *stream, int kernel, uint32_t n, const uint32_t *cnt, const
                 const uint32_t *ids, const double *x, const double *y, const double *z,
                 const double *h, double *omega, double gpart_mass) {
     return guarded([&] {
-        compute_omega((hipStream_t) stream, kernel, n, Csr{cnt, offset, ids}, x, y, z, h, omega,
-                      gpart_mass);
+        compute_omega((hipStream_t) stream, kernel, n, Csr{cnt, offset, ids}, nullptr, x, y, z, h,
+                      omega, gpart_mass);
     });
 }
 
@@ -579,11 +610,11 @@ int shamx_force(void *stream, int kernel, uint32_t n, const uint32_t *cnt, const
     return guarded([&] {
         hipStream_t s = (hipStream_t) stream;
         u64 *sl       = slots();
-        fill_u64(s, sl, ORDERED_POS_INF, 1);
-        compute_force(s, kernel, n, Csr{cnt, offset, ids}, x, y, z, vx, vy, vz, h, omega, u, P,
-                      cs, ax, ay, az, du, ax_old, ay_old, az_old, du_old, vsig, dt_cfl,
+        compute_force(s, kernel, n, Csr{cnt, offset, ids}, nullptr, x, y, z, vx, vy, vz, h,
+                      omega, u, P, cs, ax, ay, az, du, ax_old, ay_old, az_old, du_old, vsig,
+                      dt_cfl,
                       ForceArgs{gpart_mass, alpha_u, alpha_AV, beta_AV, C_cour, C_force, mixed},
-                      sl, nullptr);
+                      sl, nullptr, slot_state().part);
         if (dt_min_host) {
             u64 k;
             SHAMX_HIP(hipMemcpyAsync(&k, sl, 8, hipMemcpyDeviceToHost, s));
@@ -616,10 +647,8 @@ int shamx_corrector(void *stream, uint32_t n, double hdt, double *vx, double *vy
     return guarded([&] {
         hipStream_t s = (hipStream_t) stream;
         u64 *sl       = slots();
-        fill_u64(s, sl, ORDERED_NEG_INF, 1);
-        fill_u64(s, sl + 1, 0, 2);
         corrector(s, n, hdt, vx, vy, vz, ax, ay, az, ax_old, ay_old, az_old, u, du, du_old,
-                  gpart_mass, sl);
+                  gpart_mass, sl, slot_state().part);
         u64 k[3];
         SHAMX_HIP(hipMemcpyAsync(k, sl, 24, hipMemcpyDeviceToHost, s));
         SHAMX_HIP(hipStreamSynchronize(s));
@@ -745,9 +774,9 @@ int shamx_solver_get_u32(shamx_solver *s, int what, uint32_t *dst) {
         case 2: d2h(st, dst, s->tree.sort_index.data(), Nm); break;
         case 3: d2h(st, dst, s->tree.reduc_map.data(), (size_t) L + 2); break;
         case 4: d2h(st, dst, s->nb.owner.data(), s->N); break;
-        case 5: d2h(st, dst, s->nb.cnt.data(), s->N); break;
-        case 6: d2h(st, dst, s->nb.offset.data(), s->N); break;
-        case 7: d2h(st, dst, s->nb.ids.data(), s->nb.n_pairs); break;
+        case 5:
+        case 6:
+        case 7: s->reference_layout_csr(what, dst); break;
         case 8: d2h(st, dst, s->nb.leaf_cnt.data(), L); break;
         case 9: d2h(st, dst, s->nb.leaf_offset.data(), L); break;
         case 10: d2h(st, dst, s->nb.leaf_ids.data(), s->nb.n_leaf_pairs); break;

@@ -48,7 +48,7 @@ namespace shamx {
         DevArray<u32> lchild, rchild, endrange; // I
         DevArray<u8> lflag, rflag;     // I
         DevArray<u32> parent;          // I + L
-        DevArray<u32> visits;          // I
+        DevArray<u16> depth;           // I (root 0)
         DevArray<f64> nodes;           // (I + L) * NODE_F64 records (tree_layout.hpp)
         DevArray<u32> counters;        // small device scalars
         DevArray<unsigned char> scratch;
@@ -69,9 +69,15 @@ namespace shamx {
     struct NeighBuffers {
         u32 n_local = 0;
         u64 n_pairs = 0, n_leaf_pairs = 0;
+        bool sorted_layout = false;
         DevArray<u32> leaf_cnt, leaf_offset, leaf_ids; // stage 1 (per leaf)
         DevArray<u32> owner;                            // per local particle
-        DevArray<u32> cnt, offset, ids;                 // stage 2 CSR (per local particle)
+        // stage 2 CSR.  Reference layout: one row per local particle id, entries are particle
+        // ids.  Sorted layout: one row per sorted position (ghost rows empty), entries are
+        // sorted positions, so a row's candidates are runs of adjacent sorted records.
+        DevArray<u32> cnt, offset, ids;
+        DevArray<u32> lpos;                             // sorted layout: local rows in order
+        DevArray<u32> lflag, lscan;
         DevArray<double4> ps;                           // (x, y, z, h) in sorted order
         DevArray<u32> inv;                              // sorted position of each object
         DevArray<u32> mism;                             // owner leaf != own leaf
@@ -83,19 +89,33 @@ namespace shamx {
     // Synchronises twice to size the leaf list and the particle list.
     void neigh_build(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
                      const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
-                     f64 Rkern);
+                     f64 Rkern, bool sorted_layout);
 
     // ------------------------------------------------------------------ sph.hip
+    // Neighbour CSR as the pair loops read it.  Particle j of a launch (j < n) reads row
+    // sel[j] and writes particle outi[row]; null sel / outi mean the identity (reference
+    // layout).  In the sorted layout row == sorted position and ids are sorted positions.
     struct Csr {
         const u32 *cnt, *offset, *ids;
+        const u32 *sel  = nullptr;
+        const u32 *outi = nullptr;
+    };
+    // reductions of per-block partials (one u64 per block and slot) by k_finalize
+    enum RedOp : int { RED_MAX_KEY = 0, RED_MIN_KEY = 1, RED_SUM_U64 = 2, RED_SUM_F64 = 3 };
+    struct RedSpec {
+        int nslot;
+        int op[4];
+        int out_slot[4];
     };
     // IterateSmoothingLengthDensity (one Newton sweep); eps_minmax (device, 2 ordered keys)
-    // receives max and min of eps over all n particles.
-    void h_iterate(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
-                   const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
-                   f64 h_evol_max, f64 h_evol_iter_max, u64 *eps_minmax, int mixed);
-    void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
-                       const f64 *z, const f64 *h, f64 *omega, f64 mass);
+    // receives max and min of eps over all n particles.  Neighbour positions come from the
+    // sorted records ps (row-indexed) when given, else from x, y, z.
+    void h_iterate(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
+                   const f64 *y, const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
+                   f64 h_evol_max, f64 h_evol_iter_max, u64 *eps_minmax, int mixed,
+                   DevArray<u64> &part);
+    void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
+                       const f64 *y, const f64 *z, const f64 *h, f64 *omega, f64 mass);
     struct EosArgs {
         int kind;
         f64 gamma, cs, cs0, q, r0;
@@ -108,23 +128,34 @@ namespace shamx {
         f64 C_cour, C_force; // CFL factors (cfl * multiplier)
         int mixed;           // 1: FP32 pair sums with Kahan compensation
     };
-    // UpdateDerivs constant AV (+ a_old save, vsig, per-particle CFL dt folded into dt_min key)
-    void compute_force(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
-                       const f64 *z, const f64 *vx, const f64 *vy, const f64 *vz, const f64 *h,
-                       const f64 *omega, const f64 *u, const f64 *P, const f64 *cs, f64 *ax,
-                       f64 *ay, f64 *az, f64 *du, f64 *ax_old, f64 *ay_old, f64 *az_old,
-                       f64 *du_old, f64 *vsig, f64 *dt_cfl, ForceArgs fa, u64 *dt_min_key,
-                       u64 *pair_count);
+    // bytes of one force record (128 FP64, 64 mixed)
+    size_t force_record_bytes(int mixed);
+    // EOS on nm merged particles fused with the force-record build; record s describes
+    // particle perm[s] (identity when null).  P and cs are written per particle id.
+    void pack_force(hipStream_t s, int kernel, u32 nm, const u32 *perm, const f64 *x,
+                    const f64 *y, const f64 *z, const f64 *h, const f64 *vx, const f64 *vy,
+                    const f64 *vz, const f64 *u, const f64 *omega, f64 *P, f64 *cs, void *rec,
+                    f64 mass, EosArgs e, int mixed);
+    // UpdateDerivs constant AV (+ a_old save, vsig, per-particle CFL dt reduced into the
+    // dt_min key, interacting-pair count).  Neighbours come from the records rec (row-indexed)
+    // when given, else from the SoA arrays.
+    void compute_force(hipStream_t s, int kernel, u32 n, Csr c, const void *rec, const f64 *x,
+                       const f64 *y, const f64 *z, const f64 *vx, const f64 *vy, const f64 *vz,
+                       const f64 *h, const f64 *omega, const f64 *u, const f64 *P, const f64 *cs,
+                       f64 *ax, f64 *ay, f64 *az, f64 *du, f64 *ax_old, f64 *ay_old,
+                       f64 *az_old, f64 *du_old, f64 *vsig, f64 *dt_cfl, ForceArgs fa,
+                       u64 *dt_min_key, u64 *pair_count, DevArray<u64> &part);
     // leapfrog predictor (Solver.cpp:390-524) fused with the periodic wrap (integrators.cpp:206)
     void predictor(hipStream_t s, u32 n, f64 dt, f64 *x, f64 *y, f64 *z, f64 *vx, f64 *vy,
                    f64 *vz, const f64 *ax, const f64 *ay, const f64 *az, f64 *u, const f64 *du,
                    int periodic, const f64 *box6);
     // leapfrog corrector (integrators.cpp:87-119) + reductions for eps_v and energy:
-    // red[0] ordered key of max |dv|^2, red[1] sum |v|^2 (f64 bits), red[2] energy (f64 bits)
+    // red[0] ordered key of max |dv|^2, red[1] sum |v|^2 (f64 bits), red[2] energy (f64 bits);
+    // the sums are reduced in a fixed order (deterministic)
     void corrector(hipStream_t s, u32 n, f64 hdt, f64 *vx, f64 *vy, f64 *vz, const f64 *ax,
                    const f64 *ay, const f64 *az, const f64 *ax_old, const f64 *ay_old,
                    const f64 *az_old, f64 *u, const f64 *du, const f64 *du_old, f64 mass,
-                   u64 *red);
+                   u64 *red, DevArray<u64> &part);
     // max over n values as an ordered key (device)
     void max_key_f64(hipStream_t s, u32 n, const f64 *v, u64 *key);
     // periodic image selection (BasicSPHGhosts.cpp:260-579): returns G (synchronises) and

@@ -13,11 +13,11 @@
 #include "launch.hpp"
 #include "tree_layout.hpp"
 
+#include <algorithm>
+
 namespace shamx {
 
     namespace {
-        constexpr int STACK = 40; // Karras tree on unique 30-bit codes has depth <= 31
-
         struct Box {
             f64 lo[3], hi[3];
         };
@@ -32,46 +32,69 @@ namespace shamx {
                    (fmax(alo[2], blo[2]) <= fmin(ahi[2], bhi[2]));
         }
 
+        // Depth-first traversal shared by the 64 lanes of a wave.  The stack lives in lanes
+        // (entry i in lane i: node id and the mask of lanes still descending); a node is
+        // tested by the lanes of its mask, and its children are pushed with the mask of the
+        // lanes that passed.  Each lane therefore visits exactly the nodes of its own
+        // depth-first walk (KarrasTreeTraverser.hpp:69-118, left child first), in the same
+        // order.  crit(rec) is evaluated per lane; found(node) is called in the lanes whose
+        // walk reaches leaf node.  Returns false on stack overflow (wave-uniform).
         template<class Crit, class Found>
-        __device__ inline bool traverse(const f64 *__restrict nodes, u32 I, u32 ncell, Crit &&crit,
-                                        Found &&found) {
-            u32 stack[STACK];
+        __device__ inline bool wave_traverse(const f64 *__restrict nodes, u32 I, u32 ncell,
+                                             u64 live, Crit &&crit, Found &&found) {
+            const int lane = __lane_id();
+            u32 st_node = 0, st_lo = 0, st_hi = 0; // lane i holds stack entry i
             int sp      = 0;
-            stack[sp++] = 0;
+            auto push   = [&](u32 node, u64 mask) { // sp is wave-uniform
+                if (lane == sp) {
+                    st_node = node;
+                    st_lo   = (u32) mask;
+                    st_hi   = (u32) (mask >> 32);
+                }
+                sp++;
+            };
+            push(0, live);
             while (sp > 0) {
-                u32 node       = stack[--sp];
+                sp--;
+                u32 node = (u32) __builtin_amdgcn_readlane((int) st_node, sp);
+                u64 mask = (u64) (u32) __builtin_amdgcn_readlane((int) st_lo, sp) |
+                           ((u64) (u32) __builtin_amdgcn_readlane((int) st_hi, sp) << 32);
                 const f64 *rec = nodes + (u64) node * NODE_F64;
-                if (crit(rec)) {
-                    if (node >= I) {
-                        found(node);
-                    } else {
-                        if (sp + 2 > STACK) return false;
-                        u32 l, r;
-                        node_children(nodes, node, l, r);
-                        if (l >= ncell || r >= ncell) return false;
-                        stack[sp++] = r;
-                        stack[sp++] = l;
-                    }
+                bool pass      = ((mask >> lane) & 1ull) && crit(rec);
+                u64 pm         = __ballot(pass);
+                if (!pm) continue;
+                if (node >= I) {
+                    if (pass) found(node);
+                } else {
+                    if (sp + 2 > WAVE) return false;
+                    u32 l, r;
+                    node_children(nodes, node, l, r);
+                    if (l >= ncell || r >= ncell) return false;
+                    push(r, pm);
+                    push(l, pm);
                 }
             }
             return true;
         }
 
-        // stage 1 count / fill
+        // stage 1 count / fill, one lane per leaf, 64 consecutive leaves per wave
         template<bool FILL>
-        __global__ void k_leaf_neigh(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
-                                     u32 *__restrict cnt, const u32 *__restrict offset,
-                                     u32 *__restrict ids, u32 *__restrict err) {
-            u32 g = blockIdx.x * blockDim.x + threadIdx.x;
-            if (g >= L) return;
-            const f64 *ra = nodes + (u64) (I + g) * NODE_F64;
+        __global__ void __launch_bounds__(256)
+            k_leaf_neigh_w(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
+                           u32 *__restrict cnt, const u32 *__restrict offset,
+                           u32 *__restrict ids, u32 *__restrict err) {
+            u32 g      = blockIdx.x * blockDim.x + threadIdx.x;
+            bool valid = g < L;
+            u64 live   = __ballot(valid);
+            if (!live) return;
+            const f64 *ra = nodes + (u64) (I + (valid ? g : 0u)) * NODE_F64;
             f64 rint_a    = ra[6] * Rkern;
             f64 alo[3] = {ra[0], ra[1], ra[2]}, ahi[3] = {ra[3], ra[4], ra[5]};
             f64 elo[3] = {alo[0] - rint_a, alo[1] - rint_a, alo[2] - rint_a};
             f64 ehi[3] = {ahi[0] + rint_a, ahi[1] + rint_a, ahi[2] + rint_a};
-            u32 c      = FILL ? offset[g] : 0u;
-            bool ok    = traverse(
-                nodes, I, I + L,
+            u32 c      = FILL && valid ? offset[g] : 0u;
+            bool ok    = wave_traverse(
+                nodes, I, I + L, live,
                 [&](const f64 *rec) {
                     f64 r      = rec[6] * Rkern;
                     f64 nlo[3] = {rec[0], rec[1], rec[2]}, nhi[3] = {rec[3], rec[4], rec[5]};
@@ -83,32 +106,41 @@ namespace shamx {
                     if (FILL) ids[c] = node;
                     c++;
                 });
-            if (!ok) atomicOr(err, 1u);
-            if (!FILL) cnt[g] = c;
+            if (!ok && __lane_id() == __builtin_ctzll(live)) atomicOr(err, 1u);
+            if (!FILL && valid) cnt[g] = c;
         }
 
-        __global__ void k_owner(u32 n, u32 I, const f64 *__restrict nodes,
-                                const f64 *__restrict x, const f64 *__restrict y,
-                                const f64 *__restrict z, u32 L, u32 *__restrict owner,
-                                u32 *__restrict err) {
-            u32 a = blockIdx.x * blockDim.x + threadIdx.x;
-            if (a >= n) return;
-            f64 p[3]  = {x[a], y[a], z[a]};
+        // owner leaf, lanes over sorted positions (spatially coherent waves)
+        __global__ void __launch_bounds__(256)
+            k_owner_w(u32 nm, u32 n, u32 I, const f64 *__restrict nodes,
+                      const u32 *__restrict sort_index, const double4 *__restrict ps, u32 L,
+                      u32 *__restrict owner, u32 *__restrict err) {
+            u32 s      = blockIdx.x * blockDim.x + threadIdx.x;
+            u32 a      = s < nm ? sort_index[s] : 0xffffffffu;
+            bool valid = a < n;
+            u64 live   = __ballot(valid);
+            if (!live) return;
+            double4 q = ps[valid ? s : 0u];
+            f64 p[3]  = {q.x, q.y, q.z};
             u32 found = 0x7fffffffu;
-            bool ok   = traverse(
-                nodes, I, I + L,
+            bool ok   = wave_traverse(
+                nodes, I, I + L, live,
                 [&](const f64 *rec) { // is_coord_in_range_incl_max (geometry_utils.hpp:74-81)
                     return (rec[0] <= p[0]) && (p[0] <= rec[3]) && (rec[1] <= p[1]) &&
                            (p[1] <= rec[4]) && (rec[2] <= p[2]) && (p[2] <= rec[5]);
                 },
                 [&](u32 node) { found = node - I; });
-            if (!ok) atomicOr(err, 1u);
-            if (found >= L) atomicOr(err, 2u);
-            owner[a] = (found < L) ? found : 0u;
+            if (!ok && __lane_id() == __builtin_ctzll(live)) atomicOr(err, 1u);
+            if (valid) {
+                if (found >= L) atomicOr(err, 2u);
+                owner[a] = (found < L) ? found : 0u;
+            }
         }
 
-        // stage 2 count / fill
-        template<bool FILL>
+        // stage 2 count / fill.  SORTED layout: the CSR is indexed by the sorted position of
+        // the particle and holds sorted positions of its neighbours; otherwise both are
+        // particle ids (the reference layout).
+        template<bool FILL, bool SORTED>
         __global__ void k_part_neigh(u32 n, const u32 *__restrict list,
                                      const u32 *__restrict list_len, u32 I,
                                      const f64 *__restrict nodes,
@@ -118,18 +150,19 @@ namespace shamx {
                                      const u32 *__restrict leaf_offset,
                                      const u32 *__restrict leaf_ids, const f64 *__restrict x,
                                      const f64 *__restrict y, const f64 *__restrict z,
-                                     const f64 *__restrict h, f64 htol, f64 Rker2,
-                                     u32 *__restrict cnt, const u32 *__restrict offset,
-                                     u32 *__restrict ids) {
+                                     const f64 *__restrict h, const u32 *__restrict inv,
+                                     f64 htol, f64 Rker2, u32 *__restrict cnt,
+                                     const u32 *__restrict offset, u32 *__restrict ids) {
             u32 t = blockIdx.x * blockDim.x + threadIdx.x;
             if (t >= (list ? *list_len : n)) return;
             u32 a      = list ? list[t] : t;
+            u32 ia     = SORTED ? inv[a] : a;
             f64 rint_a = h[a] * htol;
             f64 xa = x[a], ya = y[a], za = z[a];
             f64 ca = rint_a * rint_a * Rker2;
             u32 lo = owner[a];
             u32 k0 = leaf_offset[lo], k1 = k0 + leaf_cnt[lo];
-            u32 c  = FILL ? offset[a] : 0u;
+            u32 c  = FILL ? offset[ia] : 0u;
             for (u32 k = k0; k < k1; k++) {
                 u32 s0, s1;
                 node_children(nodes, leaf_ids[k], s0, s1); // leaf record: sorted range
@@ -140,12 +173,12 @@ namespace shamx {
                     f64 rb   = h[b] * htol;
                     bool no_interact = rab2 > ca && rab2 > rb * rb * Rker2;
                     if (!no_interact) {
-                        if (FILL) ids[c] = b;
+                        if (FILL) ids[c] = SORTED ? s : b;
                         c++;
                     }
                 }
             }
-            if (!FILL) cnt[a] = c;
+            if (!FILL) cnt[ia] = c;
         }
 
         // sorted copy of positions and h (AoS, 32 B per object) + inverse permutation
@@ -172,6 +205,18 @@ namespace shamx {
             if (!(s0 <= sa && sa < s1)) list[atomicAdd(list_len, 1u)] = a;
         }
 
+        // local particles in sorted order: flag, then scatter after the scan
+        __global__ void k_local_flag(u32 nm, u32 n, const u32 *__restrict sort_index,
+                                     u32 *__restrict flag) {
+            u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+            if (s < nm) flag[s] = sort_index[s] < n ? 1u : 0u;
+        }
+        __global__ void k_local_scatter(u32 nm, const u32 *__restrict flag,
+                                        const u32 *__restrict pos, u32 *__restrict lpos) {
+            u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+            if (s < nm && flag[s]) lpos[pos[s]] = s;
+        }
+
         __device__ inline f64 wave_bcast(f64 v, int lane) {
             i64 b  = __double_as_longlong(v);
             int lo = __builtin_amdgcn_readlane((int) (b & 0xffffffff), lane);
@@ -180,14 +225,12 @@ namespace shamx {
         }
 
         constexpr int GRP_WAVES = 4;
-        constexpr int GRP_CAP   = 512; // candidate window per wave (LDS)
 
         // stage 2, one wave per leaf: the members are the leaf's local particles whose owner
         // is this leaf; the candidates (particles of the neighbour leaves, in list order then
-        // sorted order) are streamed through an LDS window 64 at a time, each lane holding one
-        // candidate; for each member the keep-mask is a ballot, so the compacted output keeps
-        // candidate order.
-        template<bool FILL>
+        // sorted order) are taken 64 at a time, each lane holding one candidate; for each
+        // member the keep-mask is a ballot, so the compacted output keeps candidate order.
+        template<bool FILL, bool SORTED>
         __global__ void __launch_bounds__(GRP_WAVES * WAVE)
             k_group_neigh(u32 n, u32 L, u32 I, const f64 *__restrict nodes,
                           const u32 *__restrict sort_index, const double4 *__restrict ps,
@@ -195,11 +238,10 @@ namespace shamx {
                           const u32 *__restrict leaf_offset, const u32 *__restrict leaf_ids,
                           f64 htol, f64 Rker2, u32 *__restrict cnt, const u32 *__restrict offset,
                           u32 *__restrict ids) {
-            __shared__ u32 win[GRP_WAVES][GRP_CAP];
+            __shared__ u32 rstart[GRP_WAVES][WAVE], rexcl[GRP_WAVES][WAVE];
             const int lane = __lane_id(), wid = threadIdx.x / WAVE;
             const u32 l    = blockIdx.x * GRP_WAVES + wid;
             if (l >= L) return;
-            u32 *w = win[wid];
             u32 m0, m1;
             node_children(nodes, I + l, m0, m1);
             const u64 lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -223,37 +265,43 @@ namespace shamx {
                 }
                 u64 members = __ballot(mem);
                 if (!members) continue;
-                u32 cursor = (FILL && mem) ? offset[a] : 0u;
-                u32 count  = 0;
-                u32 k = k0, part = 0;
-                while (k < k1) {
-                    // fill the candidate window with whole (or split) leaf ranges
-                    u32 wc = 0;
-                    while (k < k1 && wc < (u32) GRP_CAP) {
-                        u32 r0, r1;
-                        node_children(nodes, leaf_ids[k], r0, r1);
-                        r0 += part;
-                        u32 take = min(r1 - r0, (u32) GRP_CAP - wc);
-                        for (u32 j = lane; j < take; j += WAVE) w[wc + j] = r0 + j;
-                        wc += take;
-                        if (r0 + take == r1) {
-                            k++;
-                            part = 0;
-                        } else {
-                            part += take;
-                        }
+                const u32 ia = SORTED ? sm : a;
+                u32 cursor   = (FILL && mem) ? offset[ia] : 0u;
+                u32 count = 0;
+                // neighbour leaves in batches of up to 64: lane i loads the range of the i-th
+                // leaf of the batch; candidates are the concatenated ranges in list order
+                for (u32 kb = k0; kb < k1; kb += WAVE) {
+                    u32 k  = kb + lane;
+                    u32 r0 = 0, r1 = 0;
+                    if (k < k1) node_children(nodes, leaf_ids[k], r0, r1); // leaf: sorted range
+                    u32 sz  = r1 - r0;
+                    u32 inc = sz;
+                    for (int o = 1; o < WAVE; o <<= 1) {
+                        u32 t = __shfl_up(inc, o, WAVE);
+                        if (lane >= o) inc += t;
                     }
+                    const u32 total = __shfl(inc, WAVE - 1, WAVE);
+                    const u32 nr    = min(k1 - kb, (u32) WAVE);
+                    rstart[wid][lane] = r0;
+                    rexcl[wid][lane]  = inc - sz;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    for (u32 c0 = 0; c0 < wc; c0 += WAVE) {
+                    for (u32 c0 = 0; c0 < total; c0 += WAVE) {
                         u32 c      = c0 + lane;
-                        bool valid = c < wc;
-                        u32 sb     = valid ? w[c] : 0u;
-                        double4 q  = ps[valid ? sb : 0u];
+                        bool valid = c < total;
+                        // the range holding candidate c: largest j with rexcl[j] <= c (leaves
+                        // are never empty, so that range has c - rexcl[j] < its size)
+                        u32 j = 0;
+                        for (u32 st = WAVE / 2; st > 0; st >>= 1) {
+                            u32 t = j + st;
+                            if (t < nr && rexcl[wid][t] <= c) j = t;
+                        }
+                        u32 sb     = valid ? rstart[wid][j] + (c - rexcl[wid][j]) : 0u;
+                        double4 q  = ps[sb];
                         f64 rb     = q.w * htol;
                         f64 cb     = rb * rb * Rker2;
-                        u32 b      = FILL && valid ? sort_index[sb] : 0u;
+                        u32 b      = FILL && valid ? (SORTED ? sb : sort_index[sb]) : 0u;
                         u64 mm     = members;
                         while (mm) {
                             int m  = __builtin_ctzll(mm);
@@ -275,32 +323,79 @@ namespace shamx {
                             }
                         }
                     }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
-                if (!FILL && mem) cnt[a] = count;
+                if (!FILL && mem) cnt[ia] = count;
             }
+        }
+    } // namespace
+
+    namespace {
+        template<bool SORTED>
+        void stage2(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
+                    const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
+                    f64 Rker2, u32 *hc) {
+            u32 L = t.L, I = t.I, nm = t.n_obj;
+            u32 rows  = SORTED ? nm : n_local; // CSR rows
+            u32 *mism_len = nb.counters.data() + 5;
+            u32 *err      = nb.counters.data() + 4;
+            u32 gblocks   = div_up(L, GRP_WAVES);
+            if (SORTED) fill_u32(s, nb.cnt.data(), 0, rows); // ghost rows stay empty
+            k_group_neigh<false, SORTED><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+                n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
+                nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
+                htol, Rker2, nb.cnt.data(), nullptr, nullptr);
+            k_part_neigh<false, SORTED><<<div_up(n_local, 128), 128, 0, s>>>(
+                n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
+                nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
+                y, z, h, nb.inv.data(), htol, Rker2, nb.cnt.data(), nullptr, nullptr);
+            exclusive_scan_u32(s, nb.cnt.data(), nb.offset.data(), rows, nb.counters.data() + 1,
+                               nb.scratch.data());
+            SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
+                                     hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
+            if (hc[4] & 2u)
+                throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
+            (void) err;
+            nb.n_pairs    = hc[1];
+            nb.n_mismatch = hc[5];
+            nb.ids.resize(std::max<u64>(nb.n_pairs, 1));
+            k_group_neigh<true, SORTED><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+                n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
+                nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
+                htol, Rker2, nullptr, nb.offset.data(), nb.ids.data());
+            k_part_neigh<true, SORTED><<<div_up(n_local, 128), 128, 0, s>>>(
+                n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
+                nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
+                y, z, h, nb.inv.data(), htol, Rker2, nullptr, nb.offset.data(), nb.ids.data());
         }
     } // namespace
 
     void neigh_build(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
                      const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
-                     f64 Rkern) {
-        u32 L = t.L, I = t.I;
-        nb.n_local = n_local;
+                     f64 Rkern, bool sorted_layout) {
+        u32 L = t.L, I = t.I, nm = t.n_obj;
+        nb.n_local       = n_local;
+        nb.sorted_layout = sorted_layout;
+        u32 rows         = sorted_layout ? nm : n_local;
         nb.counters.resize(8);
-        nb.scratch.resize(std::max(scan_scratch_bytes(L), scan_scratch_bytes(n_local)));
+        nb.scratch.resize(std::max({scan_scratch_bytes(L), scan_scratch_bytes(rows),
+                                    scan_scratch_bytes(nm)}));
         nb.leaf_cnt.resize(L);
         nb.leaf_offset.resize(L);
         nb.owner.resize(std::max<u32>(n_local, 1));
-        nb.cnt.resize(std::max<u32>(n_local, 1));
-        nb.offset.resize(std::max<u32>(n_local, 1));
+        nb.cnt.resize(std::max<u32>(rows, 1));
+        nb.offset.resize(std::max<u32>(rows, 1));
         u32 *err = nb.counters.data() + 4;
         fill_u32(s, nb.counters.data(), 0, 8);
 
         // stage 1
-        k_leaf_neigh<false><<<div_up(L, 128), 128, 0, s>>>(L, I, t.nodes.data(), Rkern,
-                                                           nb.leaf_cnt.data(), nullptr, nullptr,
-                                                           err);
+        k_leaf_neigh_w<false><<<div_up(L, 256), 256, 0, s>>>(L, I, t.nodes.data(), Rkern,
+                                                             nb.leaf_cnt.data(), nullptr, nullptr,
+                                                             err);
         exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.leaf_offset.data(), L, nb.counters.data(),
                            nb.scratch.data());
         u32 hc[8];
@@ -309,56 +404,44 @@ namespace shamx {
         if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
         nb.n_leaf_pairs = hc[0];
         nb.leaf_ids.resize(std::max<u64>(nb.n_leaf_pairs, 1));
-        k_leaf_neigh<true><<<div_up(L, 128), 128, 0, s>>>(L, I, t.nodes.data(), Rkern,
-                                                          nullptr, nb.leaf_offset.data(),
-                                                          nb.leaf_ids.data(), err);
+        k_leaf_neigh_w<true><<<div_up(L, 256), 256, 0, s>>>(L, I, t.nodes.data(), Rkern,
+                                                            nullptr, nb.leaf_offset.data(),
+                                                            nb.leaf_ids.data(), err);
         if (n_local == 0) {
             nb.n_pairs = 0;
             return;
         }
-        // owner leaf
-        k_owner<<<div_up(n_local, 128), 128, 0, s>>>(n_local, I, t.nodes.data(), x, y, z, L,
-                                                     nb.owner.data(), err);
-        // sorted copy, inverse permutation, owner != own-leaf list
-        u32 nm = t.n_obj;
+        // sorted copy and inverse permutation, owner leaf, owner != own-leaf list
         nb.ps.resize(nm);
         nb.inv.resize(nm);
         nb.mism.resize(std::max<u32>(n_local, 1));
         k_sorted_copy<<<div_up(nm, 256), 256, 0, s>>>(nm, t.sort_index.data(), x, y, z, h,
                                                       nb.ps.data(), nb.inv.data());
+        k_owner_w<<<div_up(nm, 256), 256, 0, s>>>(nm, n_local, I, t.nodes.data(),
+                                                  t.sort_index.data(), nb.ps.data(), L,
+                                                  nb.owner.data(), err);
         u32 *mism_len = nb.counters.data() + 5;
         k_mismatch<<<div_up(n_local, 256), 256, 0, s>>>(n_local, I, t.nodes.data(),
                                                         nb.owner.data(), nb.inv.data(),
                                                         nb.mism.data(), mism_len);
-        // stage 2 count
+        if (sorted_layout) {
+            // sorted positions of the local particles, in sorted order
+            nb.lflag.resize(nm);
+            nb.lscan.resize(nm);
+            nb.lpos.resize(std::max<u32>(n_local, 1));
+            k_local_flag<<<div_up(nm, 256), 256, 0, s>>>(nm, n_local, t.sort_index.data(),
+                                                         nb.lflag.data());
+            exclusive_scan_u32(s, nb.lflag.data(), nb.lscan.data(), nm, nb.counters.data() + 6,
+                               nb.scratch.data());
+            k_local_scatter<<<div_up(nm, 256), 256, 0, s>>>(nm, nb.lflag.data(), nb.lscan.data(),
+                                                            nb.lpos.data());
+        }
+        // stage 2
         f64 Rker2 = Rkern * Rkern;
-        u32 gblocks = div_up(L, GRP_WAVES);
-        k_group_neigh<false><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
-            n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(), nb.owner.data(),
-            nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), htol, Rker2,
-            nb.cnt.data(), nullptr, nullptr);
-        k_part_neigh<false><<<div_up(n_local, 128), 128, 0, s>>>(
-            n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
-            nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x, y,
-            z, h, htol, Rker2, nb.cnt.data(), nullptr, nullptr);
-        exclusive_scan_u32(s, nb.cnt.data(), nb.offset.data(), n_local, nb.counters.data() + 1,
-                           nb.scratch.data());
-        SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32), hipMemcpyDeviceToHost, s));
-        SHAMX_HIP(hipStreamSynchronize(s));
-        if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
-        if (hc[4] & 2u)
-            throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
-        nb.n_pairs    = hc[1];
-        nb.n_mismatch = hc[5];
-        nb.ids.resize(std::max<u64>(nb.n_pairs, 1));
-        k_group_neigh<true><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
-            n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(), nb.owner.data(),
-            nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), htol, Rker2, nullptr,
-            nb.offset.data(), nb.ids.data());
-        k_part_neigh<true><<<div_up(n_local, 128), 128, 0, s>>>(
-            n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
-            nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x, y,
-            z, h, htol, Rker2, nullptr, nb.offset.data(), nb.ids.data());
+        if (sorted_layout)
+            stage2<true>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
+        else
+            stage2<false>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
         SHAMX_HIP(hipGetLastError());
     }
 

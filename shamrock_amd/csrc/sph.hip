@@ -15,6 +15,9 @@
 // Kahan compensation.
 #include "launch.hpp"
 
+#include <cstdlib>
+#include <type_traits>
+
 namespace shamx {
 
     namespace {
@@ -25,63 +28,96 @@ namespace shamx {
         __device__ inline u64 umax64(u64 a, u64 b) { return a > b ? a : b; }
         __device__ inline u64 umin64(u64 a, u64 b) { return a < b ? a : b; }
 
-        // ------------------------------------------------------------- h iteration
-        template<class K>
-        __global__ void __launch_bounds__(BLK)
-            k_h_iterate(u32 n, const u32 *__restrict cnt, const u32 *__restrict offset,
-                        const u32 *__restrict ids, const f64 *__restrict x,
-                        const f64 *__restrict y, const f64 *__restrict z,
-                        const f64 *__restrict h_old, f64 *__restrict h_new, f64 *__restrict eps,
-                        f64 mass, f64 h_evol_max, f64 h_evol_iter_max, u64 *__restrict minmax) {
-            using KK = Kern<K, f64>;
-            u32 a    = blockIdx.x * BLK + threadIdx.x;
-            u64 kmax = ORDERED_NEG_INF, kmin = ORDERED_POS_INF;
-            if (a < n) {
-                f64 e = eps[a];
-                if (e > 1e-6) {
-                    const f64 Rkern = K::Rkern;
-                    f64 h_max_evol_p = h_evol_iter_max, h_max_evol_m = 1 / h_evol_iter_max;
-                    f64 xa = x[a], ya = y[a], za = z[a];
-                    f64 h_a  = h_new[a];
-                    f64 dint = h_a * h_a * Rkern * Rkern;
-                    f64 rho_sum = 0, sumdWdh = 0;
-                    u32 k0 = offset[a], k1 = k0 + cnt[a];
-                    for (u32 k = k0; k < k1; k++) {
-                        u32 b    = ids[k];
-                        f64 dx   = xa - x[b], dy = ya - y[b], dz = za - z[b];
-                        f64 rab2 = dx * dx + dy * dy + dz * dz;
-                        if (rab2 > dint) continue;
-                        f64 rab = sqrt(rab2);
-                        rho_sum += mass * KK::W(rab, h_a);
-                        sumdWdh += mass * KK::dhW(rab, h_a);
-                    }
-                    f64 rho_ha  = rho_h(mass, h_a, KK::hfactd);
-                    f64 f_iter  = rho_sum - rho_ha;
-                    f64 df_iter = sumdWdh + 3 * rho_ha / h_a;
-                    f64 new_h   = h_a - f_iter / df_iter;
-                    if (new_h < h_a * h_max_evol_m) new_h = h_max_evol_m * h_a;
-                    if (new_h > h_a * h_max_evol_p) new_h = h_max_evol_p * h_a;
-                    f64 ha_0 = h_old[a];
-                    if (new_h < ha_0 * h_evol_max) {
-                        h_new[a] = new_h;
-                        e        = fabs(new_h - h_a) / ha_0;
-                    } else {
-                        h_new[a] = ha_0 * h_evol_max;
-                        e        = -1;
-                    }
-                    eps[a] = e;
-                }
-                kmax = kmin = f64_to_ordered(e);
+        // ------------------------------------------------------------- launch helpers
+        // The dispatcher hands block b to XCD b % 8.  Renumbering so that every XCD walks one
+        // contiguous run of logical blocks keeps particles that are close in sorted order, and
+        // the neighbour records they share, in the same L2.
+        __device__ inline u32 xcd_block(u32 b, u32 nb) {
+            constexpr u32 NX = 8;
+            u32 per = nb / NX, rem = nb % NX, x = b % NX, i = b / NX;
+            return x * per + (x < rem ? x : rem) + i;
+        }
+
+        // block-wide reduction; every thread gets the result
+        template<class T, class Op>
+        __device__ inline T block_reduce(T v, Op op, T *lds) {
+            v = wave_reduce(v, op);
+            if (__lane_id() == 0) lds[threadIdx.x / WAVE] = v;
+            __syncthreads();
+            T r = lds[0];
+            for (int w = 1; w < BLK / WAVE; w++) r = op(r, lds[w]);
+            __syncthreads();
+            return r;
+        }
+
+        // value held by lane L of this lane's group of G lanes (G = 4: DPP quad_perm
+        // broadcast, G = 16: DPP row_newbcast)
+        template<int G, int L>
+        __device__ inline f64 group_lane(f64 v) {
+            static_assert(G == 4 || G == 16, "DPP broadcasts cover quads and rows");
+            constexpr int ctrl = (G == 16) ? 0x150 + L : (L | (L << 2) | (L << 4) | (L << 6));
+            i64 b  = __double_as_longlong(v);
+            int lo = __builtin_amdgcn_mov_dpp((int) b, ctrl, 0xf, 0xf, false);
+            int hi = __builtin_amdgcn_mov_dpp((int) (b >> 32), ctrl, 0xf, 0xf, false);
+            return __longlong_as_double(((i64) hi << 32) | (u32) lo);
+        }
+        // acc + t[lane 0] + t[lane 1] + ... + t[lane G-1], left to right, in every lane of the
+        // group: the G terms of one iteration enter the sum in list order, one IEEE add each,
+        // exactly as in a sequential loop over the row.  Must be reached by the whole group.
+        template<int G, int L = 0>
+        __device__ inline f64 ordered_fold(f64 acc, f64 t) {
+            if constexpr (G == 1) {
+                return acc + t;
+            } else {
+                acc = acc + group_lane<G, L>(t);
+                if constexpr (L + 1 < G) return ordered_fold<G, L + 1>(acc, t);
+                else return acc;
             }
-            kmax = wave_reduce(kmax, umax64);
-            kmin = wave_reduce(kmin, umin64);
-            if (__lane_id() == 0) {
-                atomic_max_key(&minmax[0], kmax);
-                atomic_min_key(&minmax[1], kmin);
+        }
+        // the same with two terms per list entry, added in the order t1, t2
+        template<int G, int L = 0>
+        __device__ inline f64 ordered_fold2(f64 acc, f64 t1, f64 t2) {
+            if constexpr (G == 1) {
+                acc = acc + t1;
+                return acc + t2;
+            } else {
+                acc = acc + group_lane<G, L>(t1);
+                acc = acc + group_lane<G, L>(t2);
+                if constexpr (L + 1 < G) return ordered_fold2<G, L + 1>(acc, t1, t2);
+                else return acc;
             }
         }
 
-        // mixed: density sums in FP32 (Kahan) on FP64 r_ab; Newton step and test in FP64
+        template<int G>
+        __device__ inline f64 group_sum(f64 v) {
+            for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+            return v;
+        }
+        template<int G>
+        __device__ inline f64 group_max(f64 v) {
+            for (int o = G / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, G));
+            return v;
+        }
+
+        // rows of the neighbour CSR processed by a kernel: the j-th processed particle has row
+        // sel[j] (identity when null) and particle id outi[row] (identity when null)
+        struct Rows {
+            const u32 *sel, *outi, *cnt, *off, *ids;
+        };
+
+        // neighbour positions: sorted (x, y, z, h) records or the SoA arrays
+        struct PosRec {
+            const double4 *__restrict ps;
+            __device__ double3 pos(u32 b) const {
+                double4 q = ps[b];
+                return make_double3(q.x, q.y, q.z);
+            }
+        };
+        struct PosSoA {
+            const f64 *__restrict x, *__restrict y, *__restrict z;
+            __device__ double3 pos(u32 b) const { return make_double3(x[b], y[b], z[b]); }
+        };
+
         __device__ inline void kahan_add(f32 &s, f32 &c, f32 v) {
             f32 y = v - c;
             f32 t = s + y;
@@ -89,95 +125,163 @@ namespace shamx {
             s     = t;
         }
 
-        template<class K>
+        // ------------------------------------------------------------- h iteration
+        // One Newton sweep (IterateSmoothingLengthDensity.cpp:28-120).  G lanes share one
+        // particle and stride through its row; the density sums are combined across the group.
+        // MIXED: sums in FP32 with Kahan compensation on FP64 r_ab; Newton step in FP64.
+        template<class K, int G, bool MIXED, class Src>
         __global__ void __launch_bounds__(BLK)
-            k_h_iterate_mixed(u32 n, const u32 *__restrict cnt, const u32 *__restrict offset,
-                              const u32 *__restrict ids, const f64 *__restrict x,
-                              const f64 *__restrict y, const f64 *__restrict z,
-                              const f64 *__restrict h_old, f64 *__restrict h_new,
-                              f64 *__restrict eps, f64 mass, f64 h_evol_max,
-                              f64 h_evol_iter_max, u64 *__restrict minmax) {
+            k_h_iterate(u32 n, Rows r, Src src, const f64 *__restrict h_old,
+                        f64 *__restrict h_new, f64 *__restrict eps, f64 mass, f64 h_evol_max,
+                        f64 h_evol_iter_max, u64 *__restrict part) {
+            using KK = Kern<K, f64>;
             using KF = Kern<K, f32>;
-            u32 a    = blockIdx.x * BLK + threadIdx.x;
-            u64 kmax = ORDERED_NEG_INF, kmin = ORDERED_POS_INF;
-            if (a < n) {
-                f64 e = eps[a];
-                if (e > 1e-6) {
-                    const f64 Rkern = K::Rkern;
-                    f64 h_max_evol_p = h_evol_iter_max, h_max_evol_m = 1 / h_evol_iter_max;
-                    f64 xa = x[a], ya = y[a], za = z[a];
-                    f64 h_a  = h_new[a];
-                    f64 dint = h_a * h_a * Rkern * Rkern;
+            __shared__ u64 lds[2][BLK / WAVE];
+            const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
+            const u32 lg   = threadIdx.x % G;
+            const bool act = j < n;
+            u32 row = 0, a = 0;
+            f64 e = 0;
+            if (act) {
+                row = r.sel ? r.sel[j] : j;
+                a   = r.outi ? r.outi[row] : row;
+                e   = eps[a];
+            }
+            const bool work = act && e > 1e-6;
+            const f64 Rkern = K::Rkern;
+            f64 h_a = 0, rho_sum = 0, sumdWdh = 0;
+            if (work) {
+                h_a         = h_new[a];
+                double3 pa  = src.pos(row);
+                f64 dint    = h_a * h_a * Rkern * Rkern;
+                u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+                if constexpr (MIXED) {
                     f32 hf = (f32) h_a;
                     f32 rs = 0, rc = 0, ds = 0, dc = 0;
-                    u32 k0 = offset[a], k1 = k0 + cnt[a];
-                    for (u32 k = k0; k < k1; k++) {
-                        u32 b    = ids[k];
-                        f64 dx   = xa - x[b], dy = ya - y[b], dz = za - z[b];
+                    for (u32 k = k0 + lg; k < k1; k += G) {
+                        double3 q = src.pos(r.ids[k]);
+                        f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
                         f64 rab2 = dx * dx + dy * dy + dz * dz;
                         if (rab2 > dint) continue;
                         f32 rab = sqrtf((f32) rab2);
                         kahan_add(rs, rc, KF::W(rab, hf));
                         kahan_add(ds, dc, KF::dhW(rab, hf));
                     }
-                    f64 rho_sum = mass * ((f64) rs - (f64) rc);
-                    f64 sumdWdh = mass * ((f64) ds - (f64) dc);
-                    f64 rho_ha  = rho_h(mass, h_a, (f64) K::hfactd);
-                    f64 f_iter  = rho_sum - rho_ha;
-                    f64 df_iter = sumdWdh + 3 * rho_ha / h_a;
-                    f64 new_h   = h_a - f_iter / df_iter;
-                    if (new_h < h_a * h_max_evol_m) new_h = h_max_evol_m * h_a;
-                    if (new_h > h_a * h_max_evol_p) new_h = h_max_evol_p * h_a;
-                    f64 ha_0 = h_old[a];
-                    if (new_h < ha_0 * h_evol_max) {
-                        h_new[a] = new_h;
-                        e        = fabs(new_h - h_a) / ha_0;
-                    } else {
-                        h_new[a] = ha_0 * h_evol_max;
-                        e        = -1;
+                    rho_sum = (f64) rs - (f64) rc;
+                    sumdWdh = (f64) ds - (f64) dc;
+                } else {
+                    // group-uniform trip count; terms folded in list order
+                    for (u32 base = k0; base < k1; base += G) {
+                        u32 k  = base + lg;
+                        f64 tw = 0, th = 0;
+                        if (k < k1) {
+                            double3 q = src.pos(r.ids[k]);
+                            f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
+                            f64 rab2 = dx * dx + dy * dy + dz * dz;
+                            if (!(rab2 > dint)) {
+                                f64 rab = sqrt(rab2);
+                                tw      = mass * KK::W(rab, h_a);
+                                th      = mass * KK::dhW(rab, h_a);
+                            }
+                        }
+                        rho_sum = ordered_fold<G>(rho_sum, tw);
+                        sumdWdh = ordered_fold<G>(sumdWdh, th);
                     }
-                    eps[a] = e;
                 }
-                kmax = kmin = f64_to_ordered(e);
             }
-            kmax = wave_reduce(kmax, umax64);
-            kmin = wave_reduce(kmin, umin64);
-            if (__lane_id() == 0) {
-                atomic_max_key(&minmax[0], kmax);
-                atomic_min_key(&minmax[1], kmin);
+            if (MIXED) {
+                rho_sum = mass * group_sum<G>(rho_sum);
+                sumdWdh = mass * group_sum<G>(sumdWdh);
+            }
+            if (work && lg == 0) {
+                f64 h_max_evol_p = h_evol_iter_max, h_max_evol_m = 1 / h_evol_iter_max;
+                f64 rho_ha  = rho_h(mass, h_a, KK::hfactd);
+                f64 f_iter  = rho_sum - rho_ha;
+                f64 df_iter = sumdWdh + 3 * rho_ha / h_a;
+                f64 new_h   = h_a - f_iter / df_iter;
+                if (new_h < h_a * h_max_evol_m) new_h = h_max_evol_m * h_a;
+                if (new_h > h_a * h_max_evol_p) new_h = h_max_evol_p * h_a;
+                f64 ha_0 = h_old[a];
+                if (new_h < ha_0 * h_evol_max) {
+                    h_new[a] = new_h;
+                    e        = fabs(new_h - h_a) / ha_0;
+                } else {
+                    h_new[a] = ha_0 * h_evol_max;
+                    e        = -1;
+                }
+                eps[a] = e;
+            }
+            u64 kmax = ORDERED_NEG_INF, kmin = ORDERED_POS_INF;
+            if (act && lg == 0) kmax = kmin = f64_to_ordered(e);
+            kmax = block_reduce(kmax, umax64, lds[0]);
+            kmin = block_reduce(kmin, umin64, lds[1]);
+            if (threadIdx.x == 0) {
+                part[blockIdx.x]             = kmax;
+                part[gridDim.x + blockIdx.x] = kmin;
             }
         }
 
         // ------------------------------------------------------------- omega
-        template<class K>
+        // ComputeOmega.cpp:25-74, same group layout as the h iteration
+        template<class K, int G, class Src>
         __global__ void __launch_bounds__(BLK)
-            k_omega(u32 n, const u32 *__restrict cnt, const u32 *__restrict offset,
-                    const u32 *__restrict ids, const f64 *__restrict x, const f64 *__restrict y,
-                    const f64 *__restrict z, const f64 *__restrict h, f64 *__restrict omega,
+            k_omega(u32 n, Rows r, Src src, const f64 *__restrict h, f64 *__restrict omega,
                     f64 mass) {
             using KK = Kern<K, f64>;
-            u32 a    = blockIdx.x * BLK + threadIdx.x;
-            if (a >= n) return;
+            const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
+            const u32 lg   = threadIdx.x % G;
+            const bool act = j < n;
             const f64 Rkern = K::Rkern;
-            f64 xa = x[a], ya = y[a], za = z[a];
-            f64 h_a  = h[a];
-            f64 dint = h_a * h_a * Rkern * Rkern;
-            f64 rho_sum = 0, part_omega_sum = 0;
-            u32 k0 = offset[a], k1 = k0 + cnt[a];
-            for (u32 k = k0; k < k1; k++) {
-                u32 b    = ids[k];
-                f64 dx   = xa - x[b], dy = ya - y[b], dz = za - z[b];
-                f64 rab2 = dx * dx + dy * dy + dz * dz;
-                if (rab2 > dint) continue;
-                f64 rab = sqrt(rab2);
-                rho_sum += mass * KK::W(rab, h_a);
-                part_omega_sum += mass * KK::dhW(rab, h_a);
+            u32 row = 0, a = 0;
+            f64 h_a = 0, part_omega_sum = 0;
+            if (act) {
+                row        = r.sel ? r.sel[j] : j;
+                a          = r.outi ? r.outi[row] : row;
+                h_a        = h[a];
+                double3 pa = src.pos(row);
+                f64 dint   = h_a * h_a * Rkern * Rkern;
+                u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+                for (u32 base = k0; base < k1; base += G) {
+                    u32 k  = base + lg;
+                    f64 th = 0;
+                    if (k < k1) {
+                        double3 q = src.pos(r.ids[k]);
+                        f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
+                        f64 rab2 = dx * dx + dy * dy + dz * dz;
+                        if (!(rab2 > dint)) {
+                            f64 rab = sqrt(rab2);
+                            th      = mass * KK::dhW(rab, h_a);
+                        }
+                    }
+                    part_omega_sum = ordered_fold<G>(part_omega_sum, th);
+                }
             }
-            f64 rho_ha = rho_h(mass, h_a, KK::hfactd);
-            omega[a]   = 1 + (h_a / (3 * rho_ha)) * part_omega_sum;
+            if (act && lg == 0) {
+                f64 rho_ha = rho_h(mass, h_a, KK::hfactd);
+                omega[a]   = 1 + (h_a / (3 * rho_ha)) * part_omega_sum;
+            }
         }
 
         // ------------------------------------------------------------- eos
+        // ComputeEos.cpp:152-247 (adiabatic), 656-665 (isothermal), 726-798 (LP07)
+        __device__ inline void eos_eval(const EosArgs &e, f64 rho, f64 u, f64 x, f64 y, f64 z,
+                                        f64 &P, f64 &cs) {
+            if (e.kind == 0) {
+                P  = (e.gamma - 1) * rho * u;
+                cs = sqrt(e.gamma * P / rho);
+            } else if (e.kind == 1) {
+                P  = e.cs * e.cs * rho;
+                cs = e.cs;
+            } else {
+                f64 r0sq  = e.r0 * e.r0;
+                f64 mq    = -e.q;
+                f64 Rsq   = x * x + y * y + z * z;
+                f64 cs_sq = (e.cs0 * e.cs0) * pow(Rsq / r0sq, mq);
+                cs        = sqrt(cs_sq);
+                P         = cs_sq * rho;
+            }
+        }
+
         template<class K>
         __global__ void k_eos(u32 nm, const f64 *__restrict h, const f64 *__restrict u,
                               const f64 *__restrict x, const f64 *__restrict y,
@@ -186,233 +290,362 @@ namespace shamx {
             u32 i = blockIdx.x * blockDim.x + threadIdx.x;
             if (i >= nm) return;
             f64 rho = rho_h(mass, h[i], (f64) K::hfactd);
-            if (e.kind == 0) {
-                f64 P_a = (e.gamma - 1) * rho * u[i];
-                P[i]    = P_a;
-                cs[i]   = sqrt(e.gamma * P_a / rho);
-            } else if (e.kind == 1) {
-                P[i]  = e.cs * e.cs * rho;
-                cs[i] = e.cs;
-            } else {
-                f64 r0sq  = e.r0 * e.r0;
-                f64 mq    = -e.q;
-                f64 Rsq   = x[i] * x[i] + y[i] * y[i] + z[i] * z[i];
-                f64 cs_sq = (e.cs0 * e.cs0) * pow(Rsq / r0sq, mq);
-                cs[i]     = sqrt(cs_sq);
-                P[i]      = cs_sq * rho;
-            }
+            f64 p, c;
+            eos_eval(e, rho, u[i], x[i], y[i], z[i], p, c);
+            P[i]  = p;
+            cs[i] = c;
         }
 
-        // ------------------------------------------------------------- force (FP64)
+        // ------------------------------------------------------------- force records
         __device__ inline f64 inv_sat_positive(f64 v) { return (v >= 1e-9) ? 1. / v : 0.; }
         __device__ inline f64 inv_sat_zero(f64 v) { return (v != 0. && v == v) ? 1. / v : 0.; }
 
+        // one 128 B line per particle: everything the pair loop reads of a neighbour, with the
+        // per-particle factors of UpdateDerivs.cpp:140-240 evaluated once (same IEEE operations
+        // as in the pair loop, so the values are bit-identical)
+        struct alignas(16) FRec {
+            f64 x, y, z, h, vx, vy, vz, u, P, cs, omega, rho, inv_sub, romega, hR2, pad;
+        };
+        static_assert(sizeof(FRec) == 128, "force record is one 128 B line");
+        // mixed: FP64 positions for r_ab, FP32 fields (64 B)
+        struct alignas(16) FRec32 {
+            f64 x, y, z;
+            f32 h, vx, vy, vz, u, P, cs, rho, inv_sub, inv_romega;
+        };
+        static_assert(sizeof(FRec32) == 64, "mixed force record is 64 B");
+
         template<class K>
-        __global__ void __launch_bounds__(BLK)
-            k_force(u32 n, const u32 *__restrict cnt, const u32 *__restrict offset,
-                    const u32 *__restrict ids, const f64 *__restrict x, const f64 *__restrict y,
-                    const f64 *__restrict z, const f64 *__restrict vx, const f64 *__restrict vy,
-                    const f64 *__restrict vz, const f64 *__restrict h,
-                    const f64 *__restrict omega, const f64 *__restrict u,
-                    const f64 *__restrict P, const f64 *__restrict cs, f64 *__restrict ax,
-                    f64 *__restrict ay, f64 *__restrict az, f64 *__restrict du,
-                    f64 *__restrict ax_old, f64 *__restrict ay_old, f64 *__restrict az_old,
-                    f64 *__restrict du_old, f64 *__restrict vsig_out, f64 *__restrict dt_out,
-                    ForceArgs fa, u64 *__restrict dt_min, u64 *__restrict pair_count) {
-            using KK = Kern<K, f64>;
-            u32 a    = blockIdx.x * BLK + threadIdx.x;
-            u64 key  = ORDERED_POS_INF;
-            u64 npair = 0;
-            if (a < n) {
-                const f64 pmass = fa.mass;
-                const f64 Rker2 = K::Rkern * K::Rkern;
-                f64 h_a = h[a];
-                f64 xa = x[a], ya = y[a], za = z[a];
-                f64 vxa = vx[a], vya = vy[a], vza = vz[a];
-                f64 P_a = P[a], omega_a = omega[a];
-                const f64 u_a = u[a];
-                f64 rho_a     = rho_h(pmass, h_a, KK::hfactd);
-                f64 rho_a_sq  = rho_a * rho_a;
-                f64 rho_a_inv = 1. / rho_a;
-                f64 omega_a_rho_a_inv = 1 / (omega_a * rho_a);
-                f64 cs_a = cs[a];
-                f64 fx = 0, fy = 0, fz = 0, tdu = 0, vsig_max = 0;
-                f64 ca = h_a * h_a * Rker2;
-                u32 k0 = offset[a], k1 = k0 + cnt[a];
-                for (u32 k = k0; k < k1; k++) {
-                    u32 b    = ids[k];
-                    f64 dx   = xa - x[b], dy = ya - y[b], dz = za - z[b];
-                    f64 rab2 = dx * dx + dy * dy + dz * dz;
-                    f64 h_b  = h[b];
-                    if (rab2 > ca && rab2 > h_b * h_b * Rker2) continue;
-                    npair++;
-                    f64 rab = sqrt(rab2);
-                    f64 vxb = vx[b], vyb = vy[b], vzb = vz[b];
-                    const f64 u_b = u[b];
-                    f64 rho_b = rho_h(pmass, h_b, KK::hfactd);
-                    f64 P_b = P[b], omega_b = omega[b], cs_b = cs[b];
-                    f64 Fab_a = KK::dW(rab, h_a), Fab_b = KK::dW(rab, h_b);
-                    f64 vabx = vxa - vxb, vaby = vya - vyb, vabz = vza - vzb;
-                    f64 inv  = inv_sat_positive(rab);
-                    f64 rux = dx * inv, ruy = dy * inv, ruz = dz * inv;
-                    f64 v_ab_r_ab = vabx * rux + vaby * ruy + vabz * ruz;
-                    f64 abs_v     = fabs(v_ab_r_ab);
-                    f64 vsig_a = fa.alpha_AV * cs_a + fa.beta_AV * abs_v;
-                    f64 vsig_b = fa.alpha_AV * cs_b + fa.beta_AV * abs_v;
-                    f64 rho_avg = (rho_a + rho_b) * 0.5;
-                    f64 vsig_u  = sqrt(fabs(P_a - P_b) / rho_avg);
-                    f64 qa_ab = fmax(-0.5 * rho_a * vsig_a * v_ab_r_ab, 0.);
-                    f64 qb_ab = fmax(-0.5 * rho_b * vsig_b * v_ab_r_ab, 0.);
-                    f64 AV_P_a = P_a + qa_ab, AV_P_b = P_b + qb_ab;
-                    f64 nax = rux * Fab_a, nay = ruy * Fab_a, naz = ruz * Fab_a;
-                    f64 nbx = rux * Fab_b, nby = ruy * Fab_b, nbz = ruz * Fab_b;
-                    f64 ia = AV_P_a * inv_sat_zero(rho_a_sq * omega_a);
-                    f64 ib = AV_P_b * inv_sat_zero((rho_b * rho_b) * omega_b);
-                    f64 mm = -pmass;
-                    fx     = fx + mm * (ia * nax + ib * nbx);
-                    fy     = fy + mm * (ia * nay + ib * nby);
-                    fz     = fz + mm * (ia * naz + ib * nbz);
-                    tdu += AV_P_a * (omega_a_rho_a_inv * rho_a_inv) * pmass *
-                           (vabx * nax + vaby * nay + vabz * naz);
-                    tdu += pmass * fa.alpha_u * vsig_u * (u_a - u_b) * 0.5 *
-                           (Fab_a * omega_a_rho_a_inv + Fab_b / (rho_b * omega_b));
-                    // CFL signal speed (Solver.cpp:2749-2783): r_ab / |r_ab|, alpha 1, beta 2
-                    f64 r2x = dx / rab, r2y = dy / rab, r2z = dz / rab;
-                    if (rab < 1e-9) r2x = r2y = r2z = 0;
-                    f64 vdot = vabx * r2x + vaby * r2y + vabz * r2z;
-                    vsig_max = fmax(vsig_max, 1.0 * cs_a + 2.0 * fabs(vdot));
-                }
-                ax_old[a] = ax[a];
-                ay_old[a] = ay[a];
-                az_old[a] = az[a];
-                du_old[a] = du[a];
-                ax[a]     = fx;
-                ay[a]     = fy;
-                az[a]     = fz;
-                du[a]     = tdu;
-                vsig_out[a] = vsig_max;
-                f64 dt_c   = fa.C_cour * h_a / vsig_max;
-                f64 abs_a  = sqrt(fx * fx + fy * fy + fz * fz);
-                f64 dt_f   = fa.C_force * sqrt(h_a / abs_a);
-                f64 cfl_dt = fmin(__builtin_inf(), dt_c);
-                cfl_dt     = fmin(cfl_dt, dt_f);
-                dt_out[a]  = cfl_dt;
-                key        = f64_to_ordered(cfl_dt);
+        __device__ inline FRec make_frec(f64 x, f64 y, f64 z, f64 h, f64 vx, f64 vy, f64 vz,
+                                         f64 u, f64 P, f64 cs, f64 omega, f64 mass) {
+            const f64 Rker2 = K::Rkern * K::Rkern;
+            FRec r;
+            r.x = x, r.y = y, r.z = z, r.h = h;
+            r.vx = vx, r.vy = vy, r.vz = vz, r.u = u;
+            r.P = P, r.cs = cs, r.omega = omega;
+            r.rho     = rho_h(mass, h, (f64) K::hfactd);
+            r.inv_sub = inv_sat_zero((r.rho * r.rho) * omega);
+            r.romega  = r.rho * omega;
+            r.hR2     = h * h * Rker2;
+            r.pad     = 0;
+            return r;
+        }
+        template<class K>
+        __device__ inline FRec32 make_frec32(f64 x, f64 y, f64 z, f64 h, f64 vx, f64 vy, f64 vz,
+                                             f64 u, f64 P, f64 cs, f64 omega, f64 mass) {
+            FRec32 r;
+            r.x = x, r.y = y, r.z = z;
+            r.h = (f32) h, r.vx = (f32) vx, r.vy = (f32) vy, r.vz = (f32) vz, r.u = (f32) u;
+            r.P = (f32) P, r.cs = (f32) cs;
+            f32 om       = (f32) omega;
+            r.rho        = rho_h((f32) mass, r.h, (f32) K::hfactd);
+            r.inv_sub    = 1.f / ((r.rho * r.rho) * om);
+            r.inv_romega = 1.f / (om * r.rho);
+            return r;
+        }
+
+        // record sources: packed records, or the SoA arrays (reference-layout entry points)
+        template<class Rec>
+        struct RecSrc {
+            const Rec *__restrict rec;
+            __device__ Rec get(u32 b) const { return rec[b]; }
+        };
+        template<class K, class Rec>
+        struct SoASrc {
+            const f64 *__restrict x, *__restrict y, *__restrict z, *__restrict vx, *__restrict vy,
+                *__restrict vz, *__restrict h, *__restrict omega, *__restrict u, *__restrict P,
+                *__restrict cs;
+            f64 mass;
+            __device__ Rec get(u32 b) const {
+                if constexpr (sizeof(Rec) == sizeof(FRec))
+                    return make_frec<K>(x[b], y[b], z[b], h[b], vx[b], vy[b], vz[b], u[b], P[b],
+                                        cs[b], omega[b], mass);
+                else
+                    return make_frec32<K>(x[b], y[b], z[b], h[b], vx[b], vy[b], vz[b], u[b], P[b],
+                                          cs[b], omega[b], mass);
             }
-            key   = wave_reduce(key, umin64);
-            npair = wave_reduce(npair, [](u64 p, u64 q) { return p + q; });
-            if (__lane_id() == 0) {
-                atomic_min_key(dt_min, key);
-                if (pair_count) atomicAdd((unsigned long long *) pair_count, npair);
+        };
+
+        // EOS on the merged set fused with the record build: record s takes particle perm[s]
+        template<class K, bool MIXED>
+        __global__ void k_pack_force(u32 nm, const u32 *__restrict perm, const f64 *__restrict x,
+                                     const f64 *__restrict y, const f64 *__restrict z,
+                                     const f64 *__restrict h, const f64 *__restrict vx,
+                                     const f64 *__restrict vy, const f64 *__restrict vz,
+                                     const f64 *__restrict u, const f64 *__restrict omega,
+                                     f64 *__restrict P, f64 *__restrict cs, void *__restrict rec,
+                                     f64 mass, EosArgs e) {
+            u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+            if (s >= nm) return;
+            u32 b  = perm ? perm[s] : s;
+            f64 hb = h[b], ub = u[b], xb = x[b], yb = y[b], zb = z[b];
+            f64 rho = rho_h(mass, hb, (f64) K::hfactd);
+            f64 p, c;
+            eos_eval(e, rho, ub, xb, yb, zb, p, c);
+            P[b]  = p;
+            cs[b] = c;
+            if constexpr (MIXED)
+                ((FRec32 *) rec)[s] =
+                    make_frec32<K>(xb, yb, zb, hb, vx[b], vy[b], vz[b], ub, p, c, omega[b], mass);
+            else
+                ((FRec *) rec)[s] =
+                    make_frec<K>(xb, yb, zb, hb, vx[b], vy[b], vz[b], ub, p, c, omega[b], mass);
+        }
+
+        struct ForceOut {
+            f64 *ax, *ay, *az, *du, *ax_old, *ay_old, *az_old, *du_old, *vsig, *dtc;
+        };
+
+        // a_old/du_old save, outputs, CFL dt (ComputeCFLCourant.hpp, ComputeCFLForce.hpp:50-62)
+        __device__ inline f64 force_store(const ForceOut &o, u32 a, f64 fx, f64 fy, f64 fz,
+                                          f64 tdu, f64 vsig_max, f64 h_a, const ForceArgs &fa) {
+            o.ax_old[a] = o.ax[a];
+            o.ay_old[a] = o.ay[a];
+            o.az_old[a] = o.az[a];
+            o.du_old[a] = o.du[a];
+            o.ax[a]     = fx;
+            o.ay[a]     = fy;
+            o.az[a]     = fz;
+            o.du[a]     = tdu;
+            o.vsig[a]   = vsig_max;
+            f64 dt_c    = fa.C_cour * h_a / vsig_max;
+            f64 abs_a   = sqrt(fx * fx + fy * fy + fz * fz);
+            f64 dt_f    = fa.C_force * sqrt(h_a / abs_a);
+            f64 cfl_dt  = fmin(__builtin_inf(), dt_c);
+            cfl_dt      = fmin(cfl_dt, dt_f);
+            o.dtc[a]    = cfl_dt;
+            return cfl_dt;
+        }
+
+        // ------------------------------------------------------------- force (FP64)
+        // UpdateDerivs.cpp:88-288 (constant AV) fused with the CFL signal speed
+        // (Solver.cpp:2749-2783).  G lanes per particle stride through its row.
+        template<class K, int G, class Src>
+        __global__ void __launch_bounds__(BLK)
+            k_force(u32 n, Rows r, Src src, ForceOut o, ForceArgs fa, u64 *__restrict part) {
+            using KK = Kern<K, f64>;
+            __shared__ u64 lds[2][BLK / WAVE];
+            const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
+            const u32 lg   = threadIdx.x % G;
+            const bool act = j < n;
+            u64 npair = 0;
+            f64 fx = 0, fy = 0, fz = 0, tdu = 0, vsig_max = 0;
+            u32 row = 0;
+            FRec A{};
+            if (act) {
+                row             = r.sel ? r.sel[j] : j;
+                A               = src.get(row);
+                const f64 pmass = fa.mass;
+                const f64 h_a = A.h, xa = A.x, ya = A.y, za = A.z;
+                const f64 vxa = A.vx, vya = A.vy, vza = A.vz;
+                const f64 P_a = A.P, u_a = A.u, cs_a = A.cs;
+                const f64 rho_a = A.rho;
+                const f64 rho_a_inv = 1. / rho_a;
+                const f64 omega_a_rho_a_inv = 1 / (A.omega * rho_a);
+                const f64 ia_fac = A.inv_sub;
+                const f64 ca     = A.hR2;
+                (void) h_a;
+                u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+                // group-uniform trip count; the sums take the terms in list order
+                for (u32 base = k0; base < k1; base += G) {
+                    u32 k  = base + lg;
+                    f64 tx = 0, ty = 0, tz = 0, tu1 = 0, tu2 = 0;
+                    if (k < k1) {
+                        const FRec B = src.get(r.ids[k]);
+                        f64 dx = xa - B.x, dy = ya - B.y, dz = za - B.z;
+                        f64 rab2 = dx * dx + dy * dy + dz * dz;
+                        if (!(rab2 > ca && rab2 > B.hR2)) {
+                            npair++;
+                            f64 rab   = sqrt(rab2);
+                            f64 rho_b = B.rho;
+                            f64 Fab_a = KK::dW(rab, A.h), Fab_b = KK::dW(rab, B.h);
+                            f64 vabx = vxa - B.vx, vaby = vya - B.vy, vabz = vza - B.vz;
+                            f64 inv  = inv_sat_positive(rab);
+                            f64 rux = dx * inv, ruy = dy * inv, ruz = dz * inv;
+                            f64 v_ab_r_ab = vabx * rux + vaby * ruy + vabz * ruz;
+                            f64 abs_v     = fabs(v_ab_r_ab);
+                            f64 vsig_a = fa.alpha_AV * cs_a + fa.beta_AV * abs_v;
+                            f64 vsig_b = fa.alpha_AV * B.cs + fa.beta_AV * abs_v;
+                            f64 rho_avg = (rho_a + rho_b) * 0.5;
+                            f64 vsig_u  = sqrt(fabs(P_a - B.P) / rho_avg);
+                            f64 qa_ab = fmax(-0.5 * rho_a * vsig_a * v_ab_r_ab, 0.);
+                            f64 qb_ab = fmax(-0.5 * rho_b * vsig_b * v_ab_r_ab, 0.);
+                            f64 AV_P_a = P_a + qa_ab, AV_P_b = B.P + qb_ab;
+                            f64 nax = rux * Fab_a, nay = ruy * Fab_a, naz = ruz * Fab_a;
+                            f64 nbx = rux * Fab_b, nby = ruy * Fab_b, nbz = ruz * Fab_b;
+                            f64 ia = AV_P_a * ia_fac;
+                            f64 ib = AV_P_b * B.inv_sub;
+                            f64 mm = -pmass;
+                            tx     = mm * (ia * nax + ib * nbx);
+                            ty     = mm * (ia * nay + ib * nby);
+                            tz     = mm * (ia * naz + ib * nbz);
+                            tu1    = AV_P_a * (omega_a_rho_a_inv * rho_a_inv) * pmass *
+                                  (vabx * nax + vaby * nay + vabz * naz);
+                            tu2 = pmass * fa.alpha_u * vsig_u * (u_a - B.u) * 0.5 *
+                                  (Fab_a * omega_a_rho_a_inv + Fab_b / B.romega);
+                            // CFL signal speed: r_ab / |r_ab|, alpha 1, beta 2
+                            f64 r2x = dx / rab, r2y = dy / rab, r2z = dz / rab;
+                            if (rab < 1e-9) r2x = r2y = r2z = 0;
+                            f64 vdot = vabx * r2x + vaby * r2y + vabz * r2z;
+                            vsig_max = fmax(vsig_max, 1.0 * cs_a + 2.0 * fabs(vdot));
+                        }
+                    }
+                    // a skipped entry adds +0.0, which leaves a sum that starts at +0.0
+                    // unchanged (it can never become -0.0 under round-to-nearest)
+                    fx  = ordered_fold<G>(fx, tx);
+                    fy  = ordered_fold<G>(fy, ty);
+                    fz  = ordered_fold<G>(fz, tz);
+                    tdu = ordered_fold2<G>(tdu, tu1, tu2);
+                }
+            }
+            vsig_max = group_max<G>(vsig_max);
+            u64 key  = ORDERED_POS_INF;
+            if (act && lg == 0) {
+                u32 a = r.outi ? r.outi[row] : row;
+                key   = f64_to_ordered(force_store(o, a, fx, fy, fz, tdu, vsig_max, A.h, fa));
+            }
+            key   = block_reduce(key, umin64, lds[0]);
+            npair = block_reduce(npair, [](u64 p, u64 q) { return p + q; }, lds[1]);
+            if (threadIdx.x == 0) {
+                part[blockIdx.x]             = key;
+                part[gridDim.x + blockIdx.x] = npair;
             }
         }
 
         // ------------------------------------------------------------- force (mixed)
-        template<class K>
+        // r_ab and the neighbour criterion in FP64, pair terms in FP32, Kahan-compensated
+        // per-lane sums combined across the group in FP64
+        template<class K, int G, class Src>
         __global__ void __launch_bounds__(BLK)
-            k_force_mixed(u32 n, const u32 *__restrict cnt, const u32 *__restrict offset,
-                          const u32 *__restrict ids, const f64 *__restrict x,
-                          const f64 *__restrict y, const f64 *__restrict z,
-                          const f64 *__restrict vx, const f64 *__restrict vy,
-                          const f64 *__restrict vz, const f64 *__restrict h,
-                          const f64 *__restrict omega, const f64 *__restrict u,
-                          const f64 *__restrict P, const f64 *__restrict cs, f64 *__restrict ax,
-                          f64 *__restrict ay, f64 *__restrict az, f64 *__restrict du,
-                          f64 *__restrict ax_old, f64 *__restrict ay_old,
-                          f64 *__restrict az_old, f64 *__restrict du_old,
-                          f64 *__restrict vsig_out, f64 *__restrict dt_out, ForceArgs fa,
-                          u64 *__restrict dt_min, u64 *__restrict pair_count) {
+            k_force_mixed(u32 n, Rows r, Src src, const f64 *__restrict h64, ForceOut o,
+                          ForceArgs fa, u64 *__restrict part) {
             using KF = Kern<K, f32>;
-            u32 a    = blockIdx.x * BLK + threadIdx.x;
-            u64 key  = ORDERED_POS_INF;
+            __shared__ u64 lds[2][BLK / WAVE];
+            const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
+            const u32 lg   = threadIdx.x % G;
+            const bool act = j < n;
+            const f64 Rker2 = K::Rkern * K::Rkern;
             u64 npair = 0;
-            if (a < n) {
+            f32 sx = 0, cx = 0, sy = 0, cy = 0, sz = 0, cz = 0, su = 0, cu = 0, vsig_max = 0;
+            u32 row = 0, a = 0;
+            f64 h_a64 = 0;
+            if (act) {
+                row = r.sel ? r.sel[j] : j;
+                a   = r.outi ? r.outi[row] : row;
+                const FRec32 A = src.get(row);
+                h_a64 = h64[a];
                 const f32 pmass = (f32) fa.mass;
-                const f64 Rker2 = K::Rkern * K::Rkern;
-                const f32 hfact = (f32) K::hfactd;
-                f64 h_a64 = h[a];
-                f32 h_a   = (f32) h_a64;
-                f64 xa = x[a], ya = y[a], za = z[a];
-                f32 vxa = (f32) vx[a], vya = (f32) vy[a], vza = (f32) vz[a];
-                f32 P_a = (f32) P[a], omega_a = (f32) omega[a];
-                f32 u_a = (f32) u[a];
-                f32 rho_a = rho_h(pmass, h_a, hfact);
-                f32 rho_a_sq = rho_a * rho_a;
-                f32 rho_a_inv = 1.f / rho_a;
-                f32 omega_a_rho_a_inv = 1.f / (omega_a * rho_a);
-                f32 inv_sub_a = 1.f / (rho_a_sq * omega_a);
-                f32 cs_a = (f32) cs[a];
-                f32 alpha_AV = (f32) fa.alpha_AV, beta_AV = (f32) fa.beta_AV,
-                    alpha_u = (f32) fa.alpha_u;
-                f32 sx = 0, cx = 0, sy = 0, cy = 0, sz = 0, cz = 0, su = 0, cu = 0;
-                f32 vsig_max = 0;
-                f64 ca = h_a64 * h_a64 * Rker2;
-                u32 k0 = offset[a], k1 = k0 + cnt[a];
-                for (u32 k = k0; k < k1; k++) {
-                    u32 b    = ids[k];
-                    f64 dx   = xa - x[b], dy = ya - y[b], dz = za - z[b];
+                const f32 h_a = A.h;
+                const f32 vxa = A.vx, vya = A.vy, vza = A.vz, P_a = A.P, u_a = A.u, cs_a = A.cs;
+                const f32 rho_a = A.rho, rho_a_inv = 1.f / rho_a;
+                const f32 omega_a_rho_a_inv = A.inv_romega, inv_sub_a = A.inv_sub;
+                const f32 alpha_AV = (f32) fa.alpha_AV, beta_AV = (f32) fa.beta_AV,
+                          alpha_u = (f32) fa.alpha_u;
+                const f64 ca = h_a64 * h_a64 * Rker2;
+                u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+                for (u32 k = k0 + lg; k < k1; k += G) {
+                    const FRec32 B = src.get(r.ids[k]);
+                    f64 dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
                     f64 rab2 = dx * dx + dy * dy + dz * dz;
-                    f64 h_b64 = h[b];
-                    if (rab2 > ca && rab2 > h_b64 * h_b64 * Rker2) continue;
+                    f64 hb   = (f64) B.h;
+                    if (rab2 > ca && rab2 > hb * hb * Rker2) continue;
                     npair++;
                     f32 fdx = (f32) dx, fdy = (f32) dy, fdz = (f32) dz;
                     f32 rab = sqrtf((f32) rab2);
-                    f32 h_b = (f32) h_b64;
-                    f32 vabx = vxa - (f32) vx[b], vaby = vya - (f32) vy[b],
-                        vabz = vza - (f32) vz[b];
-                    f32 u_b = (f32) u[b];
-                    f32 rho_b = rho_h(pmass, h_b, hfact);
-                    f32 P_b = (f32) P[b], omega_b = (f32) omega[b], cs_b = (f32) cs[b];
-                    f32 Fab_a = KF::dW(rab, h_a), Fab_b = KF::dW(rab, h_b);
+                    f32 vabx = vxa - B.vx, vaby = vya - B.vy, vabz = vza - B.vz;
+                    f32 rho_b = B.rho;
+                    f32 Fab_a = KF::dW(rab, h_a), Fab_b = KF::dW(rab, B.h);
                     f32 inv = (rab >= 1e-9f) ? 1.f / rab : 0.f;
                     f32 rux = fdx * inv, ruy = fdy * inv, ruz = fdz * inv;
                     f32 v_ab_r_ab = vabx * rux + vaby * ruy + vabz * ruz;
                     f32 abs_v     = fabsf(v_ab_r_ab);
                     f32 vsig_a = alpha_AV * cs_a + beta_AV * abs_v;
-                    f32 vsig_b = alpha_AV * cs_b + beta_AV * abs_v;
+                    f32 vsig_b = alpha_AV * B.cs + beta_AV * abs_v;
                     f32 rho_avg = (rho_a + rho_b) * 0.5f;
-                    f32 vsig_u  = sqrtf(fabsf(P_a - P_b) / rho_avg);
+                    f32 vsig_u  = sqrtf(fabsf(P_a - B.P) / rho_avg);
                     f32 qa_ab = fmaxf(-0.5f * rho_a * vsig_a * v_ab_r_ab, 0.f);
                     f32 qb_ab = fmaxf(-0.5f * rho_b * vsig_b * v_ab_r_ab, 0.f);
-                    f32 AV_P_a = P_a + qa_ab, AV_P_b = P_b + qb_ab;
+                    f32 AV_P_a = P_a + qa_ab, AV_P_b = B.P + qb_ab;
                     f32 ia = AV_P_a * inv_sub_a * Fab_a;
-                    f32 ib = AV_P_b * (1.f / ((rho_b * rho_b) * omega_b)) * Fab_b;
+                    f32 ib = AV_P_b * B.inv_sub * Fab_b;
                     f32 s  = -pmass * (ia + ib);
                     kahan_add(sx, cx, s * rux);
                     kahan_add(sy, cy, s * ruy);
                     kahan_add(sz, cz, s * ruz);
-                    f32 t1 = AV_P_a * (omega_a_rho_a_inv * rho_a_inv) * pmass * Fab_a *
-                             (vabx * rux + vaby * ruy + vabz * ruz);
-                    f32 t2 = pmass * alpha_u * vsig_u * (u_a - u_b) * 0.5f *
-                             (Fab_a * omega_a_rho_a_inv + Fab_b / (rho_b * omega_b));
+                    f32 t1 = AV_P_a * (omega_a_rho_a_inv * rho_a_inv) * pmass * Fab_a * v_ab_r_ab;
+                    f32 t2 = pmass * alpha_u * vsig_u * (u_a - B.u) * 0.5f *
+                             (Fab_a * omega_a_rho_a_inv + Fab_b * B.inv_romega);
                     kahan_add(su, cu, t1 + t2);
                     vsig_max = fmaxf(vsig_max, cs_a + 2.f * abs_v);
                 }
-                f64 fx = (f64) sx - (f64) cx, fy = (f64) sy - (f64) cy, fz = (f64) sz - (f64) cz;
-                f64 tdu = (f64) su - (f64) cu;
-                ax_old[a] = ax[a];
-                ay_old[a] = ay[a];
-                az_old[a] = az[a];
-                du_old[a] = du[a];
-                ax[a]     = fx;
-                ay[a]     = fy;
-                az[a]     = fz;
-                du[a]     = tdu;
-                f64 vs    = (f64) vsig_max;
-                vsig_out[a] = vs;
-                f64 dt_c   = fa.C_cour * h_a64 / vs;
-                f64 abs_a  = sqrt(fx * fx + fy * fy + fz * fz);
-                f64 dt_f   = fa.C_force * sqrt(h_a64 / abs_a);
-                f64 cfl_dt = fmin(__builtin_inf(), dt_c);
-                cfl_dt     = fmin(cfl_dt, dt_f);
-                dt_out[a]  = cfl_dt;
-                key        = f64_to_ordered(cfl_dt);
             }
-            key   = wave_reduce(key, umin64);
-            npair = wave_reduce(npair, [](u64 p, u64 q) { return p + q; });
-            if (__lane_id() == 0) {
-                atomic_min_key(dt_min, key);
-                if (pair_count) atomicAdd((unsigned long long *) pair_count, npair);
+            f64 fx   = group_sum<G>((f64) sx - (f64) cx);
+            f64 fy   = group_sum<G>((f64) sy - (f64) cy);
+            f64 fz   = group_sum<G>((f64) sz - (f64) cz);
+            f64 tdu  = group_sum<G>((f64) su - (f64) cu);
+            f64 vs   = group_max<G>((f64) vsig_max);
+            u64 key  = ORDERED_POS_INF;
+            if (act && lg == 0)
+                key = f64_to_ordered(force_store(o, a, fx, fy, fz, tdu, vs, h_a64, fa));
+            key   = block_reduce(key, umin64, lds[0]);
+            npair = block_reduce(npair, [](u64 p, u64 q) { return p + q; }, lds[1]);
+            if (threadIdx.x == 0) {
+                part[blockIdx.x]             = key;
+                part[gridDim.x + blockIdx.x] = npair;
+            }
+        }
+
+        // ------------------------------------------------------------- partial reductions
+        // slot k of the nb block partials is reduced in a fixed order (fixed chunks, then one
+        // block), so the result, an FP64 sum included, does not depend on scheduling
+        template<int OP>
+        __device__ inline u64 red_combine(u64 p, u64 q) {
+            if constexpr (OP == RED_MAX_KEY) return umax64(p, q);
+            else if constexpr (OP == RED_MIN_KEY) return umin64(p, q);
+            else if constexpr (OP == RED_SUM_U64) return p + q;
+            else {
+                f64 r = __longlong_as_double((long long) p) + __longlong_as_double((long long) q);
+                return (u64) __double_as_longlong(r);
+            }
+        }
+        template<int OP>
+        __device__ inline u64 red_identity() {
+            if constexpr (OP == RED_MAX_KEY) return ORDERED_NEG_INF;
+            else if constexpr (OP == RED_MIN_KEY) return ORDERED_POS_INF;
+            else return 0; // 0 (u64) and +0.0 (f64 bits)
+        }
+        constexpr int FIN_BLK   = 1024;
+        constexpr u32 FIN_CHUNK = 8192; // partials per block of the first pass
+
+        // block b reduces p[b * chunk, min((b + 1) * chunk, n)) into out[b]
+        template<int OP>
+        __device__ inline void fin_block(const u64 *__restrict p, u32 n, u32 chunk,
+                                         u64 *__restrict out, u64 *lds) {
+            u32 i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+            u64 acc = red_identity<OP>();
+            for (u32 i = i0 + threadIdx.x; i < i1; i += FIN_BLK) acc = red_combine<OP>(acc, p[i]);
+            for (int o = 32; o > 0; o >>= 1) acc = red_combine<OP>(acc, __shfl_xor(acc, o, WAVE));
+            if (__lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                u64 r = lds[0];
+                for (int w = 1; w < FIN_BLK / WAVE; w++) r = red_combine<OP>(r, lds[w]);
+                out[blockIdx.x] = r;
+            }
+            __syncthreads();
+        }
+        __global__ void __launch_bounds__(FIN_BLK)
+            k_finalize(u32 nb, u32 chunk, const u64 *__restrict part, u64 stride, RedSpec sp,
+                       u64 *__restrict out, u64 out_stride) {
+            __shared__ u64 lds[FIN_BLK / WAVE];
+            for (int k = 0; k < sp.nslot; k++) {
+                const u64 *p = part + (u64) k * stride;
+                u64 *o       = out + (u64) sp.out_slot[k] * out_stride;
+                switch (sp.op[k]) {
+                case RED_MAX_KEY: fin_block<RED_MAX_KEY>(p, nb, chunk, o, lds); break;
+                case RED_MIN_KEY: fin_block<RED_MIN_KEY>(p, nb, chunk, o, lds); break;
+                case RED_SUM_U64: fin_block<RED_SUM_U64>(p, nb, chunk, o, lds); break;
+                default: fin_block<RED_SUM_F64>(p, nb, chunk, o, lds); break;
+                }
             }
         }
 
@@ -463,7 +696,7 @@ namespace shamx {
                         const f64 *__restrict az, const f64 *__restrict ax_old,
                         const f64 *__restrict ay_old, const f64 *__restrict az_old,
                         f64 *__restrict u, const f64 *__restrict du,
-                        const f64 *__restrict du_old, f64 mass, u64 *__restrict red) {
+                        const f64 *__restrict du_old, f64 mass, u64 *__restrict part) {
             u32 i    = blockIdx.x * BLK + threadIdx.x;
             u64 kmax = ORDERED_NEG_INF;
             f64 vsq = 0, en = 0;
@@ -481,13 +714,16 @@ namespace shamx {
                 vsq    = v0 * v0 + v1 * v1 + v2 * v2;
                 en     = mass * (uu + 0.5 * vsq);
             }
-            kmax = wave_reduce(kmax, umax64);
-            vsq  = wave_reduce(vsq, [](f64 p, f64 q) { return p + q; });
-            en   = wave_reduce(en, [](f64 p, f64 q) { return p + q; });
-            if (__lane_id() == 0) {
-                atomic_max_key(&red[0], kmax);
-                atomicAdd((f64 *) &red[1], vsq);
-                atomicAdd((f64 *) &red[2], en);
+            __shared__ u64 lds[BLK / WAVE];
+            __shared__ f64 ldf[2][BLK / WAVE];
+            auto add = [](f64 p, f64 q) { return p + q; };
+            kmax = block_reduce(kmax, umax64, lds);
+            vsq  = block_reduce(vsq, add, ldf[0]);
+            en   = block_reduce(en, add, ldf[1]);
+            if (threadIdx.x == 0) {
+                part[blockIdx.x]                 = kmax;
+                part[gridDim.x + blockIdx.x]     = (u64) __double_as_longlong(vsq);
+                part[2 * gridDim.x + blockIdx.x] = (u64) __double_as_longlong(en);
             }
         }
 
@@ -575,31 +811,93 @@ namespace shamx {
         }
     } // namespace
 
-    void h_iterate(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
-                   const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
-                   f64 h_evol_max, f64 h_evol_iter_max, u64 *eps_minmax, int mixed) {
+    namespace {
+        // lanes per particle in the neighbour loops: 1, 4 or 16 (SHAMX_PAIR_LANES; the FP64
+        // sums are taken in list order whatever the choice, so results do not depend on it)
+        int pair_lanes() {
+            static const int g = [] {
+                const char *e = std::getenv("SHAMX_PAIR_LANES");
+                int v         = e ? std::atoi(e) : 4;
+                return (v == 1 || v == 4 || v == 16) ? v : 4;
+            }();
+            return g;
+        }
+        template<class F>
+        void dispatch_lanes(F &&f) {
+            switch (pair_lanes()) {
+            case 1: f(std::integral_constant<int, 1>{}); break;
+            case 4: f(std::integral_constant<int, 4>{}); break;
+            case 16: f(std::integral_constant<int, 16>{}); break;
+            default: f(std::integral_constant<int, 4>{}); break;
+            }
+        }
+
+        Rows rows_of(const Csr &c) { return Rows{c.sel, c.outi, c.cnt, c.offset, c.ids}; }
+
+        // partials of slot k at part[k * nb + b]; results to out[sp.out_slot[k]].  Large
+        // counts go through a first pass of FIN_CHUNK-sized chunks whose results go to mid
+        // (sp.nslot * div_up(nb, FIN_CHUNK) words that must not overlap the partials).
+        void finalize(hipStream_t s, u32 nb, const u64 *part, RedSpec sp, u64 *out, u64 *mid) {
+            if (nb <= FIN_CHUNK) {
+                k_finalize<<<1, FIN_BLK, 0, s>>>(nb, nb, part, nb, sp, out, 1);
+            } else {
+                u32 nb2     = div_up(nb, FIN_CHUNK);
+                RedSpec sp1 = sp;
+                for (int k = 0; k < sp.nslot; k++) sp1.out_slot[k] = k;
+                k_finalize<<<nb2, FIN_BLK, 0, s>>>(nb, FIN_CHUNK, part, nb, sp1, mid, nb2);
+                k_finalize<<<1, FIN_BLK, 0, s>>>(nb2, nb2, mid, nb2, sp, out, 1);
+            }
+            SHAMX_HIP(hipGetLastError());
+        }
+        // words of partial storage for nslot slots of nb blocks, followed by the first-pass room
+        size_t part_words(int nslot, u32 nb) { return (size_t) nslot * (nb + div_up(nb, FIN_CHUNK)); }
+    } // namespace
+
+    void h_iterate(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
+                   const f64 *y, const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
+                   f64 h_evol_max, f64 h_evol_iter_max, u64 *eps_minmax, int mixed,
+                   DevArray<u64> &part) {
         if (n == 0) return;
+        u32 nb = div_up((u64) n * pair_lanes(), BLK);
+        part.resize(part_words(2, nb));
+        Rows r = rows_of(c);
         dispatch_kernel(kernel, [&](auto K) {
             using KT = decltype(K);
-            if (mixed)
-                k_h_iterate_mixed<KT><<<div_up(n, BLK), BLK, 0, s>>>(
-                    n, c.cnt, c.offset, c.ids, x, y, z, h_old, h_new, eps, mass, h_evol_max,
-                    h_evol_iter_max, eps_minmax);
-            else
-                k_h_iterate<KT><<<div_up(n, BLK), BLK, 0, s>>>(n, c.cnt, c.offset, c.ids, x, y,
-                                                               z, h_old, h_new, eps, mass,
-                                                               h_evol_max, h_evol_iter_max,
-                                                               eps_minmax);
+            dispatch_lanes([&](auto GC) {
+                constexpr int G = decltype(GC)::value;
+                auto go         = [&](auto src) {
+                    if (mixed)
+                        k_h_iterate<KT, G, true><<<nb, BLK, 0, s>>>(
+                            n, r, src, h_old, h_new, eps, mass, h_evol_max, h_evol_iter_max,
+                            part.data());
+                    else
+                        k_h_iterate<KT, G, false><<<nb, BLK, 0, s>>>(
+                            n, r, src, h_old, h_new, eps, mass, h_evol_max, h_evol_iter_max,
+                            part.data());
+                };
+                if (ps) go(PosRec{ps});
+                else go(PosSoA{x, y, z});
+            });
         });
         SHAMX_HIP(hipGetLastError());
+        finalize(s, nb, part.data(), RedSpec{2, {RED_MAX_KEY, RED_MIN_KEY}, {0, 1}}, eps_minmax,
+                 part.data() + 2 * (size_t) nb);
     }
 
-    void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
-                       const f64 *z, const f64 *h, f64 *omega, f64 mass) {
+    void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
+                       const f64 *y, const f64 *z, const f64 *h, f64 *omega, f64 mass) {
         if (n == 0) return;
+        u32 nb = div_up((u64) n * pair_lanes(), BLK);
+        Rows r = rows_of(c);
         dispatch_kernel(kernel, [&](auto K) {
-            k_omega<decltype(K)><<<div_up(n, BLK), BLK, 0, s>>>(n, c.cnt, c.offset, c.ids, x, y,
-                                                                z, h, omega, mass);
+            using KT = decltype(K);
+            dispatch_lanes([&](auto GC) {
+                constexpr int G = decltype(GC)::value;
+                if (ps)
+                    k_omega<KT, G><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega, mass);
+                else
+                    k_omega<KT, G><<<nb, BLK, 0, s>>>(n, r, PosSoA{x, y, z}, h, omega, mass);
+            });
         });
         SHAMX_HIP(hipGetLastError());
     }
@@ -614,25 +912,68 @@ namespace shamx {
         SHAMX_HIP(hipGetLastError());
     }
 
-    void compute_force(hipStream_t s, int kernel, u32 n, Csr c, const f64 *x, const f64 *y,
-                       const f64 *z, const f64 *vx, const f64 *vy, const f64 *vz, const f64 *h,
-                       const f64 *omega, const f64 *u, const f64 *P, const f64 *cs, f64 *ax,
-                       f64 *ay, f64 *az, f64 *du, f64 *ax_old, f64 *ay_old, f64 *az_old,
-                       f64 *du_old, f64 *vsig, f64 *dt_cfl, ForceArgs fa, u64 *dt_min_key,
-                       u64 *pair_count) {
-        if (n == 0) return;
+    size_t force_record_bytes(int mixed) { return mixed ? sizeof(FRec32) : sizeof(FRec); }
+
+    void pack_force(hipStream_t s, int kernel, u32 nm, const u32 *perm, const f64 *x,
+                    const f64 *y, const f64 *z, const f64 *h, const f64 *vx, const f64 *vy,
+                    const f64 *vz, const f64 *u, const f64 *omega, f64 *P, f64 *cs, void *rec,
+                    f64 mass, EosArgs e, int mixed) {
+        if (nm == 0) return;
         dispatch_kernel(kernel, [&](auto K) {
             using KT = decltype(K);
-            if (fa.mixed)
-                k_force_mixed<KT><<<div_up(n, BLK), BLK, 0, s>>>(
-                    n, c.cnt, c.offset, c.ids, x, y, z, vx, vy, vz, h, omega, u, P, cs, ax, ay,
-                    az, du, ax_old, ay_old, az_old, du_old, vsig, dt_cfl, fa, dt_min_key, pair_count);
+            if (mixed)
+                k_pack_force<KT, true><<<div_up(nm, 256), 256, 0, s>>>(
+                    nm, perm, x, y, z, h, vx, vy, vz, u, omega, P, cs, rec, mass, e);
             else
-                k_force<KT><<<div_up(n, BLK), BLK, 0, s>>>(
-                    n, c.cnt, c.offset, c.ids, x, y, z, vx, vy, vz, h, omega, u, P, cs, ax, ay,
-                    az, du, ax_old, ay_old, az_old, du_old, vsig, dt_cfl, fa, dt_min_key, pair_count);
+                k_pack_force<KT, false><<<div_up(nm, 256), 256, 0, s>>>(
+                    nm, perm, x, y, z, h, vx, vy, vz, u, omega, P, cs, rec, mass, e);
         });
         SHAMX_HIP(hipGetLastError());
+    }
+
+    void compute_force(hipStream_t s, int kernel, u32 n, Csr c, const void *rec, const f64 *x,
+                       const f64 *y, const f64 *z, const f64 *vx, const f64 *vy, const f64 *vz,
+                       const f64 *h, const f64 *omega, const f64 *u, const f64 *P, const f64 *cs,
+                       f64 *ax, f64 *ay, f64 *az, f64 *du, f64 *ax_old, f64 *ay_old,
+                       f64 *az_old, f64 *du_old, f64 *vsig, f64 *dt_cfl, ForceArgs fa,
+                       u64 *dt_min_key, u64 *pair_count, DevArray<u64> &part) {
+        if (n == 0) return;
+        u32 nb = div_up((u64) n * pair_lanes(), BLK);
+        part.resize(part_words(2, nb));
+        Rows r = rows_of(c);
+        ForceOut o{ax, ay, az, du, ax_old, ay_old, az_old, du_old, vsig, dt_cfl};
+        dispatch_kernel(kernel, [&](auto K) {
+            using KT = decltype(K);
+            dispatch_lanes([&](auto GC) {
+                constexpr int G = decltype(GC)::value;
+                if (fa.mixed) {
+                    if (rec)
+                        k_force_mixed<KT, G><<<nb, BLK, 0, s>>>(
+                            n, r, RecSrc<FRec32>{(const FRec32 *) rec}, h, o, fa, part.data());
+                    else
+                        k_force_mixed<KT, G><<<nb, BLK, 0, s>>>(
+                            n, r,
+                            SoASrc<KT, FRec32>{x, y, z, vx, vy, vz, h, omega, u, P, cs, fa.mass},
+                            h, o, fa, part.data());
+                } else {
+                    if (rec)
+                        k_force<KT, G><<<nb, BLK, 0, s>>>(
+                            n, r, RecSrc<FRec>{(const FRec *) rec}, o, fa, part.data());
+                    else
+                        k_force<KT, G><<<nb, BLK, 0, s>>>(
+                            n, r,
+                            SoASrc<KT, FRec>{x, y, z, vx, vy, vz, h, omega, u, P, cs, fa.mass}, o,
+                            fa, part.data());
+                }
+            });
+        });
+        SHAMX_HIP(hipGetLastError());
+        // dt_min and the pair count may live in different buffers: reduce each slot separately;
+        // both calls use the room after the two slots in turn (stream order)
+        u64 *mid = part.data() + 2 * (size_t) nb;
+        finalize(s, nb, part.data(), RedSpec{1, {RED_MIN_KEY}, {0}}, dt_min_key, mid);
+        if (pair_count)
+            finalize(s, nb, part.data() + nb, RedSpec{1, {RED_SUM_U64}, {0}}, pair_count, mid);
     }
 
     void predictor(hipStream_t s, u32 n, f64 dt, f64 *x, f64 *y, f64 *z, f64 *vx, f64 *vy,
@@ -647,11 +988,16 @@ namespace shamx {
     void corrector(hipStream_t s, u32 n, f64 hdt, f64 *vx, f64 *vy, f64 *vz, const f64 *ax,
                    const f64 *ay, const f64 *az, const f64 *ax_old, const f64 *ay_old,
                    const f64 *az_old, f64 *u, const f64 *du, const f64 *du_old, f64 mass,
-                   u64 *red) {
+                   u64 *red, DevArray<u64> &part) {
         if (n == 0) return;
-        k_corrector<<<div_up(n, BLK), BLK, 0, s>>>(n, hdt, vx, vy, vz, ax, ay, az, ax_old, ay_old,
-                                                   az_old, u, du, du_old, mass, red);
+        u32 nb = div_up(n, BLK);
+        part.resize(part_words(3, nb));
+        k_corrector<<<nb, BLK, 0, s>>>(n, hdt, vx, vy, vz, ax, ay, az, ax_old, ay_old, az_old, u,
+                                       du, du_old, mass, part.data());
         SHAMX_HIP(hipGetLastError());
+        finalize(s, nb, part.data(),
+                 RedSpec{3, {RED_MAX_KEY, RED_SUM_F64, RED_SUM_F64}, {0, 1, 2}}, red,
+                 part.data() + 3 * (size_t) nb);
     }
 
     void max_key_f64(hipStream_t s, u32 n, const f64 *v, u64 *key) {

@@ -6,9 +6,11 @@
 //   stable onesweep LSD radix sort        (replaces bitonicSort_updated_usm.cpp:284-396)
 //   leaf reduction                        (reduction_alg.cpp:62-585, MortonReducedSet.cpp:24-73)
 //   Karras 2012 radix tree                (KarrasRadixTree.cpp:45-180)
-//   leaf AABB + max-h field, propagated bottom-up in one pass with arrival counters
-//   (replaces the tree_depth = 32 full passes of KarrasRadixTreeAABB.cpp:32-74 and
-//    KarrasRadixTreeField.hpp:133-169; min / max are exact so the result is identical).
+//   leaf AABB + max-h field, then the internal cells level by level from the deepest up,
+//   one launch per depth (replaces the tree_depth = 32 full passes of
+//   KarrasRadixTreeAABB.cpp:32-74 and KarrasRadixTreeField.hpp:133-169; min / max are exact
+//   so the result is identical).  A bottom-up climb with arrival counters would need a
+//   device-scope fence per step, which on a multi-XCD GPU writes back and invalidates L2.
 //
 // Node records: every cell (I internal cells then L leaves) owns one 64-byte record
 //   [0..2] aabb min, [3..5] aabb max, [6] rint = max h * htol,
@@ -349,13 +351,11 @@ namespace shamx {
             if (iu == 0) parent[0] = NO_PARENT;
         }
 
-        // leaf boxes + max h, then climb: the second child to arrive at a parent merges
-        __global__ void k_leaf_aabb_climb(u32 L, u32 I, const u32 *__restrict map,
-                                          const u32 *__restrict sort_index,
-                                          const f64 *__restrict x, const f64 *__restrict y,
-                                          const f64 *__restrict z, const f64 *__restrict h,
-                                          f64 htol, const u32 *__restrict parent,
-                                          u32 *__restrict visits, f64 *__restrict nodes) {
+        // leaf boxes + max h of the leaf's sorted range
+        __global__ void k_leaf_boxes(u32 L, u32 I, const u32 *__restrict map,
+                                     const u32 *__restrict sort_index, const f64 *__restrict x,
+                                     const f64 *__restrict y, const f64 *__restrict z,
+                                     const f64 *__restrict h, f64 htol, f64 *__restrict nodes) {
             u32 l = blockIdx.x * blockDim.x + threadIdx.x;
             if (l >= L) return;
             const f64 mx = 1.7976931348623157e308;
@@ -385,26 +385,48 @@ namespace shamx {
             // so scaling before the max-propagation gives the identical doubles
             rec[6] = hm * htol;
             node_set_range(nodes, node, s0, s1);
-            if (I == 0) return;
-            while (true) {
-                u32 p = parent[node];
-                if (p >= I) break; // root (NO_PARENT) or a corrupted link: stop climbing
-                __threadfence();
-                u32 old = atomicAdd(&visits[p], 1u);
-                if (old == 0) break;
-                __threadfence();
-                u32 lt, rt;
-                node_children(nodes, p, lt, rt);
-                if (lt >= I + L || rt >= I + L) break;
-                const f64 *A = nodes + (u64) lt * NODE_F64;
-                const f64 *B = nodes + (u64) rt * NODE_F64;
-                f64 *P       = nodes + (u64) p * NODE_F64;
-#pragma unroll
-                for (int k = 0; k < 3; k++) P[k] = fmin(A[k], B[k]);
-#pragma unroll
-                for (int k = 3; k < 7; k++) P[k] = fmax(A[k], B[k]);
-                node = p;
+        }
+
+        // depth of every internal node (root 0) by walking the parent links; max depth to
+        // maxd, broken links or depth >= DEPTH_MAX to err
+        constexpr u32 DEPTH_MAX = 1024;
+        __global__ void k_node_depth(u32 I, const u32 *__restrict parent, u16 *__restrict depth,
+                                     u32 *__restrict maxd, u32 *__restrict err) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            u32 d = 0;
+            if (i < I) {
+                u32 p = parent[i];
+                while (p < I && d < DEPTH_MAX) {
+                    d++;
+                    p = parent[p];
+                }
+                if (p != NO_PARENT) atomicOr(err, 1u);
+                depth[i] = (u16) min(d, DEPTH_MAX - 1);
             }
+            d = wave_reduce(d, [](u32 p, u32 q) { return p > q ? p : q; });
+            if (__lane_id() == 0) atomicMax(maxd, d);
+        }
+
+        // boxes and h max of the internal nodes at depth d from their two children (all of
+        // them complete: leaves, or internal nodes of depth d + 1 merged by the previous launch).
+        // min / max are exact, so this equals any other merge order bit for bit.
+        __global__ void k_merge_level(u32 I, u32 L, u32 d, const u16 *__restrict depth,
+                                      f64 *__restrict nodes, u32 *__restrict err) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= I || depth[i] != d) return;
+            u32 lt, rt;
+            node_children(nodes, i, lt, rt);
+            if (lt >= I + L || rt >= I + L) {
+                atomicOr(err, 2u);
+                return;
+            }
+            const f64 *A = nodes + (u64) lt * NODE_F64;
+            const f64 *B = nodes + (u64) rt * NODE_F64;
+            f64 *P       = nodes + (u64) i * NODE_F64;
+#pragma unroll
+            for (int k = 0; k < 3; k++) P[k] = fmin(A[k], B[k]);
+#pragma unroll
+            for (int k = 3; k < 7; k++) P[k] = fmax(A[k], B[k]);
         }
     } // namespace
 
@@ -516,14 +538,14 @@ namespace shamx {
         k_leaf_codes<<<div_up(L, 256), 256, 0, s>>>(n, L, t.reduc_map.data(), t.sorted.data(),
                                                    t.leaf_codes.data());
         u32 I = t.I;
-        for (auto *a : {&t.lchild, &t.rchild, &t.endrange, &t.visits}) a->resize(std::max<u32>(I, 1));
+        for (auto *a : {&t.lchild, &t.rchild, &t.endrange}) a->resize(std::max<u32>(I, 1));
+        t.depth.resize(std::max<u32>(I, 1));
         t.lflag.resize(std::max<u32>(I, 1));
         t.rflag.resize(std::max<u32>(I, 1));
         t.parent.resize((size_t) I + L);
         t.nodes.resize((size_t) (I + L) * NODE_F64);
         fill_u32(s, t.parent.data(), NO_PARENT, (size_t) I + L);
         if (I > 0) {
-            fill_u32(s, t.visits.data(), 0, I);
             k_karras<<<div_up(I, 256), 256, 0, s>>>(L, t.leaf_codes.data(), t.lchild.data(),
                                                    t.rchild.data(), t.lflag.data(),
                                                    t.rflag.data(), t.endrange.data(),
@@ -531,10 +553,22 @@ namespace shamx {
         } else {
             fill_u32(s, t.parent.data(), NO_PARENT, 1);
         }
-        k_leaf_aabb_climb<<<div_up(L, 128), 128, 0, s>>>(L, I, t.reduc_map.data(),
-                                                        t.sort_index.data(), x, y, z, h, htol,
-                                                        t.parent.data(), t.visits.data(),
-                                                        t.nodes.data());
+        k_leaf_boxes<<<div_up(L, 256), 256, 0, s>>>(L, I, t.reduc_map.data(), t.sort_index.data(),
+                                                    x, y, z, h, htol, t.nodes.data());
+        if (I > 0) {
+            // internal boxes level by level, deepest first (the kernel boundary between two
+            // levels orders the writes of one level before the reads of the next)
+            u32 *maxd = t.counters.data() + 4, *err = t.counters.data() + 5;
+            k_node_depth<<<div_up(I, 256), 256, 0, s>>>(I, t.parent.data(), t.depth.data(), maxd,
+                                                       err);
+            u32 hd[2];
+            SHAMX_HIP(hipMemcpyAsync(hd, maxd, 2 * sizeof(u32), hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            if (hd[1]) throw std::runtime_error("tree build: broken parent links");
+            for (u32 d = hd[0] + 1; d-- > 0;)
+                k_merge_level<<<div_up(I, 256), 256, 0, s>>>(I, L, d, t.depth.data(),
+                                                            t.nodes.data(), err);
+        }
         SHAMX_HIP(hipGetLastError());
     }
 
