@@ -75,6 +75,30 @@ def mixed_error(cfg, g64):
     return relL2(out[1][0], out[0][0]), relL2(out[1][1], out[0][1])
 
 
+def copy_probe(nbytes=512 << 20):
+    """One device-to-device copy of known size: rocprofv3 --pmc runs of this script read its
+    FETCH_SIZE / WRITE_SIZE to scale the counters of the step's kernels to bytes."""
+    import torch
+
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: {nbytes} B read + {nbytes} B written")
+
+
+def load_traffic(workload):
+    """HBM bytes per launch measured by rocprofv3 (profiles/traffic.json), for this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    return t.get("kernels", {}) if t.get("workload") == workload else {}
+
+
 def cpu_info():
     info = {"model": platform.processor() or "unknown"}
     try:
@@ -115,6 +139,9 @@ def main():
     ap.add_argument("--n-side", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 512 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU design "
@@ -127,19 +154,28 @@ def main():
     _lib.lib()
     cfg, fields = setups.sedov(args.n_side)
     n = fields["x"].size
+    workload = f"sedov_{args.n_side}^3_M6_periodic"
+    if args.copy_probe:
+        copy_probe()
     log(f"[bench] Sedov {args.n_side}^3 N={n} FP64: warmup {args.warmup}, steps {args.steps}")
     g64, logs64, el64 = run_gpu(cfg, fields, args.steps, args.warmup)
     log(f"[bench] FP64 {el64 / args.steps * 1e3:.2f} ms/step")
-    cfg_m = dict(cfg, mixed_precision=1)
-    log("[bench] mixed")
-    gm, logsm, elm = run_gpu(cfg_m, fields, args.steps, args.warmup)
-    log(f"[bench] mixed {elm / args.steps * 1e3:.2f} ms/step")
-    err_a, err_du = mixed_error(cfg, g64)
     e64 = [lg["energy"] for lg in logs64]
-    em = [lg["energy"] for lg in logsm]
-
     value = n * args.steps / el64
-    value_m = n * args.steps / elm
+    mixed = None
+    if not args.no_mixed:
+        cfg_m = dict(cfg, mixed_precision=1)
+        log("[bench] mixed")
+        gm, logsm, elm = run_gpu(cfg_m, fields, args.steps, args.warmup)
+        log(f"[bench] mixed {elm / args.steps * 1e3:.2f} ms/step")
+        err_a, err_du = mixed_error(cfg, g64)
+        em = [lg["energy"] for lg in logsm]
+        value_m = n * args.steps / elm
+        mixed = {"value": value_m, "ms_per_step": elm / args.steps * 1e3,
+                 "speedup_vs_fp64": value_m / value, "rel_l2_accel_vs_fp64": err_a,
+                 "rel_l2_du_vs_fp64": err_du,
+                 "energy_drift_rel": abs(em[-1] - em[0]) / abs(em[0])}
+        gm.close()
     # roofline of the dominant kernels, from the library's HIP-event timings on its stream
     t_sort = sum(lg["t_sort"] for lg in logs64)
     n_sorts = sum(lg["n_sorts"] for lg in logs64)
@@ -152,13 +188,15 @@ def main():
     phases = {k: sum(lg[k] for lg in logs64) / args.steps * 1e3
               for k in ("t_ghost", "t_tree", "t_sort", "t_neigh", "t_hiter", "t_omega", "t_eos",
                         "t_force", "t_corr")}
+    traffic = load_traffic(workload)
     kernels = {
         "sort": {"bound": "hbm", "achieved": sort_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": sort_gbs / HBM_PEAK_GBS, "traffic": None,
+                 "frac": sort_gbs / HBM_PEAK_GBS, "traffic": traffic.get("sort"),
                  "avg_ms": t_sort / max(n_sorts, 1) * 1e3,
                  "algorithmic": f"{SORT_BYTES_PER_KEY} B x {nm} keys"},
         "force": {"bound": "mfma", "achieved": force_tfs, "peak": FP64_PEAK_TFS,
-                  "unit": "TFLOP/s", "frac": force_tfs / FP64_PEAK_TFS, "traffic": None,
+                  "unit": "TFLOP/s", "frac": force_tfs / FP64_PEAK_TFS,
+                  "traffic": traffic.get("force"),
                   "avg_ms": t_force / max(n_fe, 1) * 1e3,
                   "algorithmic": f"{FORCE_FLOPS_PER_PAIR} FP64 flop x {pairs} interacting pairs",
                   "note": "FP64 VALU loop; the MI355X FP64 vector peak equals its FP64 matrix "
@@ -173,20 +211,16 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": el64 / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"sedov_{args.n_side}^3_M6_periodic", "n_particles": n,
+        "config": {"workload": workload, "n_particles": n,
                    "n_merged_with_ghosts": nm, "kernel": "M6", "gamma": 5 / 3,
                    "av": "constant(1,1,2)", "parallelism": "single_gpu"},
-        "mixed": {"value": value_m, "ms_per_step": elm / args.steps * 1e3,
-                  "speedup_vs_fp64": value_m / value, "rel_l2_accel_vs_fp64": err_a,
-                  "rel_l2_du_vs_fp64": err_du,
-                  "energy_drift_rel": abs(em[-1] - em[0]) / abs(em[0])},
+        "mixed": mixed,
         "fp64_energy_drift_rel": abs(e64[-1] - e64[0]) / abs(e64[0]),
         "phase_ms_fp64": phases,
         "roofline": roof,
         "kernels": kernels,
     }
     g64.close()
-    gm.close()
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline (oracle, {args.cpu_threads} threads)")
         out["cpu_baseline"] = cpu_baseline(cfg, fields, args.cpu_threads)

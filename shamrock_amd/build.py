@@ -22,7 +22,7 @@ FLAGS = [
     "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-command-line-argument",
+    "-Wno-unused-command-line-argument", "-Wshadow",
 ]
 
 

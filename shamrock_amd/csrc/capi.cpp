@@ -171,6 +171,11 @@ struct shamx_solver {
         return Csr{nb.cnt.data(), nb.offset.data(), nb.ids.data(), nb.lpos.data(),
                    tree.sort_index.data()};
     }
+    // the force-interacting subsequences of the rows (written by the Omega pass)
+    Csr csr_force() const {
+        return Csr{nb.fcnt.data(), nb.offset.data(), nb.fids.data(), nb.lpos.data(),
+                   tree.sort_index.data()};
+    }
 
     // Solver.cpp:984-1046 + BasicSPHGhosts: images of the local particles after them
     void ghosts() {
@@ -244,8 +249,15 @@ struct shamx_solver {
         last.subcycles = hstep + 1;
         if (hstep == hmax) throw std::runtime_error("the h iterator is not converged");
         int b = timer.begin(stream);
+        // final h of the images and in the sorted records, then Omega fused with the force
+        // list filter (h and positions stay fixed for the rest of the step)
+        FieldPtrs fh{{h.data()}, 1};
+        gather_ghosts(stream, N, G, ghost_src.data(), fh);
+        ps_refresh_h(stream, N + G, tree.sort_index.data(), h.data(), nb.ps.data());
+        nb.fcnt.resize(std::max<u32>(N + G, 1));
+        nb.fids.resize(std::max<size_t>(nb.ids.size(), 1)); // same row offsets as ids
         compute_omega(stream, cfg.kernel, N, csr(), nb.ps.data(), x.data(), y.data(), z.data(),
-                      h.data(), omega.data(), cfg.gpart_mass);
+                      h.data(), omega.data(), cfg.gpart_mass, nb.fcnt.data(), nb.fids.data());
         timer.end(stream, PH_OMEGA, b);
     }
 
@@ -292,7 +304,7 @@ struct shamx_solver {
                          cfg.cfl_cour * cfl_mult,
                          cfg.cfl_force * cfl_mult,
                          cfg.mixed_precision};
-            compute_force(stream, cfg.kernel, N, csr(), frec.data(), x.data(), y.data(),
+            compute_force(stream, cfg.kernel, N, csr_force(), frec.data(), x.data(), y.data(),
                           z.data(), vx.data(), vy.data(), vz.data(), h.data(), omega.data(),
                           u.data(), P.data(), cs.data(), ax.data(), ay.data(), az.data(),
                           du.data(), ax_old.data(), ay_old.data(), az_old.data(), du_old.data(),
@@ -376,14 +388,14 @@ struct shamx_solver {
     // rebuilt on the host from the sorted layout: 5 counts, 6 offsets, 7 ids
     void reference_layout_csr(int what, u32 *dst) {
         u32 Nm = N + G;
-        std::vector<u32> cnt_s(Nm), off_s(Nm), ids_s(nb.n_pairs), sidx(Nm), inv(Nm);
+        std::vector<u32> cnt_s(Nm), off_s(Nm), ids_s(nb.ids.size()), sidx(Nm), inv(Nm);
         SHAMX_HIP(hipMemcpyAsync(cnt_s.data(), nb.cnt.data(), Nm * 4, hipMemcpyDeviceToHost, stream));
         SHAMX_HIP(hipMemcpyAsync(off_s.data(), nb.offset.data(), Nm * 4, hipMemcpyDeviceToHost,
                                  stream));
         SHAMX_HIP(hipMemcpyAsync(sidx.data(), tree.sort_index.data(), Nm * 4,
                                  hipMemcpyDeviceToHost, stream));
-        if (what == 7 && nb.n_pairs)
-            SHAMX_HIP(hipMemcpyAsync(ids_s.data(), nb.ids.data(), nb.n_pairs * 4,
+        if (what == 7 && ids_s.size())
+            SHAMX_HIP(hipMemcpyAsync(ids_s.data(), nb.ids.data(), ids_s.size() * 4,
                                      hipMemcpyDeviceToHost, stream));
         SHAMX_HIP(hipStreamSynchronize(stream));
         for (u32 k = 0; k < Nm; k++) inv[sidx[k]] = k;
@@ -395,6 +407,28 @@ struct shamx_solver {
             if (what == 7)
                 for (u32 k = 0; k < c; k++) dst[acc + k] = sidx[ids_s[off_s[row] + k]];
             acc += c;
+        }
+    }
+
+    // leaf lists in the packed layout: 8 counts, 9 offsets, 10 ids
+    void reference_layout_leaves(int what, u32 *dst) {
+        u32 L = tree.L;
+        std::vector<u32> cnt(L), off(L), ids(nb.leaf_ids.size());
+        SHAMX_HIP(hipMemcpyAsync(cnt.data(), nb.leaf_cnt.data(), L * 4, hipMemcpyDeviceToHost,
+                                 stream));
+        SHAMX_HIP(hipMemcpyAsync(off.data(), nb.leaf_offset.data(), L * 4, hipMemcpyDeviceToHost,
+                                 stream));
+        if (what == 10 && ids.size())
+            SHAMX_HIP(hipMemcpyAsync(ids.data(), nb.leaf_ids.data(), ids.size() * 4,
+                                     hipMemcpyDeviceToHost, stream));
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        u64 acc = 0;
+        for (u32 l = 0; l < L; l++) {
+            if (what == 8) dst[l] = cnt[l];
+            if (what == 9) dst[l] = (u32) acc;
+            if (what == 10)
+                for (u32 k = 0; k < cnt[l]; k++) dst[acc + k] = ids[off[l] + k];
+            acc += cnt[l];
         }
     }
 
@@ -584,7 +618,7 @@ int shamx_omega(void *stream, int kernel, uint32_t n, const uint32_t *cnt, const
                 const double *h, double *omega, double gpart_mass) {
     return guarded([&] {
         compute_omega((hipStream_t) stream, kernel, n, Csr{cnt, offset, ids}, nullptr, x, y, z, h,
-                      omega, gpart_mass);
+                      omega, gpart_mass, nullptr, nullptr);
     });
 }
 
@@ -777,9 +811,9 @@ int shamx_solver_get_u32(shamx_solver *s, int what, uint32_t *dst) {
         case 5:
         case 6:
         case 7: s->reference_layout_csr(what, dst); break;
-        case 8: d2h(st, dst, s->nb.leaf_cnt.data(), L); break;
-        case 9: d2h(st, dst, s->nb.leaf_offset.data(), L); break;
-        case 10: d2h(st, dst, s->nb.leaf_ids.data(), s->nb.n_leaf_pairs); break;
+        case 8:
+        case 9:
+        case 10: s->reference_layout_leaves(what, dst); break;
         case 11: d2h(st, dst, s->tree.lchild.data(), I); break;
         case 12: d2h(st, dst, s->tree.rchild.data(), I); break;
         case 13: d2h(st, dst, s->tree.endrange.data(), I); break;

@@ -77,7 +77,12 @@ namespace shamx {
         // sorted positions, so a row's candidates are runs of adjacent sorted records.
         DevArray<u32> cnt, offset, ids;
         DevArray<u32> lpos;                             // sorted layout: local rows in order
-        DevArray<u32> lflag, lscan;
+        DevArray<u32> fcnt, fids;                       // force-interacting subsequences
+        DevArray<u32> lflag, lscan, psum;
+        // sorted layout: one pass per stage into rows of fixed capacity; leaf l's list at
+        // leaf_ids[l * leaf_stride], row r at ids[r * row_stride] (0: packed layout).  The
+        // capacities persist from build to build and grow on overflow.
+        u32 cap_leaf = 0, cap_part = 0, leaf_stride = 0, row_stride = 0;
         DevArray<double4> ps;                           // (x, y, z, h) in sorted order
         DevArray<u32> inv;                              // sorted position of each object
         DevArray<u32> mism;                             // owner leaf != own leaf
@@ -114,8 +119,13 @@ namespace shamx {
                    const f64 *y, const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
                    f64 h_evol_max, f64 h_evol_iter_max, u64 *eps_minmax, int mixed,
                    DevArray<u64> &part);
+    // ComputeOmega; with records ps (row-indexed, .w = final h) and non-null fids, also writes
+    // the force-interacting subsequence of every row to fids (same offsets) and fcnt
     void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
-                       const f64 *y, const f64 *z, const f64 *h, f64 *omega, f64 mass);
+                       const f64 *y, const f64 *z, const f64 *h, f64 *omega, f64 mass,
+                       u32 *fcnt, u32 *fids);
+    // ps[s].w = h[sort_index[s]] for s < nm
+    void ps_refresh_h(hipStream_t s, u32 nm, const u32 *sort_index, const f64 *h, double4 *ps);
     struct EosArgs {
         int kind;
         f64 gamma, cs, cs0, q, r0;

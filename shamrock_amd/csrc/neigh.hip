@@ -77,12 +77,14 @@ namespace shamx {
             return true;
         }
 
-        // stage 1 count / fill, one lane per leaf, 64 consecutive leaves per wave
-        template<bool FILL>
+        // stage 1 count / fill, one lane per leaf, 64 consecutive leaves per wave.
+        // STRIDE: single pass, leaf g owns ids[g * cap, (g + 1) * cap); the count goes on past
+        // cap (entries beyond it are dropped) so the caller can detect the overflow.
+        template<bool FILL, bool STRIDE>
         __global__ void __launch_bounds__(256)
             k_leaf_neigh_w(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
-                           u32 *__restrict cnt, const u32 *__restrict offset,
-                           u32 *__restrict ids, u32 *__restrict err) {
+                           u32 *__restrict cnt, u32 *__restrict offset, u32 *__restrict ids,
+                           u32 cap, u32 *__restrict err) {
             u32 g      = blockIdx.x * blockDim.x + threadIdx.x;
             bool valid = g < L;
             u64 live   = __ballot(valid);
@@ -92,8 +94,10 @@ namespace shamx {
             f64 alo[3] = {ra[0], ra[1], ra[2]}, ahi[3] = {ra[3], ra[4], ra[5]};
             f64 elo[3] = {alo[0] - rint_a, alo[1] - rint_a, alo[2] - rint_a};
             f64 ehi[3] = {ahi[0] + rint_a, ahi[1] + rint_a, ahi[2] + rint_a};
-            u32 c      = FILL && valid ? offset[g] : 0u;
-            bool ok    = wave_traverse(
+            const u32 c0 = STRIDE ? (valid ? g : 0u) * cap : (FILL && valid ? offset[g] : 0u);
+            const u32 ce = c0 + cap;
+            u32 c        = c0;
+            bool ok      = wave_traverse(
                 nodes, I, I + L, live,
                 [&](const f64 *rec) {
                     f64 r      = rec[6] * Rkern;
@@ -103,11 +107,12 @@ namespace shamx {
                     return cella_neigh_b(alo, ahi, xlo, xhi) || cella_neigh_b(elo, ehi, nlo, nhi);
                 },
                 [&](u32 node) {
-                    if (FILL) ids[c] = node;
+                    if (FILL && (!STRIDE || c < ce)) ids[c] = node;
                     c++;
                 });
             if (!ok && __lane_id() == __builtin_ctzll(live)) atomicOr(err, 1u);
-            if (!FILL && valid) cnt[g] = c;
+            if ((!FILL || STRIDE) && valid) cnt[g] = c - c0;
+            if (STRIDE && valid) offset[g] = c0;
         }
 
         // owner leaf, lanes over sorted positions (spatially coherent waves)
@@ -140,7 +145,7 @@ namespace shamx {
         // stage 2 count / fill.  SORTED layout: the CSR is indexed by the sorted position of
         // the particle and holds sorted positions of its neighbours; otherwise both are
         // particle ids (the reference layout).
-        template<bool FILL, bool SORTED>
+        template<bool FILL, bool SORTED, bool STRIDE>
         __global__ void k_part_neigh(u32 n, const u32 *__restrict list,
                                      const u32 *__restrict list_len, u32 I,
                                      const f64 *__restrict nodes,
@@ -152,7 +157,8 @@ namespace shamx {
                                      const f64 *__restrict y, const f64 *__restrict z,
                                      const f64 *__restrict h, const u32 *__restrict inv,
                                      f64 htol, f64 Rker2, u32 *__restrict cnt,
-                                     const u32 *__restrict offset, u32 *__restrict ids) {
+                                     u32 *__restrict offset, u32 *__restrict ids, u32 cap,
+                                     u32 leaf_cap) {
             u32 t = blockIdx.x * blockDim.x + threadIdx.x;
             if (t >= (list ? *list_len : n)) return;
             u32 a      = list ? list[t] : t;
@@ -161,8 +167,11 @@ namespace shamx {
             f64 xa = x[a], ya = y[a], za = z[a];
             f64 ca = rint_a * rint_a * Rker2;
             u32 lo = owner[a];
-            u32 k0 = leaf_offset[lo], k1 = k0 + leaf_cnt[lo];
-            u32 c  = FILL ? offset[ia] : 0u;
+            u32 k0 = leaf_offset[lo],
+                k1 = k0 + (leaf_cap ? min(leaf_cnt[lo], leaf_cap) : leaf_cnt[lo]);
+            const u32 c0 = STRIDE ? ia * cap : (FILL ? offset[ia] : 0u);
+            const u32 ce = c0 + cap;
+            u32 c        = c0;
             for (u32 k = k0; k < k1; k++) {
                 u32 s0, s1;
                 node_children(nodes, leaf_ids[k], s0, s1); // leaf record: sorted range
@@ -173,12 +182,13 @@ namespace shamx {
                     f64 rb   = h[b] * htol;
                     bool no_interact = rab2 > ca && rab2 > rb * rb * Rker2;
                     if (!no_interact) {
-                        if (FILL) ids[c] = SORTED ? s : b;
+                        if (FILL && (!STRIDE || c < ce)) ids[c] = SORTED ? s : b;
                         c++;
                     }
                 }
             }
-            if (!FILL) cnt[ia] = c;
+            if (!FILL || STRIDE) cnt[ia] = c - c0;
+            if (STRIDE) offset[ia] = c0;
         }
 
         // sorted copy of positions and h (AoS, 32 B per object) + inverse permutation
@@ -230,14 +240,14 @@ namespace shamx {
         // is this leaf; the candidates (particles of the neighbour leaves, in list order then
         // sorted order) are taken 64 at a time, each lane holding one candidate; for each
         // member the keep-mask is a ballot, so the compacted output keeps candidate order.
-        template<bool FILL, bool SORTED>
+        template<bool FILL, bool SORTED, bool STRIDE>
         __global__ void __launch_bounds__(GRP_WAVES * WAVE)
             k_group_neigh(u32 n, u32 L, u32 I, const f64 *__restrict nodes,
                           const u32 *__restrict sort_index, const double4 *__restrict ps,
                           const u32 *__restrict owner, const u32 *__restrict leaf_cnt,
                           const u32 *__restrict leaf_offset, const u32 *__restrict leaf_ids,
-                          f64 htol, f64 Rker2, u32 *__restrict cnt, const u32 *__restrict offset,
-                          u32 *__restrict ids) {
+                          f64 htol, f64 Rker2, u32 *__restrict cnt, u32 *__restrict offset,
+                          u32 *__restrict ids, u32 cap, u32 leaf_cap) {
             __shared__ u32 rstart[GRP_WAVES][WAVE], rexcl[GRP_WAVES][WAVE];
             const int lane = __lane_id(), wid = threadIdx.x / WAVE;
             const u32 l    = blockIdx.x * GRP_WAVES + wid;
@@ -245,7 +255,10 @@ namespace shamx {
             u32 m0, m1;
             node_children(nodes, I + l, m0, m1);
             const u64 lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            const u32 k0 = leaf_offset[l], k1 = k0 + leaf_cnt[l];
+            // leaf_cap (fixed-capacity leaf rows): a row that overflowed holds only leaf_cap
+            // entries; the build is repeated then, but nothing may be read past the row
+            const u32 k0 = leaf_offset[l],
+                      k1 = k0 + (leaf_cap ? min(leaf_cnt[l], leaf_cap) : leaf_cnt[l]);
             for (u32 mb = m0; mb < m1; mb += WAVE) { // member batches of 64
                 u32 sm     = mb + lane;
                 bool mem   = false;
@@ -265,8 +278,10 @@ namespace shamx {
                 }
                 u64 members = __ballot(mem);
                 if (!members) continue;
-                const u32 ia = SORTED ? sm : a;
-                u32 cursor   = (FILL && mem) ? offset[ia] : 0u;
+                const u32 ia   = SORTED ? sm : a;
+                const u32 row0 = STRIDE ? (mem ? ia * cap : 0u)
+                                        : ((FILL && mem) ? offset[ia] : 0u);
+                u32 cursor     = row0;
                 u32 count = 0;
                 // neighbour leaves in batches of up to 64: lane i loads the range of the i-th
                 // leaf of the batch; candidates are the concatenated ranges in list order
@@ -315,7 +330,13 @@ namespace shamx {
                             u32 kc    = (u32) __popcll(km);
                             if (FILL) {
                                 u32 base = (u32) __builtin_amdgcn_readlane((int) cursor, m);
-                                if (keep) ids[base + (u32) __popcll(km & lt)] = b;
+                                u32 pos  = base + (u32) __popcll(km & lt);
+                                if (STRIDE) {
+                                    u32 end = (u32) __builtin_amdgcn_readlane((int) row0, m) + cap;
+                                    if (keep && pos < end) ids[pos] = b;
+                                } else if (keep) {
+                                    ids[pos] = b;
+                                }
                             }
                             if (lane == m) {
                                 cursor += kc;
@@ -327,30 +348,50 @@ namespace shamx {
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
-                if (!FILL && mem) cnt[ia] = count;
+                if ((!FILL || STRIDE) && mem) cnt[ia] = count;
+                if (STRIDE && mem) offset[ia] = row0;
             }
         }
     } // namespace
 
     namespace {
+        // max of n u32 values into *out (device; *out must start at 0)
+        __global__ void __launch_bounds__(256)
+            k_max_u32(u32 n, const u32 *__restrict v, u32 *__restrict out) {
+            __shared__ u32 lds[256 / WAVE];
+            u32 m = 0;
+            for (u32 i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+                m = max(m, v[i]);
+            m = wave_reduce(m, [](u32 p, u32 q) { return p > q ? p : q; });
+            if (__lane_id() == 0) lds[threadIdx.x / WAVE] = m;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (int w = 1; w < 256 / WAVE; w++) m = max(m, lds[w]);
+                atomicMax(out, m);
+            }
+        }
+        void max_u32(hipStream_t s, u32 n, const u32 *v, u32 *out) {
+            if (n) k_max_u32<<<std::min<u32>(div_up(n, 256), 1024), 256, 0, s>>>(n, v, out);
+        }
+
+        // packed rows, two passes (count, scan, fill); SORTED: rows by sorted position
         template<bool SORTED>
-        void stage2(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
-                    const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
-                    f64 Rker2, u32 *hc) {
-            u32 L = t.L, I = t.I, nm = t.n_obj;
-            u32 rows  = SORTED ? nm : n_local; // CSR rows
+        void stage2_packed(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
+                           const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
+                           f64 Rker2, u32 *hc) {
+            u32 L = t.L, I = t.I;
+            u32 rows      = SORTED ? t.n_obj : n_local;
             u32 *mism_len = nb.counters.data() + 5;
-            u32 *err      = nb.counters.data() + 4;
             u32 gblocks   = div_up(L, GRP_WAVES);
             if (SORTED) fill_u32(s, nb.cnt.data(), 0, rows); // ghost rows stay empty
-            k_group_neigh<false, SORTED><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+            k_group_neigh<false, SORTED, false><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
-                htol, Rker2, nb.cnt.data(), nullptr, nullptr);
-            k_part_neigh<false, SORTED><<<div_up(n_local, 128), 128, 0, s>>>(
+                htol, Rker2, nb.cnt.data(), nullptr, nullptr, 0, 0);
+            k_part_neigh<false, SORTED, false><<<div_up(n_local, 128), 128, 0, s>>>(
                 n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
-                y, z, h, nb.inv.data(), htol, Rker2, nb.cnt.data(), nullptr, nullptr);
+                y, z, h, nb.inv.data(), htol, Rker2, nb.cnt.data(), nullptr, nullptr, 0, 0);
             exclusive_scan_u32(s, nb.cnt.data(), nb.offset.data(), rows, nb.counters.data() + 1,
                                nb.scratch.data());
             SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
@@ -359,19 +400,20 @@ namespace shamx {
             if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
             if (hc[4] & 2u)
                 throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
-            (void) err;
             nb.n_pairs    = hc[1];
             nb.n_mismatch = hc[5];
             nb.ids.resize(std::max<u64>(nb.n_pairs, 1));
-            k_group_neigh<true, SORTED><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+            k_group_neigh<true, SORTED, false><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
-                htol, Rker2, nullptr, nb.offset.data(), nb.ids.data());
-            k_part_neigh<true, SORTED><<<div_up(n_local, 128), 128, 0, s>>>(
+                htol, Rker2, nullptr, nb.offset.data(), nb.ids.data(), 0, 0);
+            k_part_neigh<true, SORTED, false><<<div_up(n_local, 128), 128, 0, s>>>(
                 n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
-                y, z, h, nb.inv.data(), htol, Rker2, nullptr, nb.offset.data(), nb.ids.data());
+                y, z, h, nb.inv.data(), htol, Rker2, nullptr, nb.offset.data(), nb.ids.data(), 0, 0);
         }
+
+        u32 grow_cap(u32 need) { return need + need / 8 + 16; }
     } // namespace
 
     void neigh_build(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
@@ -381,7 +423,7 @@ namespace shamx {
         nb.n_local       = n_local;
         nb.sorted_layout = sorted_layout;
         u32 rows         = sorted_layout ? nm : n_local;
-        nb.counters.resize(8);
+        nb.counters.resize(16);
         nb.scratch.resize(std::max({scan_scratch_bytes(L), scan_scratch_bytes(rows),
                                     scan_scratch_bytes(nm)}));
         nb.leaf_cnt.resize(L);
@@ -389,59 +431,143 @@ namespace shamx {
         nb.owner.resize(std::max<u32>(n_local, 1));
         nb.cnt.resize(std::max<u32>(rows, 1));
         nb.offset.resize(std::max<u32>(rows, 1));
+        nb.psum.resize(std::max<u32>(std::max(rows, L), 1));
         u32 *err = nb.counters.data() + 4;
-        fill_u32(s, nb.counters.data(), 0, 8);
+        fill_u32(s, nb.counters.data(), 0, 16);
+        f64 Rker2 = Rkern * Rkern;
+        u32 hc[16];
 
-        // stage 1
-        k_leaf_neigh_w<false><<<div_up(L, 256), 256, 0, s>>>(L, I, t.nodes.data(), Rkern,
-                                                             nb.leaf_cnt.data(), nullptr, nullptr,
-                                                             err);
-        exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.leaf_offset.data(), L, nb.counters.data(),
-                           nb.scratch.data());
-        u32 hc[8];
-        SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32), hipMemcpyDeviceToHost, s));
-        SHAMX_HIP(hipStreamSynchronize(s));
-        if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
-        nb.n_leaf_pairs = hc[0];
-        nb.leaf_ids.resize(std::max<u64>(nb.n_leaf_pairs, 1));
-        k_leaf_neigh_w<true><<<div_up(L, 256), 256, 0, s>>>(L, I, t.nodes.data(), Rkern,
-                                                            nullptr, nb.leaf_offset.data(),
-                                                            nb.leaf_ids.data(), err);
-        if (n_local == 0) {
-            nb.n_pairs = 0;
-            return;
-        }
-        // sorted copy and inverse permutation, owner leaf, owner != own-leaf list
+        // sorted copy and inverse permutation (independent of the leaf lists)
         nb.ps.resize(nm);
         nb.inv.resize(nm);
         nb.mism.resize(std::max<u32>(n_local, 1));
         k_sorted_copy<<<div_up(nm, 256), 256, 0, s>>>(nm, t.sort_index.data(), x, y, z, h,
                                                       nb.ps.data(), nb.inv.data());
+
+        if (!sorted_layout) {
+            // reference layout: stage 1 in two passes
+            k_leaf_neigh_w<false, false><<<div_up(L, 256), 256, 0, s>>>(
+                L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nullptr, nullptr, 0, err);
+            exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.leaf_offset.data(), L,
+                               nb.counters.data(), nb.scratch.data());
+            SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
+                                     hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
+            nb.n_leaf_pairs = hc[0];
+            nb.leaf_stride  = 0;
+            nb.leaf_ids.resize(std::max<u64>(nb.n_leaf_pairs, 1));
+            k_leaf_neigh_w<true, false><<<div_up(L, 256), 256, 0, s>>>(
+                L, I, t.nodes.data(), Rkern, nullptr, nb.leaf_offset.data(), nb.leaf_ids.data(),
+                0, err);
+            if (n_local == 0) {
+                nb.n_pairs = 0;
+                return;
+            }
+            k_owner_w<<<div_up(nm, 256), 256, 0, s>>>(nm, n_local, I, t.nodes.data(),
+                                                      t.sort_index.data(), nb.ps.data(), L,
+                                                      nb.owner.data(), err);
+            k_mismatch<<<div_up(n_local, 256), 256, 0, s>>>(n_local, I, t.nodes.data(),
+                                                            nb.owner.data(), nb.inv.data(),
+                                                            nb.mism.data(),
+                                                            nb.counters.data() + 5);
+            stage2_packed<false>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
+            nb.row_stride = 0;
+            SHAMX_HIP(hipGetLastError());
+            return;
+        }
+
+        // sorted layout: owner leaves, mismatch list, local rows in sorted order
         k_owner_w<<<div_up(nm, 256), 256, 0, s>>>(nm, n_local, I, t.nodes.data(),
                                                   t.sort_index.data(), nb.ps.data(), L,
                                                   nb.owner.data(), err);
-        u32 *mism_len = nb.counters.data() + 5;
         k_mismatch<<<div_up(n_local, 256), 256, 0, s>>>(n_local, I, t.nodes.data(),
                                                         nb.owner.data(), nb.inv.data(),
-                                                        nb.mism.data(), mism_len);
-        if (sorted_layout) {
-            // sorted positions of the local particles, in sorted order
-            nb.lflag.resize(nm);
-            nb.lscan.resize(nm);
-            nb.lpos.resize(std::max<u32>(n_local, 1));
-            k_local_flag<<<div_up(nm, 256), 256, 0, s>>>(nm, n_local, t.sort_index.data(),
-                                                         nb.lflag.data());
-            exclusive_scan_u32(s, nb.lflag.data(), nb.lscan.data(), nm, nb.counters.data() + 6,
+                                                        nb.mism.data(), nb.counters.data() + 5);
+        nb.lflag.resize(nm);
+        nb.lscan.resize(nm);
+        nb.lpos.resize(std::max<u32>(n_local, 1));
+        k_local_flag<<<div_up(nm, 256), 256, 0, s>>>(nm, n_local, t.sort_index.data(),
+                                                     nb.lflag.data());
+        exclusive_scan_u32(s, nb.lflag.data(), nb.lscan.data(), nm, nb.counters.data() + 6,
+                           nb.scratch.data());
+        k_local_scatter<<<div_up(nm, 256), 256, 0, s>>>(nm, nb.lflag.data(), nb.lscan.data(),
+                                                        nb.lpos.data());
+
+        // both stages in one pass each, rows of fixed capacity.  The capacities come from the
+        // previous build; on overflow the counts (exact, they run past the capacity) size a
+        // rebuild.  Stage 2 counts are only meaningful once stage 1 fitted.
+        if (nb.cap_leaf == 0) nb.cap_leaf = 128;
+        if (nb.cap_part == 0) nb.cap_part = 256;
+        u32 *mism_len = nb.counters.data() + 5;
+        for (int attempt = 0;; attempt++) {
+            if (attempt == 4) throw std::runtime_error("neighbour cache: capacities do not settle");
+            u32 cl = nb.cap_leaf, cp = nb.cap_part;
+            if ((u64) L * cl > 0xffffffffull || (u64) nm * cp > 0xffffffffull) {
+                // u32 entry indices cannot address fixed-capacity rows: two passes instead
+                k_leaf_neigh_w<false, false><<<div_up(L, 256), 256, 0, s>>>(
+                    L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nullptr, nullptr, 0, err);
+                exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.leaf_offset.data(), L,
+                                   nb.counters.data(), nb.scratch.data());
+                SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
+                                         hipMemcpyDeviceToHost, s));
+                SHAMX_HIP(hipStreamSynchronize(s));
+                if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
+                nb.n_leaf_pairs = hc[0];
+                nb.leaf_ids.resize(std::max<u64>(nb.n_leaf_pairs, 1));
+                k_leaf_neigh_w<true, false><<<div_up(L, 256), 256, 0, s>>>(
+                    L, I, t.nodes.data(), Rkern, nullptr, nb.leaf_offset.data(),
+                    nb.leaf_ids.data(), 0, err);
+                stage2_packed<true>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
+                nb.leaf_stride = 0;
+                nb.row_stride  = 0;
+                break;
+            }
+            nb.leaf_ids.resize(std::max<u64>((u64) L * cl, 1));
+            nb.ids.resize(std::max<u64>((u64) nm * cp, 1));
+            fill_u32(s, nb.counters.data() + 8, 0, 8);
+            k_leaf_neigh_w<true, true><<<div_up(L, 256), 256, 0, s>>>(
+                L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nb.leaf_offset.data(),
+                nb.leaf_ids.data(), cl, err);
+            max_u32(s, L, nb.leaf_cnt.data(), nb.counters.data() + 8);
+            fill_u32(s, nb.cnt.data(), 0, nm); // ghost rows stay empty
+            k_group_neigh<true, true, true><<<div_up(L, GRP_WAVES), GRP_WAVES * WAVE, 0, s>>>(
+                n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
+                nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
+                htol, Rker2, nb.cnt.data(), nb.offset.data(), nb.ids.data(), cp, cl);
+            k_part_neigh<true, true, true><<<div_up(n_local, 128), 128, 0, s>>>(
+                n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
+                nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
+                y, z, h, nb.inv.data(), htol, Rker2, nb.cnt.data(), nb.offset.data(),
+                nb.ids.data(), cp, cl);
+            max_u32(s, nm, nb.cnt.data(), nb.counters.data() + 9);
+            // totals (the scans' outputs are not used)
+            exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.psum.data(), L, nb.counters.data() + 10,
                                nb.scratch.data());
-            k_local_scatter<<<div_up(nm, 256), 256, 0, s>>>(nm, nb.lflag.data(), nb.lscan.data(),
-                                                            nb.lpos.data());
+            exclusive_scan_u32(s, nb.cnt.data(), nb.psum.data(), nm, nb.counters.data() + 11,
+                               nb.scratch.data());
+            SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
+                                     hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
+            if (hc[4] & 2u)
+                throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
+            u32 max_leaf = hc[8], max_part = hc[9];
+            if (max_leaf > cl) {
+                nb.cap_leaf = grow_cap(max_leaf);
+                continue;
+            }
+            if (max_part > cp) {
+                nb.cap_part = grow_cap(max_part);
+                continue;
+            }
+            nb.n_leaf_pairs = hc[10];
+            nb.n_pairs      = hc[11];
+            nb.n_mismatch   = hc[5];
+            nb.leaf_stride  = cl;
+            nb.row_stride   = cp;
+            break;
         }
-        // stage 2
-        f64 Rker2 = Rkern * Rkern;
-        if (sorted_layout)
-            stage2<true>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
-        else
-            stage2<false>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
         SHAMX_HIP(hipGetLastError());
     }
 

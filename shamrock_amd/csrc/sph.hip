@@ -112,6 +112,7 @@ namespace shamx {
                 double4 q = ps[b];
                 return make_double3(q.x, q.y, q.z);
             }
+            __device__ double4 pos4(u32 b) const { return ps[b]; }
         };
         struct PosSoA {
             const f64 *__restrict x, *__restrict y, *__restrict z;
@@ -124,6 +125,26 @@ namespace shamx {
             c     = (t - s) - y;
             s     = t;
         }
+
+        // mixed-precision pair arithmetic: hardware reciprocal and square root (about 1 ulp),
+        // kernel values from q = r / h evaluated as r * (1 / h) with 1 / h per particle
+        __device__ inline f32 rcp_approx(f32 v) { return __builtin_amdgcn_rcpf(v); }
+        __device__ inline f32 sqrt_approx(f32 v) { return __builtin_amdgcn_sqrtf(v); }
+        template<class K>
+        struct KernInv {
+            static constexpr f32 norm = f32(K::norm_3d);
+            // dW/dr and dW/dh (Kern::dW, Kern::dhW) with the powers of 1 / h precomputed
+            __device__ static f32 dW(f32 r, f32 inv_h, f32 inv_h4) {
+                return norm * K::df(r * inv_h) * inv_h4;
+            }
+            __device__ static f32 W(f32 r, f32 inv_h, f32 inv_h3) {
+                return norm * K::f(r * inv_h) * inv_h3;
+            }
+            __device__ static f32 dhW(f32 r, f32 inv_h, f32 inv_h4) {
+                f32 q = r * inv_h;
+                return -norm * (f32(3) * K::f(q) + q * K::df(q)) * inv_h4;
+            }
+        };
 
         // ------------------------------------------------------------- h iteration
         // One Newton sweep (IterateSmoothingLengthDensity.cpp:28-120).  G lanes share one
@@ -156,16 +177,18 @@ namespace shamx {
                 f64 dint    = h_a * h_a * Rkern * Rkern;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
                 if constexpr (MIXED) {
-                    f32 hf = (f32) h_a;
+                    using KI   = KernInv<K>;
+                    f32 ih     = (f32) (1. / h_a);
+                    f32 ih3    = ih * ih * ih, ih4 = ih3 * ih;
                     f32 rs = 0, rc = 0, ds = 0, dc = 0;
                     for (u32 k = k0 + lg; k < k1; k += G) {
                         double3 q = src.pos(r.ids[k]);
                         f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
                         f64 rab2 = dx * dx + dy * dy + dz * dz;
                         if (rab2 > dint) continue;
-                        f32 rab = sqrtf((f32) rab2);
-                        kahan_add(rs, rc, KF::W(rab, hf));
-                        kahan_add(ds, dc, KF::dhW(rab, hf));
+                        f32 rab = sqrt_approx((f32) rab2);
+                        kahan_add(rs, rc, KI::W(rab, ih, ih3));
+                        kahan_add(ds, dc, KI::dhW(rab, ih, ih4));
                     }
                     rho_sum = (f64) rs - (f64) rc;
                     sumdWdh = (f64) ds - (f64) dc;
@@ -222,17 +245,21 @@ namespace shamx {
         }
 
         // ------------------------------------------------------------- omega
-        // ComputeOmega.cpp:25-74, same group layout as the h iteration
-        template<class K, int G, class Src>
+        // ComputeOmega.cpp:25-74, same group layout as the h iteration.  FILTER: h and the
+        // positions are final for the rest of the step, so the same pass also writes, for the
+        // force loop, the subsequence of each row that passes the force criterion
+        // rab2 <= max(h_a, h_b)^2 R^2 (UpdateDerivs.cpp:160-167), in row order, at the row's
+        // offset in fids, with its length in fcnt (ps.w must hold the final h then).
+        template<class K, int G, bool FILTER, class Src>
         __global__ void __launch_bounds__(BLK)
             k_omega(u32 n, Rows r, Src src, const f64 *__restrict h, f64 *__restrict omega,
-                    f64 mass) {
+                    f64 mass, u32 *__restrict fcnt, u32 *__restrict fids) {
             using KK = Kern<K, f64>;
             const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
             const u32 lg   = threadIdx.x % G;
             const bool act = j < n;
-            const f64 Rkern = K::Rkern;
-            u32 row = 0, a = 0;
+            const f64 Rkern = K::Rkern, Rker2 = K::Rkern * K::Rkern;
+            u32 row = 0, a = 0, fc = 0;
             f64 h_a = 0, part_omega_sum = 0;
             if (act) {
                 row        = r.sel ? r.sel[j] : j;
@@ -240,26 +267,52 @@ namespace shamx {
                 h_a        = h[a];
                 double3 pa = src.pos(row);
                 f64 dint   = h_a * h_a * Rkern * Rkern;
+                f64 ca     = h_a * h_a * Rker2;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
                 for (u32 base = k0; base < k1; base += G) {
-                    u32 k  = base + lg;
-                    f64 th = 0;
+                    u32 k     = base + lg;
+                    f64 th    = 0;
+                    bool keep = false;
+                    u32 id    = 0;
                     if (k < k1) {
-                        double3 q = src.pos(r.ids[k]);
-                        f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
+                        id = r.ids[k];
+                        f64 dx, dy, dz, hb = 0;
+                        if constexpr (FILTER) {
+                            double4 q = src.pos4(id);
+                            dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
+                            hb = q.w;
+                        } else {
+                            double3 q = src.pos(id);
+                            dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
+                        }
                         f64 rab2 = dx * dx + dy * dy + dz * dz;
                         if (!(rab2 > dint)) {
                             f64 rab = sqrt(rab2);
                             th      = mass * KK::dhW(rab, h_a);
                         }
+                        keep = !(rab2 > ca && rab2 > hb * hb * Rker2);
                     }
                     part_omega_sum = ordered_fold<G>(part_omega_sum, th);
+                    if constexpr (FILTER) {
+                        u64 bal   = __ballot(keep);
+                        u32 gbits = (u32) (bal >> (__lane_id() & ~(G - 1))) & ((1u << G) - 1u);
+                        if (keep) fids[k0 + fc + (u32) __popc(gbits & ((1u << lg) - 1u))] = id;
+                        fc += (u32) __popc(gbits);
+                    }
                 }
             }
             if (act && lg == 0) {
                 f64 rho_ha = rho_h(mass, h_a, KK::hfactd);
                 omega[a]   = 1 + (h_a / (3 * rho_ha)) * part_omega_sum;
+                if (FILTER) fcnt[row] = fc;
             }
+        }
+
+        // ps[s].w = h[sort_index[s]] (final h into the sorted position records)
+        __global__ void k_ps_refresh_h(u32 nm, const u32 *__restrict sort_index,
+                                       const f64 *__restrict h, double4 *__restrict ps) {
+            u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+            if (s < nm) ps[s].w = h[sort_index[s]];
         }
 
         // ------------------------------------------------------------- eos
@@ -310,7 +363,7 @@ namespace shamx {
         // mixed: FP64 positions for r_ab, FP32 fields (64 B)
         struct alignas(16) FRec32 {
             f64 x, y, z;
-            f32 h, vx, vy, vz, u, P, cs, rho, inv_sub, inv_romega;
+            f32 inv_h, vx, vy, vz, u, P, cs, rho, inv_sub, inv_romega;
         };
         static_assert(sizeof(FRec32) == 64, "mixed force record is 64 B");
 
@@ -334,10 +387,11 @@ namespace shamx {
                                              f64 u, f64 P, f64 cs, f64 omega, f64 mass) {
             FRec32 r;
             r.x = x, r.y = y, r.z = z;
-            r.h = (f32) h, r.vx = (f32) vx, r.vy = (f32) vy, r.vz = (f32) vz, r.u = (f32) u;
+            r.inv_h = (f32) (1. / h);
+            r.vx = (f32) vx, r.vy = (f32) vy, r.vz = (f32) vz, r.u = (f32) u;
             r.P = (f32) P, r.cs = (f32) cs;
             f32 om       = (f32) omega;
-            r.rho        = rho_h((f32) mass, r.h, (f32) K::hfactd);
+            r.rho        = (f32) rho_h(mass, h, (f64) K::hfactd);
             r.inv_sub    = 1.f / ((r.rho * r.rho) * om);
             r.inv_romega = 1.f / (om * r.rho);
             return r;
@@ -518,7 +572,7 @@ namespace shamx {
         __global__ void __launch_bounds__(BLK)
             k_force_mixed(u32 n, Rows r, Src src, const f64 *__restrict h64, ForceOut o,
                           ForceArgs fa, u64 *__restrict part) {
-            using KF = Kern<K, f32>;
+            using KI = KernInv<K>;
             __shared__ u64 lds[2][BLK / WAVE];
             const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
             const u32 lg   = threadIdx.x % G;
@@ -534,34 +588,36 @@ namespace shamx {
                 const FRec32 A = src.get(row);
                 h_a64 = h64[a];
                 const f32 pmass = (f32) fa.mass;
-                const f32 h_a = A.h;
+                const f32 ih_a = A.inv_h, ih4_a = ih_a * ih_a * ih_a * ih_a;
                 const f32 vxa = A.vx, vya = A.vy, vza = A.vz, P_a = A.P, u_a = A.u, cs_a = A.cs;
-                const f32 rho_a = A.rho, rho_a_inv = 1.f / rho_a;
+                const f32 rho_a = A.rho, rho_a_inv = rcp_approx(rho_a);
                 const f32 omega_a_rho_a_inv = A.inv_romega, inv_sub_a = A.inv_sub;
                 const f32 alpha_AV = (f32) fa.alpha_AV, beta_AV = (f32) fa.beta_AV,
                           alpha_u = (f32) fa.alpha_u;
+                const f32 c_du1 = omega_a_rho_a_inv * rho_a_inv * pmass;
                 const f64 ca = h_a64 * h_a64 * Rker2;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
                 for (u32 k = k0 + lg; k < k1; k += G) {
                     const FRec32 B = src.get(r.ids[k]);
                     f64 dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
                     f64 rab2 = dx * dx + dy * dy + dz * dz;
-                    f64 hb   = (f64) B.h;
-                    if (rab2 > ca && rab2 > hb * hb * Rker2) continue;
+                    // neighbour-side criterion in FP32 (the solver's rows are prefiltered in
+                    // FP64 already; this only matters for unfiltered reference-layout rows)
+                    if (rab2 > ca && (f32) rab2 * (B.inv_h * B.inv_h) > (f32) Rker2) continue;
                     npair++;
-                    f32 fdx = (f32) dx, fdy = (f32) dy, fdz = (f32) dz;
-                    f32 rab = sqrtf((f32) rab2);
+                    f32 rab  = sqrt_approx((f32) rab2);
+                    f32 inv  = (rab >= 1e-9f) ? rcp_approx(rab) : 0.f;
+                    f32 rux = (f32) dx * inv, ruy = (f32) dy * inv, ruz = (f32) dz * inv;
                     f32 vabx = vxa - B.vx, vaby = vya - B.vy, vabz = vza - B.vz;
                     f32 rho_b = B.rho;
-                    f32 Fab_a = KF::dW(rab, h_a), Fab_b = KF::dW(rab, B.h);
-                    f32 inv = (rab >= 1e-9f) ? 1.f / rab : 0.f;
-                    f32 rux = fdx * inv, ruy = fdy * inv, ruz = fdz * inv;
+                    f32 ih_b  = B.inv_h, ih4_b = ih_b * ih_b * ih_b * ih_b;
+                    f32 Fab_a = KI::dW(rab, ih_a, ih4_a), Fab_b = KI::dW(rab, ih_b, ih4_b);
                     f32 v_ab_r_ab = vabx * rux + vaby * ruy + vabz * ruz;
                     f32 abs_v     = fabsf(v_ab_r_ab);
                     f32 vsig_a = alpha_AV * cs_a + beta_AV * abs_v;
                     f32 vsig_b = alpha_AV * B.cs + beta_AV * abs_v;
                     f32 rho_avg = (rho_a + rho_b) * 0.5f;
-                    f32 vsig_u  = sqrtf(fabsf(P_a - B.P) / rho_avg);
+                    f32 vsig_u  = sqrt_approx(fabsf(P_a - B.P) * rcp_approx(rho_avg));
                     f32 qa_ab = fmaxf(-0.5f * rho_a * vsig_a * v_ab_r_ab, 0.f);
                     f32 qb_ab = fmaxf(-0.5f * rho_b * vsig_b * v_ab_r_ab, 0.f);
                     f32 AV_P_a = P_a + qa_ab, AV_P_b = B.P + qb_ab;
@@ -571,7 +627,7 @@ namespace shamx {
                     kahan_add(sx, cx, s * rux);
                     kahan_add(sy, cy, s * ruy);
                     kahan_add(sz, cz, s * ruz);
-                    f32 t1 = AV_P_a * (omega_a_rho_a_inv * rho_a_inv) * pmass * Fab_a * v_ab_r_ab;
+                    f32 t1 = AV_P_a * c_du1 * Fab_a * v_ab_r_ab;
                     f32 t2 = pmass * alpha_u * vsig_u * (u_a - B.u) * 0.5f *
                              (Fab_a * omega_a_rho_a_inv + Fab_b * B.inv_romega);
                     kahan_add(su, cu, t1 + t2);
@@ -885,20 +941,33 @@ namespace shamx {
     }
 
     void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
-                       const f64 *y, const f64 *z, const f64 *h, f64 *omega, f64 mass) {
+                       const f64 *y, const f64 *z, const f64 *h, f64 *omega, f64 mass,
+                       u32 *fcnt, u32 *fids) {
         if (n == 0) return;
+        if (fids && !ps) throw std::invalid_argument("compute_omega: the filter needs records");
         u32 nb = div_up((u64) n * pair_lanes(), BLK);
         Rows r = rows_of(c);
         dispatch_kernel(kernel, [&](auto K) {
             using KT = decltype(K);
             dispatch_lanes([&](auto GC) {
                 constexpr int G = decltype(GC)::value;
-                if (ps)
-                    k_omega<KT, G><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega, mass);
+                if (ps && fids)
+                    k_omega<KT, G, true><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega, mass,
+                                                            fcnt, fids);
+                else if (ps)
+                    k_omega<KT, G, false><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega, mass,
+                                                             nullptr, nullptr);
                 else
-                    k_omega<KT, G><<<nb, BLK, 0, s>>>(n, r, PosSoA{x, y, z}, h, omega, mass);
+                    k_omega<KT, G, false><<<nb, BLK, 0, s>>>(n, r, PosSoA{x, y, z}, h, omega,
+                                                             mass, nullptr, nullptr);
             });
         });
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void ps_refresh_h(hipStream_t s, u32 nm, const u32 *sort_index, const f64 *h, double4 *ps) {
+        if (nm == 0) return;
+        k_ps_refresh_h<<<div_up(nm, 256), 256, 0, s>>>(nm, sort_index, h, ps);
         SHAMX_HIP(hipGetLastError());
     }
 

@@ -71,14 +71,16 @@ namespace shamx {
             }
         }
 
-        __global__ void k_bbox_final(u32 nb, const f64 *__restrict part, f64 *__restrict bb) {
-            if (threadIdx.x < 6) {
-                int k = threadIdx.x;
-                f64 r = part[k];
-                for (u32 b = 1; b < nb; b++)
-                    r = (k < 3) ? fmin(r, part[b * 6 + k]) : fmax(r, part[b * 6 + k]);
-                bb[k] = (k < 3) ? next_down(r) : next_up(r);
-            }
+        // one wave per component: lanes stride over the block partials, then a wave reduction
+        __global__ void __launch_bounds__(6 * WAVE)
+            k_bbox_final(u32 nb, const f64 *__restrict part, f64 *__restrict bb) {
+            const int k = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+            f64 r       = (k < 3) ? __builtin_inf() : -__builtin_inf();
+            for (u32 b = lane; b < nb; b += WAVE)
+                r = (k < 3) ? fmin(r, part[b * 6 + k]) : fmax(r, part[b * 6 + k]);
+            if (k < 3) r = wave_reduce(r, [](f64 p, f64 q) { return fmin(p, q); });
+            else r = wave_reduce(r, [](f64 p, f64 q) { return fmax(p, q); });
+            if (lane == 0) bb[k] = (k < 3) ? next_down(r) : next_up(r);
         }
 
         // --------------------------------------------------------------------- Morton
@@ -436,7 +438,7 @@ namespace shamx {
                         f64 *bbox6, void *scratch) {
         u32 nb = std::max<u32>(1, std::min<u32>(BB_MAX_BLOCKS, div_up(n, BB_BLOCK * 8)));
         k_bbox_partial<<<nb, BB_BLOCK, 0, s>>>(n, x, y, z, (f64 *) scratch);
-        k_bbox_final<<<1, 64, 0, s>>>(nb, (const f64 *) scratch, bbox6);
+        k_bbox_final<<<1, 6 * WAVE, 0, s>>>(nb, (const f64 *) scratch, bbox6);
         SHAMX_HIP(hipGetLastError());
     }
 
