@@ -58,7 +58,9 @@ typedef struct shamx_config {
     double htol_up_coarse, htol_up_fine, epsilon_h; /* SolverConfig.hpp:605-607 */
     double cfl_cour, cfl_force, cfl_multiplier_stiffness;
     int32_t mixed_precision;      /* 0: FP64 step; 1: FP32 pair sums (Kahan) on FP64 r_ab */
-    int32_t reserved;
+    int32_t av_kind;              /* 0: AVConfig::Constant, 1: AVConfig::VaryingCD10 */
+    double alpha_min, alpha_max, sigma_decay; /* VaryingCD10 (AVConfig.hpp); alpha_u, beta_AV
+                                                 above apply to both kinds */
 } shamx_config;
 
 /* Per-step log (reference TimestepLog Solver.hpp:43-54 + per-phase timers).  Times in s. */
@@ -99,6 +101,17 @@ int shamx_sort(shamx_ctx *ctx, void *stream, uint32_t n, uint32_t key_bits, uint
  * compute_presteps_rint (Solver.cpp:1322-1356); tree kept in ctx */
 int shamx_tree_build(shamx_ctx *ctx, void *stream, uint32_t n, const double *x, const double *y,
                      const double *z, const double *h, uint32_t reduction_level, double htol);
+/* the first half of shamx_tree_build: replaces
+ * CompressedLeafBVH::internal_rebuild_from_positions_no_aabb (CompressedLeafBVH.cpp:31-76):
+ * bbox, Morton codes, sort, leaf reduction (reduction_alg.cpp:413-456) and the Karras tree
+ * (KarrasRadixTree.cpp:152-180); the cell boxes are left unset */
+int shamx_tree_reduce_karras(shamx_ctx *ctx, void *stream, uint32_t n, const double *x,
+                             const double *y, const double *z, uint32_t reduction_level);
+/* the second half: replaces compute_tree_aabb_from_positions (KarrasRadixTreeAABB.cpp:92-140)
+ * + compute_presteps_rint (Solver.cpp:1322-1356, KarrasRadixTreeField.hpp:188-222) on the
+ * tree in ctx: leaf and cell boxes, rint = max h * htol per cell */
+int shamx_tree_aabb_hmax(shamx_ctx *ctx, void *stream, const double *x, const double *y,
+                         const double *z, const double *h, double htol);
 /* sizes of the tree in ctx: out4 = {n_obj, leaf_count, internal_count, 0} */
 int shamx_tree_sizes(shamx_ctx *ctx, uint32_t *out4);
 /* copies of the tree arrays (device pointers; any may be NULL):
@@ -150,6 +163,12 @@ int shamx_force(void *stream, int kernel, uint32_t n, const uint32_t *cnt, const
                 double *az_old, double *du_old, double *vsig, double *dt_cfl, double gpart_mass,
                 double alpha_u, double alpha_AV, double beta_AV, double C_cour, double C_force,
                 int mixed, double *dt_min_host);
+/* replaces ComputeCFLCourant (ComputeCFLCourant.hpp:25-70) + ComputeCFLForce
+ * (ComputeCFLForce.hpp:50-62) and the next-dt reduction (Solver.cpp:3095-3124):
+ * dt_cfl[i] = min(C_cour h/vsig, C_force sqrt(h/|a|)) (device), dt_min_host = min over i */
+int shamx_cfl(void *stream, uint32_t n, const double *h, const double *vsig, const double *ax,
+              const double *ay, const double *az, double C_cour, double C_force, double *dt_cfl,
+              double *dt_min_host);
 /* replaces the leapfrog predictor (ForwardEuler nodes, Solver.cpp:390-524) and
  * apply_position_boundary periodic wrap (integrators.cpp:206-236); box6 is a HOST array */
 int shamx_predictor(void *stream, uint32_t n, double dt, double *x, double *y, double *z,
@@ -181,6 +200,18 @@ int shamx_solver_set_time(shamx_solver *s, const double *tdc3);
 int shamx_solver_evolve_once(shamx_solver *s, shamx_step_log *log);
 /* stream used by the solver (hipStream_t as void*) */
 int shamx_solver_stream(shamx_solver *s, void **stream);
+
+/* ---- adjoint (no reference counterpart: the reference has no adjoint) ----------------------
+ * Vector-Jacobian product of the last evolve_once (FP64, adiabatic or isothermal EOS, M4 / M6 /
+ * M8 / C2 kernels, one corrector pass, dt held fixed): seeds11 holds 11 device arrays of N
+ * doubles, the adjoints of (x, y, z, vx, vy, vz, u, ax, ay, az, du) at the end of the step,
+ * replaced in place by those at its start.  The smoothing length is differentiated through its
+ * density equation (implicit function theorem). */
+int shamx_solver_adjoint_step(shamx_solver *s, double *const *seeds11);
+/* adds to xbar, ybar, zbar (device, N) the position gradient of sum_a rho_bar_a rho_a at the
+ * current state, rho_a = m (hfact / h_a)^3 with h_a from the density equation */
+int shamx_solver_adjoint_density(shamx_solver *s, const double *rho_bar, double *xbar,
+                                 double *ybar, double *zbar);
 /* transient products of the last step (copies to HOST arrays sized by shamx_solver_sizes):
  * f64 ids: 0..10 merged {x,y,z,h,vx,vy,vz,u,omega,P,cs} (n_merged), 11 omega (n), 12 vsig (n),
  *          13 tree bbox (6), 14 rint (I+L), 15 aabb_min (3(I+L)), 16 aabb_max (3(I+L))

@@ -840,7 +840,8 @@ namespace orc {
                                           const f64 *vy, const f64 *vz, const f64 *h,
                                           const f64 *omega, const f64 *u, const f64 *P,
                                           const f64 *cs, f64 *ax, f64 *ay, f64 *az, f64 *du,
-                                          f64 pmass, AVParams av) {
+                                          f64 pmass, AVParams av,
+                                          const f64 *alpha = nullptr) {
         using KK = Kern<K>;
         constexpr f64 Rker2 = K::Rkern * K::Rkern;
 #pragma omp parallel for schedule(dynamic, 256)
@@ -873,8 +874,11 @@ namespace orc {
                 vec3 r_ab_unit = dr * inv_sat_positive(rab);
                 f64 v_ab_r_ab  = dot(v_ab, r_ab_unit);
                 f64 abs_v      = std::fabs(v_ab_r_ab);
-                f64 vsig_a = av.alpha_AV * cs_a + av.beta_AV * abs_v;
-                f64 vsig_b = av.alpha_AV * cs_b + av.beta_AV * abs_v;
+                // per-particle alpha: NodeUpdateDerivsVaryingAlphaAV.cpp:60-150
+                f64 alpha_a = alpha ? alpha[a] : av.alpha_AV;
+                f64 alpha_b = alpha ? alpha[b] : av.alpha_AV;
+                f64 vsig_a  = alpha_a * cs_a + av.beta_AV * abs_v;
+                f64 vsig_b  = alpha_b * cs_b + av.beta_AV * abs_v;
                 f64 rho_avg = (rho_a + rho_b) * 0.5;
                 f64 vsig_u  = std::sqrt(std::fabs(P_a - P_b) / rho_avg);
                 f64 qa_ab = std::fmax(-0.5 * rho_a * vsig_a * v_ab_r_ab, 0.);
@@ -926,6 +930,167 @@ namespace orc {
                 vsig_max      = std::fmax(vsig_max, vsig_a);
             }
             vsig[a] = vsig_max;
+        }
+    }
+
+    // ------------------------------------------------------------------------------------
+    // Varying artificial viscosity, Cullen & Dehnen (2010) (AVConfig VaryingCD10)
+    // ------------------------------------------------------------------------------------
+    inline vec3 cross(vec3 a, vec3 b) {
+        return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    }
+
+    // div v and curl v: DiffOperator.cpp:24-145 (update_divv) and :147-268 (update_curlv),
+    // same loop and criterion; the two sums are independent, so one loop gives both
+    template<class K>
+    inline void compute_divv_curlv(u32 n, const ObjectCache &c, const f64 *x, const f64 *y,
+                                   const f64 *z, const f64 *vx, const f64 *vy, const f64 *vz,
+                                   const f64 *h, const f64 *omega, f64 pmass, f64 *divv,
+                                   f64 *curlx, f64 *curly, f64 *curlz) {
+        using KK = Kern<K>;
+        constexpr f64 Rker2 = K::Rkern * K::Rkern;
+#pragma omp parallel for schedule(dynamic, 256)
+        for (u32 a = 0; a < n; a++) {
+            f64 h_a = h[a];
+            vec3 xyz_a{x[a], y[a], z[a]}, v_a{vx[a], vy[a], vz[a]};
+            f64 omega_a = omega[a];
+            f64 rho_a   = rho_h(pmass, h_a, K::hfactd);
+            f64 inv_rho_omega_a = 1. / (omega_a * rho_a);
+            f64 sum_nabla_v = 0;
+            vec3 sum_cross{0, 0, 0};
+            u32 s = c.offset[a], e = s + c.cnt[a];
+            for (u32 k = s; k < e; k++) {
+                u32 b    = c.ids[k];
+                vec3 dr  = xyz_a - vec3{x[b], y[b], z[b]};
+                f64 rab2 = dot(dr, dr);
+                f64 h_b  = h[b];
+                if (rab2 > h_a * h_a * Rker2 && rab2 > h_b * h_b * Rker2) continue;
+                f64 rab        = std::sqrt(rab2);
+                vec3 v_ab      = v_a - vec3{vx[b], vy[b], vz[b]};
+                vec3 r_ab_unit = dr / rab;
+                if (rab < 1e-9) r_ab_unit = {0, 0, 0};
+                vec3 dWab_a = KK::dW_3d(rab, h_a) * r_ab_unit;
+                sum_nabla_v += pmass * dot(v_ab, dWab_a);
+                sum_cross = sum_cross + pmass * cross(v_ab, dWab_a);
+            }
+            divv[a]  = -inv_rho_omega_a * sum_nabla_v;
+            curlx[a] = -inv_rho_omega_a * sum_cross.x;
+            curly[a] = -inv_rho_omega_a * sum_cross.y;
+            curlz[a] = -inv_rho_omega_a * sum_cross.z;
+        }
+    }
+
+    // 3x3 inverse and product of row-stored matrices (matrix_legacy.hpp:25-95)
+    struct mat33 {
+        vec3 r[3];
+    };
+    inline mat33 inv_33(const mat33 &m) {
+        f64 a00 = m.r[0].x, a10 = m.r[1].x, a20 = m.r[2].x;
+        f64 a01 = m.r[0].y, a11 = m.r[1].y, a21 = m.r[2].y;
+        f64 a02 = m.r[0].z, a12 = m.r[1].z, a22 = m.r[2].z;
+        f64 det = (-a02 * a11 * a20 + a01 * a12 * a20 + a02 * a10 * a21 - a00 * a12 * a21 -
+                   a01 * a10 * a22 + a00 * a11 * a22);
+        return {{vec3{-a12 * a21 + a11 * a22, a02 * a21 - a01 * a22, -a02 * a11 + a01 * a12} / det,
+                 vec3{a12 * a20 - a10 * a22, -a02 * a20 + a00 * a22, a02 * a10 - a00 * a12} / det,
+                 vec3{-a11 * a20 + a10 * a21, a01 * a20 - a00 * a21, -a01 * a10 + a00 * a11} /
+                     det}};
+    }
+    inline mat33 prod_33(const mat33 &A, const mat33 &B) {
+        f64 a00 = A.r[0].x, a10 = A.r[1].x, a20 = A.r[2].x;
+        f64 a01 = A.r[0].y, a11 = A.r[1].y, a21 = A.r[2].y;
+        f64 a02 = A.r[0].z, a12 = A.r[1].z, a22 = A.r[2].z;
+        f64 b00 = B.r[0].x, b10 = B.r[1].x, b20 = B.r[2].x;
+        f64 b01 = B.r[0].y, b11 = B.r[1].y, b21 = B.r[2].y;
+        f64 b02 = B.r[0].z, b12 = B.r[1].z, b22 = B.r[2].z;
+        return {{vec3{a00 * b00 + a01 * b10 + a02 * b20, a00 * b01 + a01 * b11 + a02 * b21,
+                      a00 * b02 + a01 * b12 + a02 * b22},
+                 vec3{a10 * b00 + a11 * b10 + a12 * b20, a10 * b01 + a11 * b11 + a12 * b21,
+                      a10 * b02 + a11 * b12 + a12 * b22},
+                 vec3{a20 * b00 + a21 * b10 + a22 * b20, a20 * b01 + a21 * b11 + a22 * b21,
+                      a20 * b02 + a21 * b12 + a22 * b22}}};
+    }
+
+    // d(div v)/dt = div a - (grad v):(grad v)^T from the matrix-corrected gradient estimates
+    // (DiffOperatorDtDivv.cpp:89-175, update_dtdivv without div / curl)
+    template<class K>
+    inline void compute_dtdivv(u32 n, const ObjectCache &c, const f64 *x, const f64 *y,
+                               const f64 *z, const f64 *vx, const f64 *vy, const f64 *vz,
+                               const f64 *ax, const f64 *ay, const f64 *az, const f64 *h,
+                               f64 pmass, f64 *dtdivv) {
+        using KK = Kern<K>;
+        constexpr f64 Rker2 = K::Rkern * K::Rkern;
+#pragma omp parallel for schedule(dynamic, 256)
+        for (u32 a = 0; a < n; a++) {
+            f64 h_a = h[a];
+            vec3 xyz_a{x[a], y[a], z[a]}, v_a{vx[a], vy[a], vz[a]}, a_a{ax[a], ay[a], az[a]};
+            mat33 R{{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}}, Rv = R, Ra = R;
+            u32 s = c.offset[a], e = s + c.cnt[a];
+            for (u32 k = s; k < e; k++) {
+                u32 b     = c.ids[k];
+                vec3 r_ab = xyz_a - vec3{x[b], y[b], z[b]};
+                f64 rab2  = dot(r_ab, r_ab);
+                f64 h_b   = h[b];
+                if (rab2 > h_a * h_a * Rker2 && rab2 > h_b * h_b * Rker2) continue;
+                f64 rab        = std::sqrt(rab2);
+                vec3 v_ab      = v_a - vec3{vx[b], vy[b], vz[b]};
+                vec3 a_ab      = a_a - vec3{ax[b], ay[b], az[b]};
+                vec3 r_ab_unit = r_ab / rab;
+                if (rab < 1e-9) r_ab_unit = {0, 0, 0};
+                vec3 dWab_a  = KK::dW_3d(rab, h_a) * r_ab_unit;
+                vec3 mdWab_b = dWab_a * pmass;
+                R.r[0]  = R.r[0] - r_ab.x * mdWab_b;
+                R.r[1]  = R.r[1] - r_ab.y * mdWab_b;
+                R.r[2]  = R.r[2] - r_ab.z * mdWab_b;
+                Rv.r[0] = Rv.r[0] - v_ab * mdWab_b.x;
+                Rv.r[1] = Rv.r[1] - v_ab * mdWab_b.y;
+                Rv.r[2] = Rv.r[2] - v_ab * mdWab_b.z;
+                Ra.r[0] = Ra.r[0] - a_ab * mdWab_b.x;
+                Ra.r[1] = Ra.r[1] - a_ab * mdWab_b.y;
+                Ra.r[2] = Ra.r[2] - a_ab * mdWab_b.z;
+            }
+            mat33 inv = inv_33(R);
+            mat33 dv  = prod_33(inv, Rv);
+            mat33 da  = prod_33(inv, Ra);
+            f64 div_a = da.r[0].x + da.r[1].y + da.r[2].z;
+            f64 tens  = dv.r[0].x * dv.r[0].x + dv.r[1].x * dv.r[0].y + dv.r[2].x * dv.r[0].z +
+                       dv.r[0].y * dv.r[1].x + dv.r[1].y * dv.r[1].y + dv.r[2].y * dv.r[1].z +
+                       dv.r[0].z * dv.r[2].x + dv.r[1].z * dv.r[2].y + dv.r[2].z * dv.r[2].z;
+            dtdivv[a] = div_a - tens;
+        }
+    }
+
+    // alpha update: UpdateViscosity.cpp:120-219 (Cullen & Dehnen 2010: limiter from div v and
+    // curl v, source from -d(div v)/dt, implicit Euler decay on the time scale h / (c_s sigma));
+    // cs is the sound speed saved at the end of the previous step (Solver.cpp:3126-3160), 0
+    // before the first step
+    struct CD10Params {
+        f64 alpha_min, alpha_max, sigma_decay;
+    };
+    inline void update_alpha_cd10(u32 n, f64 dt, const f64 *cs, const f64 *h, const f64 *alpha,
+                                  const f64 *divv, const f64 *curlx, const f64 *curly,
+                                  const f64 *curlz, const f64 *dtdivv, CD10Params p,
+                                  f64 *alpha_new) {
+        const f64 eps = std::numeric_limits<f64>::epsilon();
+        for (u32 i = 0; i < n; i++) {
+            f64 cs_a = cs[i], h_a = h[i], alpha_a = alpha[i], divv_a = divv[i];
+            vec3 curlv_a{curlx[i], curly[i], curlz[i]};
+            f64 dtdivv_a        = dtdivv[i];
+            f64 vsig            = cs_a;
+            f64 inv_tau_a       = vsig * p.sigma_decay / h_a;
+            f64 fact_t          = dt * inv_tau_a;
+            f64 euler_impl_fact = 1 / (1 + fact_t);
+            f64 fac = std::max(-divv_a, 0.);
+            fac *= fac;
+            f64 traceS  = dot(curlv_a, curlv_a);
+            f64 balsara = (fac + traceS > eps) ? fac / (fac + traceS) : 1.;
+            f64 A_a     = balsara * std::max(-dtdivv_a, 0.);
+            f64 temp    = cs_a * cs_a;
+            f64 alpha_loc_a =
+                std::min((cs_a > 0) ? 10 * h_a * h_a * A_a / (temp) : p.alpha_min, p.alpha_max);
+            alpha_loc_a   = (temp > eps) ? alpha_loc_a : p.alpha_min;
+            f64 new_alpha = (alpha_a + alpha_loc_a * fact_t) * euler_impl_fact;
+            if (alpha_loc_a > alpha_a) new_alpha = alpha_loc_a;
+            alpha_new[i] = new_alpha;
         }
     }
 

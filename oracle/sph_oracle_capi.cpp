@@ -333,6 +333,40 @@ int orc_iterate_h(int kernel, uint32_t n, const uint32_t *cnt, const uint32_t *o
     });
 }
 
+// ---- varying-AV operators on an explicit cache ------------------------------------------
+// div v, curl v (DiffOperator.cpp:24-268) and d(div v)/dt (DiffOperatorDtDivv.cpp:89-175) of
+// the n rows of the cache over the positions / velocities / accelerations / h / Omega of all
+// the particles the rows name
+int orc_cd10_operators(int kernel, uint32_t n, const uint32_t *cnt, const uint32_t *offset,
+                       const uint32_t *ids, uint32_t nids, const double *x, const double *y,
+                       const double *z, const double *vx, const double *vy, const double *vz,
+                       const double *ax, const double *ay, const double *az, const double *h,
+                       const double *omega, double mass, double *divv, double *curlx,
+                       double *curly, double *curlz, double *dtdivv) {
+    return guard([&] {
+        ObjectCache c;
+        c.cnt.assign(cnt, cnt + n);
+        c.offset.assign(offset, offset + n);
+        c.ids.assign(ids, ids + nids);
+        with_kernel(kernel, [&](auto K) {
+            using KT = decltype(K);
+            compute_divv_curlv<KT>(n, c, x, y, z, vx, vy, vz, h, omega, mass, divv, curlx,
+                                   curly, curlz);
+            compute_dtdivv<KT>(n, c, x, y, z, vx, vy, vz, ax, ay, az, h, mass, dtdivv);
+        });
+    });
+}
+// alpha update (UpdateViscosity.cpp:120-219)
+int orc_cd10_alpha(uint32_t n, double dt, const double *cs, const double *h,
+                   const double *alpha, const double *divv, const double *curlx,
+                   const double *curly, const double *curlz, const double *dtdivv,
+                   double alpha_min, double alpha_max, double sigma_decay, double *alpha_new) {
+    return guard([&] {
+        update_alpha_cd10(n, dt, cs, h, alpha, divv, curlx, curly, curlz, dtdivv,
+                          CD10Params{alpha_min, alpha_max, sigma_decay}, alpha_new);
+    });
+}
+
 // ---- periodic ghosts ---------------------------------------------------------------------
 int orc_periodic_ghosts(uint32_t n, const double *x, const double *y, const double *z,
                         const double *box6, double int_range, uint32_t cap, uint32_t *out_src,
@@ -376,12 +410,14 @@ static int with_solver(void *h, int kernel, F &&f) {
 }
 
 extern "C" {
-// field ids: 0 x 1 y 2 z 3 vx 4 vy 5 vz 6 ax 7 ay 8 az 9 h 10 u 11 du
+// field ids: 0 x 1 y 2 z 3 vx 4 vy 5 vz 6 ax 7 ay 8 az 9 h 10 u 11 du 12 alpha_AV
+// 13 soundspeed (saved sound speed of the varying AV)
 int orc_solver_field(void *h, int kernel, int field, double *buf, int set) {
     return with_solver(h, kernel, [&](auto &s) {
-        std::vector<double> *f[12] = {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz,
-                                      &s.ax, &s.ay, &s.az, &s.h, &s.u, &s.du};
-        if (field < 0 || field > 11) throw std::invalid_argument("field id");
+        std::vector<double> *f[14] = {&s.x,  &s.y,  &s.z, &s.vx, &s.vy,       &s.vz,
+                                      &s.ax, &s.ay, &s.az, &s.h, &s.u, &s.du, &s.alpha_av,
+                                      &s.soundspeed};
+        if (field < 0 || field > 13) throw std::invalid_argument("field id");
         if (set) std::copy(buf, buf + s.N, f[field]->begin());
         else std::copy(f[field]->begin(), f[field]->end(), buf);
     });

@@ -24,6 +24,10 @@ namespace orc {
         double alpha_u, alpha_AV, beta_AV;
         double htol_up_coarse, htol_up_fine, epsilon_h; // SolverConfig.hpp:605-607
         double cfl_cour, cfl_force, cfl_multiplier_stiffness;
+        // artificial viscosity: 0 constant (alpha_AV, beta_AV, alpha_u), 1 varying Cullen &
+        // Dehnen 2010 (AVConfig.hpp VaryingCD10: alpha_min, alpha_max, sigma_decay)
+        int32_t av_kind;
+        double alpha_min, alpha_max, sigma_decay;
     };
 
     struct StepStats {
@@ -45,6 +49,8 @@ namespace orc {
         OrcConfig cfg;
         u32 N = 0;
         std::vector<f64> x, y, z, vx, vy, vz, ax, ay, az, h, u, du;
+        // varying AV: per-particle alpha and the sound speed saved at the end of each step
+        std::vector<f64> alpha_av, soundspeed;
         f64 t = 0, dt = 0, cfl_mult = 1e-2; // Solver.hpp:124-147 time-state edges
         StepStats stats{};
 
@@ -53,6 +59,7 @@ namespace orc {
         std::vector<u32> ghost_src;
         std::vector<vec3> ghost_off;
         std::vector<f64> mx, my, mz, mh, mvx, mvy, mvz, mu, momega, P, cs, vsig, omega;
+        std::vector<f64> divv, curlx, curly, curlz, dtdivv, alpha_new, malpha, max_, may_, maz_;
         vec3 tree_bmin{}, tree_bmax{};
         CLBVH<u32> tree;
         std::vector<f64> rint_tree;
@@ -71,7 +78,9 @@ namespace orc {
 
         void resize(u32 n) {
             N = n;
-            for (auto *v : {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az, &h, &u, &du}) v->assign(n, 0.);
+            for (auto *v : {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az, &h, &u, &du, &alpha_av,
+                            &soundspeed})
+                v->assign(n, 0.);
         }
 
         // ghosts + merge of positions and h: Solver.cpp:984-1046, BasicSPHGhosts.cpp:260-579
@@ -139,6 +148,39 @@ namespace orc {
                 mu[N + g]     = u[s];
                 momega[N + g] = omega[s];
             }
+        }
+
+        // varying AV (Solver.cpp:2262-2361): div v, curl v and d(div v)/dt on the merged fields
+        // (accelerations of the images from their sources), the Cullen & Dehnen update of alpha
+        // (not yet written back), then the merged alpha for the force
+        void cd10_alpha(f64 dtc) {
+            u32 G = (u32) ghost_src.size();
+            for (auto *v : {&max_, &may_, &maz_, &malpha}) v->resize(Nm);
+            for (u32 i = 0; i < N; i++) {
+                max_[i] = ax[i];
+                may_[i] = ay[i];
+                maz_[i] = az[i];
+            }
+            for (u32 g = 0; g < G; g++) {
+                u32 s        = ghost_src[g];
+                max_[N + g] = ax[s];
+                may_[N + g] = ay[s];
+                maz_[N + g] = az[s];
+            }
+            for (auto *v : {&divv, &curlx, &curly, &curlz, &dtdivv, &alpha_new}) v->assign(N, 0.);
+            compute_divv_curlv<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(),
+                                  mvx.data(), mvy.data(), mvz.data(), mh.data(), momega.data(),
+                                  cfg.gpart_mass, divv.data(), curlx.data(), curly.data(),
+                                  curlz.data());
+            compute_dtdivv<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(), mvx.data(),
+                              mvy.data(), mvz.data(), max_.data(), may_.data(), maz_.data(),
+                              mh.data(), cfg.gpart_mass, dtdivv.data());
+            update_alpha_cd10(N, dtc, soundspeed.data(), h.data(), alpha_av.data(), divv.data(),
+                              curlx.data(), curly.data(), curlz.data(), dtdivv.data(),
+                              CD10Params{cfg.alpha_min, cfg.alpha_max, cfg.sigma_decay},
+                              alpha_new.data());
+            for (u32 i = 0; i < N; i++) malpha[i] = alpha_new[i];
+            for (u32 g = 0; g < G; g++) malpha[N + g] = alpha_new[ghost_src[g]];
         }
 
         // Solver.cpp:1060-1304
@@ -230,6 +272,7 @@ namespace orc {
                 if (corr_iter == 50)
                     throw std::runtime_error("the corrector has made over 50 loops");
                 merge_fields();
+                if (cfg.av_kind == 1) cd10_alpha(dtc);
                 Timer te;
                 P.assign(Nm, 0.);
                 cs.assign(Nm, 0.);
@@ -246,7 +289,8 @@ namespace orc {
                                              mvx.data(), mvy.data(), mvz.data(), mh.data(),
                                              momega.data(), mu.data(), P.data(), cs.data(),
                                              ax.data(), ay.data(), az.data(), du.data(),
-                                             cfg.gpart_mass, av);
+                                             cfg.gpart_mass, av,
+                                             cfg.av_kind == 1 ? malpha.data() : nullptr);
                 stats.t_force += tf.sec();
                 Timer tc;
                 // leapfrog_corrector: integrators.cpp:87-119 with hdt = dt/2
@@ -276,6 +320,8 @@ namespace orc {
                     cfl_mult /= 2;
                 } else {
                     rerun = false;
+                    // write back the updated alpha (Solver.cpp:2622-2650)
+                    if (cfg.av_kind == 1) alpha_av = alpha_new;
                     Timer tl;
                     vsig.assign(N, 0.);
                     compute_vsig<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(),
@@ -300,6 +346,9 @@ namespace orc {
                 corr_iter++;
             } while (rerun);
             stats.corrector_iterations = corr_iter;
+            // sound speed kept for the next step's viscosity update (Solver.cpp:3126-3160)
+            if (cfg.av_kind == 1)
+                for (u32 i = 0; i < N; i++) soundspeed[i] = cs[i];
 
             stats.t_total  = tstep.sec();
             stats.n_merged = Nm;

@@ -33,7 +33,8 @@ class ShamxConfig(C.Structure):
         ("htol_up_coarse", C.c_double), ("htol_up_fine", C.c_double), ("epsilon_h", C.c_double),
         ("cfl_cour", C.c_double), ("cfl_force", C.c_double),
         ("cfl_multiplier_stiffness", C.c_double),
-        ("mixed_precision", C.c_int32), ("reserved", C.c_int32),
+        ("mixed_precision", C.c_int32), ("av_kind", C.c_int32),
+        ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("sigma_decay", C.c_double),
     ]
 
 
@@ -72,6 +73,8 @@ SIGNATURES = {
     "shamx_morton": (_i, [_P, _P, _u32, _P, _P, _P, _P, _P]),
     "shamx_sort": (_i, [_P, _P, _u32, _u32, _P, _P]),
     "shamx_tree_build": (_i, [_P, _P, _u32, _P, _P, _P, _P, _u32, _d]),
+    "shamx_tree_reduce_karras": (_i, [_P, _P, _u32, _P, _P, _P, _u32]),
+    "shamx_tree_aabb_hmax": (_i, [_P, _P, _P, _P, _P, _P, _d]),
     "shamx_tree_sizes": (_i, [_P, C.POINTER(C.c_uint32)]),
     "shamx_tree_get": (_i, [_P, _P] + [_P] * 11),
     "shamx_neigh_build": (_i, [_P, _P, _u32, _P, _P, _P, _P, _d, _d]),
@@ -82,8 +85,11 @@ SIGNATURES = {
     "shamx_omega": (_i, [_P, _i, _u32, _P, _P, _P, _P, _P, _P, _P, _P, _d]),
     "shamx_eos": (_i, [_P, _i, _u32, _i, _d, _d, _d, _d, _d, _P, _P, _P, _P, _P, _P, _P, _d]),
     "shamx_force": (_i, [_P, _i, _u32] + [_P] * 24 + [_d] * 6 + [_i, _dp]),
+    "shamx_cfl": (_i, [_P, _u32] + [_P] * 5 + [_d, _d, _P, _dp]),
     "shamx_predictor": (_i, [_P, _u32, _d] + [_P] * 11 + [_i, _dp]),
     "shamx_corrector": (_i, [_P, _u32, _d] + [_P] * 12 + [_d, _dp]),
+    "shamx_solver_adjoint_step": (_i, [_P, C.POINTER(_P)]),
+    "shamx_solver_adjoint_density": (_i, [_P, _P, _P, _P, _P]),
     "shamx_solver_create": (_i, [C.POINTER(ShamxConfig), _u32, C.POINTER(_P)]),
     "shamx_solver_destroy": (_i, [_P]),
     "shamx_solver_set_field": (_i, [_P, _i, _P, _i]),

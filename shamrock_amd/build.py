@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libshamx_sph.so")
-SOURCES = ["prims.hip", "tree.hip", "neigh.hip", "sph.hip", "capi.cpp"]
+SOURCES = ["prims.hip", "tree.hip", "neigh.hip", "sph.hip", "adjoint.hip", "capi.cpp"]
 ARCH = os.environ.get("SHAMX_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

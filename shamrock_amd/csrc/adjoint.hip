@@ -1,0 +1,345 @@
+// SPDX-License-Identifier: CeCILL-2.1
+//
+// adjoint.hip -- hand-written adjoint (vector-Jacobian products) of one SPH step: density
+// (smoothing length through its density equation), Omega, EOS, force (pressure, constant AV,
+// conductivity) and the leapfrog.  No reference counterpart (the reference has no adjoint).
+//
+// The step map differentiated, for a step of fixed dt:
+//   predictor   x' = x + dt v + dt^2/2 a,  v' = v + dt a,  u' = u + dt du     (Solver.cpp:390-524)
+//   h'          solves sum_b m W(|x'_a - x'_b|, h_a) = rho_h(h_a)                 (implicit)
+//   Omega', P', c_s'  from (x', h', u')                     (ComputeOmega.cpp, ComputeEos.cpp)
+//   a', du'     = force(x', v', u', h', Omega')                               (UpdateDerivs.cpp)
+//   corrector   v'' = v' + dt/2 (a' - a),  u'' = u' + dt/2 (du' - du)      (integrators.cpp:87)
+// with state (x, v, u, a, du).  The smoothing length is differentiated through the implicit
+// function theorem: dh_a/dx = -(dF_a/dx) / (dF_a/dh_a), dF_a/dh_a = 3 rho_a Omega_a / h_a.
+// All pair loops are in gather form over symmetric neighbour rows (the force criterion
+// max(h_a, h_b) and the cache criterion are symmetric): particle a adds the derivative of its
+// own pair terms and of the mirrored terms of its neighbours, so there are no atomics.  Ghost
+// images carry the seeds of their source particles.
+#include "adjoint_pair.hpp"
+#include "launch.hpp"
+
+namespace shamx {
+
+    namespace {
+        constexpr int ABLK = 128;
+
+        struct Rows1 {
+            const u32 *sel, *outi, *cnt, *off, *ids;
+        };
+
+        // out[k][s] = in[k][source of the merged particle at sorted position s]
+        struct Gather {
+            const f64 *in[4];
+            f64 *out[4];
+            int n;
+        };
+        __global__ void k_gather_sorted(u32 nm, u32 N, const u32 *__restrict sort_index,
+                                        const u32 *__restrict ghost_src, Gather g) {
+            u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+            if (s >= nm) return;
+            u32 b   = sort_index[s];
+            u32 src = b < N ? b : ghost_src[b - N];
+            for (int k = 0; k < g.n; k++) g.out[k][s] = g.in[k][src];
+        }
+
+        // force-record layout (sph.hip): x y z h vx vy vz u P cs omega rho ...
+        struct FRecView {
+            f64 x, y, z, h, vx, vy, vz, u, P, cs, omega, rho, inv_sub, romega, hR2, pad;
+        };
+
+        struct ForceGrad {
+            f64 *gx, *gy, *gz, *gvx, *gvy, *gvz, *gu, *gh, *gom;
+        };
+
+        template<class K>
+        __global__ void __launch_bounds__(ABLK)
+            k_force_vjp(u32 n, Rows1 r, const FRecView *__restrict rec,
+                        const f64 *__restrict sax, const f64 *__restrict say,
+                        const f64 *__restrict saz, const f64 *__restrict sdu, PairParams pp,
+                        ForceGrad o) {
+            u32 j = blockIdx.x * ABLK + threadIdx.x;
+            if (j >= n) return;
+            u32 row = r.sel[j], a = r.outi[row];
+            const FRecView A = rec[row];
+            const PairSide SA{{A.vx, A.vy, A.vz}, A.u, A.h, A.omega};
+            const f64 seed_a[3] = {sax[row], say[row], saz[row]};
+            const f64 sdu_a     = sdu[row];
+            f64 gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gu = 0, gh = 0, gom = 0;
+            u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+            for (u32 k = k0; k < k1; k++) {
+                u32 b = r.ids[k];
+                if (b == row) continue;
+                const FRecView B = rec[b];
+                const PairSide SB{{B.vx, B.vy, B.vz}, B.u, B.h, B.omega};
+                f64 rab[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+                f64 acc[3], du, gr[3];
+                PairGrad g1, g2;
+                // own term: a's outputs seeded by a's seeds
+                force_pair<K>(pp, rab, SA, SB, acc, du, seed_a, sdu_a, gr, g1, g2);
+                for (int c = 0; c < 3; c++) {
+                    gx[c] += gr[c];
+                    gv[c] += g1.v[c];
+                }
+                gu += g1.u;
+                gh += g1.h;
+                gom += g1.omega;
+                // mirrored term: b's outputs (pair b, a) seeded by b's seeds; a is side B
+                const f64 seed_b[3] = {sax[b], say[b], saz[b]};
+                f64 rba[3]          = {-rab[0], -rab[1], -rab[2]};
+                force_pair<K>(pp, rba, SB, SA, acc, du, seed_b, sdu[b], gr, g1, g2);
+                for (int c = 0; c < 3; c++) {
+                    gx[c] -= gr[c];
+                    gv[c] += g2.v[c];
+                }
+                gu += g2.u;
+                gh += g2.h;
+                gom += g2.omega;
+            }
+            o.gx[a]  = gx[0];
+            o.gy[a]  = gx[1];
+            o.gz[a]  = gx[2];
+            o.gvx[a] = gv[0];
+            o.gvy[a] = gv[1];
+            o.gvz[a] = gv[2];
+            o.gu[a]  = gu;
+            o.gh[a]  = gh;
+            o.gom[a] = gom;
+        }
+
+        // Omega_a = 1 + C_a sum_b m dW/dh(d_ab, h_a), C = h / (3 rho_h(h)) = h^4 / (3 m hfact^3),
+        // over d_ab <= R h_a: x-gradient (own + mirrored) added to g, h-gradient to gh
+        template<class K>
+        __global__ void __launch_bounds__(ABLK)
+            k_omega_vjp(u32 n, Rows1 r, const double4 *__restrict ps,
+                        const f64 *__restrict s_omb, f64 mass, f64 hfact, f64 *__restrict gx,
+                        f64 *__restrict gy, f64 *__restrict gz, f64 *__restrict gh) {
+            u32 j = blockIdx.x * ABLK + threadIdx.x;
+            if (j >= n) return;
+            const f64 R = K::Rkern;
+            u32 row = r.sel[j], a = r.outi[row];
+            double4 A = ps[row];
+            f64 h_a   = A.w;
+            f64 rho_a = rho_h(mass, h_a, hfact);
+            f64 C_a   = h_a / (3 * rho_a);
+            f64 ob_a  = s_omb[row];
+            f64 S1 = 0, S2 = 0, g[3] = {0, 0, 0};
+            u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+            for (u32 k = k0; k < k1; k++) {
+                u32 b     = r.ids[k];
+                double4 B = ps[b];
+                f64 rr[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+                f64 d     = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+                f64 inv   = d > 1e-300 ? 1. / d : 0.;
+                if (d <= R * h_a) {
+                    KernDeriv<K> kd(d, h_a);
+                    S1 += mass * kd.Wh;
+                    S2 += mass * kd.Whh;
+                    f64 c = ob_a * C_a * mass * kd.Wrh * inv;
+                    for (int t = 0; t < 3; t++) g[t] += c * rr[t];
+                }
+                f64 h_b = B.w;
+                if (b != row && d <= R * h_b) {
+                    KernDeriv<K> kd(d, h_b);
+                    f64 C_b = h_b / (3 * rho_h(mass, h_b, hfact));
+                    f64 c   = s_omb[b] * C_b * mass * kd.Wrh * inv;
+                    for (int t = 0; t < 3; t++) g[t] += c * rr[t];
+                }
+            }
+            gx[a] += g[0];
+            gy[a] += g[1];
+            gz[a] += g[2];
+            // dOmega/dh = (4 C / h) S1 + C S2
+            gh[a] += ob_a * ((4 * C_a / h_a) * S1 + C_a * S2);
+        }
+
+        // x-gradient of sum_a lambda_a F_a, F_a = sum_b m W(d_ab, h_a) - rho_h(h_a)
+        template<class K>
+        __global__ void __launch_bounds__(ABLK)
+            k_ift_vjp(u32 n, Rows1 r, const double4 *__restrict ps, const f64 *__restrict s_lam,
+                      f64 mass, f64 *__restrict gx, f64 *__restrict gy, f64 *__restrict gz) {
+            u32 j = blockIdx.x * ABLK + threadIdx.x;
+            if (j >= n) return;
+            const f64 R = K::Rkern;
+            u32 row = r.sel[j], a = r.outi[row];
+            double4 A = ps[row];
+            f64 lam_a = s_lam[row];
+            f64 g[3]  = {0, 0, 0};
+            u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+            for (u32 k = k0; k < k1; k++) {
+                u32 b = r.ids[k];
+                if (b == row) continue;
+                double4 B = ps[b];
+                f64 rr[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+                f64 d     = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+                if (!(d > 0)) continue;
+                f64 c = 0;
+                if (d <= R * A.w) c += lam_a * mass * KernDeriv<K>(d, A.w).Wr;
+                if (d <= R * B.w) c += s_lam[b] * mass * KernDeriv<K>(d, B.w).Wr;
+                c /= d;
+                for (int t = 0; t < 3; t++) g[t] += c * rr[t];
+            }
+            gx[a] += g[0];
+            gy[a] += g[1];
+            gz[a] += g[2];
+        }
+
+        // lambda_a = -hbar_a / (3 rho_a Omega_a / h_a)
+        __global__ void k_lambda(u32 n, const f64 *__restrict h, const f64 *__restrict omega,
+                                 const f64 *__restrict hbar, f64 mass, f64 hfact,
+                                 f64 *__restrict lam) {
+            u32 a = blockIdx.x * blockDim.x + threadIdx.x;
+            if (a >= n) return;
+            f64 rho = rho_h(mass, h[a], hfact);
+            lam[a]  = -hbar[a] / (3 * rho * omega[a] / h[a]);
+        }
+
+        // hbar_a = rho_bar_a * d rho_h / dh = -3 rho_a / h_a * rho_bar_a
+        __global__ void k_rho_to_h(u32 n, const f64 *__restrict h, const f64 *__restrict rbar,
+                                   f64 mass, f64 hfact, f64 *__restrict hbar) {
+            u32 a = blockIdx.x * blockDim.x + threadIdx.x;
+            if (a >= n) return;
+            hbar[a] = rbar[a] * (-3 * rho_h(mass, h[a], hfact) / h[a]);
+        }
+
+        // corrector adjoint: force seeds and the a/du seeds of the step start
+        struct StepSeeds {
+            f64 *x[3], *v[3], *u, *a[3], *du;
+        };
+        __global__ void k_corrector_vjp(u32 n, f64 hdt, StepSeeds s, f64 *__restrict fax,
+                                        f64 *__restrict fay, f64 *__restrict faz,
+                                        f64 *__restrict fdu) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            f64 *fa[3] = {fax, fay, faz};
+            for (int c = 0; c < 3; c++) {
+                f64 vb       = s.v[c][i];
+                fa[c][i]     = s.a[c][i] + hdt * vb; // seed of a' (= a_{n+1} and in v'')
+                s.a[c][i]    = -hdt * vb;            // partial seed of a_n
+            }
+            f64 ub   = s.u[i];
+            fdu[i]   = s.du[i] + hdt * ub;
+            s.du[i]  = -hdt * ub;
+        }
+
+        // add the force / Omega / IFT gradients, then the predictor adjoint
+        __global__ void k_predictor_vjp(u32 n, f64 dt, StepSeeds s, ForceGrad g) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            f64 gxv[3] = {g.gx[i], g.gy[i], g.gz[i]}, gvv[3] = {g.gvx[i], g.gvy[i], g.gvz[i]};
+            for (int c = 0; c < 3; c++) {
+                f64 xb = s.x[c][i] + gxv[c]; // seed of x'
+                f64 vb = s.v[c][i] + gvv[c]; // seed of v'
+                s.x[c][i] = xb;
+                s.v[c][i] = vb + dt * xb;
+                s.a[c][i] += dt * vb + (dt * (dt * 0.5)) * xb;
+            }
+            f64 ub  = s.u[i] + g.gu[i];
+            s.u[i]  = ub;
+            s.du[i] += dt * ub;
+        }
+
+        template<class F>
+        void dispatch_adjoint_kernel(int id, F &&f) {
+            switch (id) {
+            case 0: f(M4{}); break;
+            case 1: f(M6{}); break;
+            case 2: f(M8{}); break;
+            case 3: f(C2{}); break;
+            default:
+                throw std::invalid_argument("adjoint: available for the M4, M6, M8 and C2 kernels");
+            }
+        }
+    } // namespace
+
+    void adjoint_gather_sorted(hipStream_t s, u32 nm, u32 N, const u32 *sort_index,
+                               const u32 *ghost_src, int nf, const f64 *const *in,
+                               f64 *const *out) {
+        if (nm == 0) return;
+        Gather g{};
+        g.n = nf;
+        for (int k = 0; k < nf; k++) {
+            g.in[k]  = in[k];
+            g.out[k] = out[k];
+        }
+        k_gather_sorted<<<div_up(nm, 256), 256, 0, s>>>(nm, N, sort_index, ghost_src, g);
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void adjoint_step(hipStream_t s, const AdjointArgs &A, f64 *const *seeds,
+                      DevArray<f64> &work) {
+        const u32 N = A.n, nm = A.nm;
+        if (N == 0) return;
+        // work: 4 force seeds (local) + 4 sorted force seeds + 9 gradients + 2 sorted (Omega
+        // bar, lambda) + lambda (local)
+        size_t need = (size_t) 4 * N + 4 * (size_t) nm + 9 * (size_t) N + 2 * (size_t) nm + N;
+        work.resize(need);
+        f64 *w    = work.data();
+        f64 *fa[4] = {w, w + N, w + 2 * (size_t) N, w + 3 * (size_t) N};
+        w += 4 * (size_t) N;
+        f64 *sf[4] = {w, w + nm, w + 2 * (size_t) nm, w + 3 * (size_t) nm};
+        w += 4 * (size_t) nm;
+        ForceGrad g{w,         w + N,     w + 2 * N, w + 3 * N, w + 4 * N,
+                    w + 5 * N, w + 6 * N, w + 7 * N, w + 8 * N};
+        w += 9 * (size_t) N;
+        f64 *s_omb = w, *s_lam = w + nm;
+        w += 2 * (size_t) nm;
+        f64 *lam = w;
+
+        StepSeeds ss{{seeds[0], seeds[1], seeds[2]},
+                     {seeds[3], seeds[4], seeds[5]},
+                     seeds[6],
+                     {seeds[7], seeds[8], seeds[9]},
+                     seeds[10]};
+        const f64 dt = A.dt;
+        k_corrector_vjp<<<div_up(N, 256), 256, 0, s>>>(N, dt / 2, ss, fa[0], fa[1], fa[2], fa[3]);
+        const f64 *fin[4] = {fa[0], fa[1], fa[2], fa[3]};
+        adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 4, fin, sf);
+        PairParams pp{A.mass,     A.hfact,   A.gamma,    A.alpha_u, A.alpha_AV,
+                      A.beta_AV,  A.eos_kind, A.cs_iso};
+        Rows1 rf{A.lpos, A.sort_index, A.fcnt, A.offset, A.fids};
+        Rows1 rc{A.lpos, A.sort_index, A.cnt, A.offset, A.ids};
+        dispatch_adjoint_kernel(A.kernel, [&](auto K) {
+            using KT = decltype(K);
+            k_force_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(
+                N, rf, (const FRecView *) A.frec, sf[0], sf[1], sf[2], sf[3], pp, g);
+            // Omega bar of the merged particles, then the Omega adjoint
+            const f64 *oin[1] = {g.gom};
+            f64 *oout[1]      = {s_omb};
+            adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 1, oin, oout);
+            k_omega_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(N, rc, A.ps, s_omb, A.mass,
+                                                             A.hfact, g.gx, g.gy, g.gz, g.gh);
+            // smoothing length through its density equation
+            k_lambda<<<div_up(N, 256), 256, 0, s>>>(N, A.h, A.omega, g.gh, A.mass, A.hfact,
+                                                    lam);
+            const f64 *lin[1] = {lam};
+            f64 *lout[1]      = {s_lam};
+            adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 1, lin, lout);
+            k_ift_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(N, rc, A.ps, s_lam, A.mass, g.gx,
+                                                           g.gy, g.gz);
+        });
+        k_predictor_vjp<<<div_up(N, 256), 256, 0, s>>>(N, dt, ss, g);
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void adjoint_density(hipStream_t s, const AdjointArgs &A, const f64 *rho_bar, f64 *gx,
+                         f64 *gy, f64 *gz, DevArray<f64> &work) {
+        const u32 N = A.n, nm = A.nm;
+        if (N == 0) return;
+        work.resize((size_t) 2 * N + nm);
+        f64 *hbar = work.data(), *lam = hbar + N, *s_lam = lam + N;
+        k_rho_to_h<<<div_up(N, 256), 256, 0, s>>>(N, A.h, rho_bar, A.mass, A.hfact, hbar);
+        k_lambda<<<div_up(N, 256), 256, 0, s>>>(N, A.h, A.omega, hbar, A.mass, A.hfact, lam);
+        const f64 *lin[1] = {lam};
+        f64 *lout[1]      = {s_lam};
+        adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 1, lin, lout);
+        Rows1 rc{A.lpos, A.sort_index, A.cnt, A.offset, A.ids};
+        dispatch_adjoint_kernel(A.kernel, [&](auto K) {
+            using KT = decltype(K);
+            k_ift_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(N, rc, A.ps, s_lam, A.mass, gx, gy,
+                                                           gz);
+        });
+        SHAMX_HIP(hipGetLastError());
+    }
+
+} // namespace shamx

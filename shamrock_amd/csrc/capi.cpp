@@ -121,6 +121,9 @@ struct shamx_solver {
     DevArray<f64> x, y, z, h, vx, vy, vz, u, omega, P, cs;
     // local arrays
     DevArray<f64> ax, ay, az, du, ax_old, ay_old, az_old, du_old, eps, h_old, vsig, dtc;
+    // varying AV (Cullen & Dehnen 2010): alpha and the sound speed saved at the end of the
+    // previous step (local), the updated alpha, 5 operator arrays (local), sorted accelerations
+    DevArray<f64> alpha_av, soundspeed, alpha_new, cd10_work, sacc;
     DevArray<u32> ghost_src, gwork;
     DevArray<u8> ghost_oid;
     DevArray<unsigned char> gscratch;
@@ -133,8 +136,9 @@ struct shamx_solver {
     PhaseTimer timer;
 
     DevArray<f64> *field(int id) {
-        DevArray<f64> *f[12] = {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az, &h, &u, &du};
-        need(id >= 0 && id < 12, "field id must be in [0, 11]");
+        DevArray<f64> *f[14] = {&x, &y, &z, &vx, &vy, &vz, &ax,       &ay,
+                                &az, &h, &u, &du, &alpha_av, &soundspeed};
+        need(id >= 0 && id < 14, "field id must be in [0, 13]");
         return f[id];
     }
 
@@ -143,10 +147,11 @@ struct shamx_solver {
         cap = n + n / 3 + 1024;
         for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega, &P, &cs}) a->resize(cap);
         for (auto *a : {&ax, &ay, &az, &du, &ax_old, &ay_old, &az_old, &du_old, &eps, &h_old,
-                        &vsig, &dtc})
+                        &vsig, &dtc, &alpha_av, &soundspeed, &alpha_new})
             a->resize(std::max<u32>(n, 1));
         red.resize(8);
-        for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega, &ax, &ay, &az, &du})
+        for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega, &ax, &ay, &az, &du,
+                        &alpha_av, &soundspeed})
             fill_f64(stream, a->data(), 0., a->capacity());
         SHAMX_HIP(hipStreamSynchronize(stream));
     }
@@ -296,6 +301,25 @@ struct shamx_solver {
                                cfg.eos_r0},
                        cfg.mixed_precision);
             timer.end(stream, PH_EOS, b);
+            if (cfg.av_kind == 1) {
+                // Solver.cpp:2370-2470: div v, curl v, d(div v)/dt from the previous
+                // accelerations, alpha update over this dt, alpha into the records
+                b = timer.begin(stream);
+                u32 Nm = N + G;
+                sacc.resize(3 * (size_t) std::max<u32>(Nm, 1));
+                cd10_work.resize(5 * (size_t) N);
+                const f64 *acc_in[3] = {ax.data(), ay.data(), az.data()};
+                f64 *acc_out[3] = {sacc.data(), sacc.data() + Nm, sacc.data() + 2 * (size_t) Nm};
+                adjoint_gather_sorted(stream, Nm, N, tree.sort_index.data(), ghost_src.data(), 3,
+                                      acc_in, acc_out);
+                f64 *w5[5];
+                for (int k = 0; k < 5; k++) w5[k] = cd10_work.data() + k * (size_t) N;
+                cd10_alpha(stream, cfg.kernel, N, Nm, csr_force(), frec.data(),
+                           tree.sort_index.data(), ghost_src.data(), acc_out, soundspeed.data(),
+                           h.data(), alpha_av.data(), dtc_, cfg.gpart_mass, cfg.alpha_min,
+                           cfg.alpha_max, cfg.sigma_decay, w5, alpha_new.data());
+                timer.end(stream, PH_EOS, b);
+            }
             b = timer.begin(stream);
             ForceArgs fa{cfg.gpart_mass,
                          cfg.alpha_u,
@@ -303,7 +327,8 @@ struct shamx_solver {
                          cfg.beta_AV,
                          cfg.cfl_cour * cfl_mult,
                          cfg.cfl_force * cfl_mult,
-                         cfg.mixed_precision};
+                         cfg.mixed_precision,
+                         cfg.av_kind == 1};
             compute_force(stream, cfg.kernel, N, csr_force(), frec.data(), x.data(), y.data(),
                           z.data(), vx.data(), vy.data(), vz.data(), h.data(), omega.data(),
                           u.data(), P.data(), cs.data(), ax.data(), ay.data(), az.data(),
@@ -337,9 +362,17 @@ struct shamx_solver {
             } else {
                 rerun    = false;
                 next_cfl = dt_min;
+                // the updated alpha is kept once the step is accepted (Solver.cpp:2622-2650)
+                if (cfg.av_kind == 1)
+                    SHAMX_HIP(hipMemcpyAsync(alpha_av.data(), alpha_new.data(), N * sizeof(f64),
+                                             hipMemcpyDeviceToDevice, stream));
             }
             corr_iter++;
         } while (rerun);
+        // sound speed kept for the next step's viscosity update (Solver.cpp:3126-3160)
+        if (cfg.av_kind == 1)
+            SHAMX_HIP(hipMemcpyAsync(soundspeed.data(), cs.data(), N * sizeof(f64),
+                                     hipMemcpyDeviceToDevice, stream));
         SHAMX_HIP(hipStreamSynchronize(stream));
         auto t1 = std::chrono::steady_clock::now();
 
@@ -432,6 +465,42 @@ struct shamx_solver {
         }
     }
 
+    // the adjoint of the last step reads the step's own intermediate products
+    DevArray<f64> adj_work;
+    AdjointArgs adjoint_args() {
+        need(N > 0 && tree.L > 0, "adjoint: run evolve_once first");
+        need(cfg.eos_kind == 0 || cfg.eos_kind == 1,
+             "adjoint: available for the adiabatic and isothermal equations of state");
+        need(!cfg.mixed_precision, "adjoint: the step must run in FP64");
+        need(cfg.av_kind == 0, "adjoint: available for the constant artificial viscosity");
+        f64 hfact = 0;
+        dispatch_kernel(cfg.kernel, [&](auto K) { hfact = decltype(K)::hfactd; });
+        return AdjointArgs{N,
+                           N + G,
+                           cfg.kernel,
+                           last.dt,
+                           cfg.gpart_mass,
+                           hfact,
+                           cfg.gamma,
+                           cfg.alpha_u,
+                           cfg.alpha_AV,
+                           cfg.beta_AV,
+                           cfg.eos_kind,
+                           cfg.eos_cs,
+                           tree.sort_index.data(),
+                           G ? ghost_src.data() : nullptr,
+                           nb.lpos.data(),
+                           nb.cnt.data(),
+                           nb.offset.data(),
+                           nb.ids.data(),
+                           nb.fcnt.data(),
+                           nb.fids.data(),
+                           frec.data(),
+                           nb.ps.data(),
+                           h.data(),
+                           omega.data()};
+    }
+
     ~shamx_solver() {
         if (stream) (void) hipStreamDestroy(stream);
     }
@@ -512,6 +581,25 @@ int shamx_tree_build(shamx_ctx *ctx, void *stream, uint32_t n, const double *x, 
     return guarded([&] {
         need(ctx != nullptr, "shamx_tree_build: null context");
         tree_build((hipStream_t) stream, ctx->tree, n, x, y, z, h, reduction_level, htol);
+    });
+}
+
+int shamx_tree_reduce_karras(shamx_ctx *ctx, void *stream, uint32_t n, const double *x,
+                             const double *y, const double *z, uint32_t reduction_level) {
+    return guarded([&] {
+        need(ctx != nullptr, "shamx_tree_reduce_karras: null context");
+        hipStream_t s = (hipStream_t) stream;
+        tree_stage_codes(s, ctx->tree, n, x, y, z);
+        tree_stage_sort(s, ctx->tree);
+        tree_stage_karras(s, ctx->tree, reduction_level);
+    });
+}
+
+int shamx_tree_aabb_hmax(shamx_ctx *ctx, void *stream, const double *x, const double *y,
+                         const double *z, const double *h, double htol) {
+    return guarded([&] {
+        need(ctx && ctx->tree.L > 0, "shamx_tree_aabb_hmax: build the tree first");
+        tree_stage_aabb((hipStream_t) stream, ctx->tree, x, y, z, h, htol);
     });
 }
 
@@ -658,6 +746,23 @@ int shamx_force(void *stream, int kernel, uint32_t n, const uint32_t *cnt, const
     });
 }
 
+int shamx_cfl(void *stream, uint32_t n, const double *h, const double *vsig, const double *ax,
+              const double *ay, const double *az, double C_cour, double C_force, double *dt_cfl,
+              double *dt_min_host) {
+    return guarded([&] {
+        hipStream_t s = (hipStream_t) stream;
+        u64 *sl       = slots();
+        fill_u64(s, sl, ORDERED_POS_INF, 1);
+        cfl_dt(s, n, h, vsig, ax, ay, az, C_cour, C_force, dt_cfl, sl, slot_state().part);
+        if (dt_min_host) {
+            u64 k;
+            SHAMX_HIP(hipMemcpyAsync(&k, sl, 8, hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            *dt_min_host = ordered_to_f64(k);
+        }
+    });
+}
+
 int shamx_predictor(void *stream, uint32_t n, double dt, double *x, double *y, double *z,
                     double *vx, double *vy, double *vz, const double *ax, const double *ay,
                     const double *az, double *u, const double *du, int periodic,
@@ -701,6 +806,10 @@ int shamx_solver_create(const shamx_config *cfg, uint32_t n, shamx_solver **out)
         need(cfg->kernel >= 0 && cfg->kernel <= 5, "unknown SPH kernel");
         need(cfg->eos_kind >= 0 && cfg->eos_kind <= 2, "unknown equation of state");
         need(cfg->tree_reduction_level >= 0, "tree_reduction_level must be >= 0");
+        need(cfg->av_kind == 0 || cfg->av_kind == 1,
+             "av_kind must be 0 (constant) or 1 (varying, Cullen & Dehnen 2010)");
+        need(!(cfg->av_kind == 1 && cfg->mixed_precision),
+             "the varying artificial viscosity runs in FP64 only");
         need(cfg->htol_up_coarse >= cfg->htol_up_fine,
              "htol_up_coarse_cycle must be larger or equal to htol_up_fine_cycle");
         if (cfg->periodic)
@@ -759,6 +868,26 @@ int shamx_solver_evolve_once(shamx_solver *s, shamx_step_log *log) {
     return guarded([&] {
         s->evolve_once();
         if (log) *log = s->last;
+    });
+}
+
+int shamx_solver_adjoint_step(shamx_solver *s, double *const *seeds11) {
+    return guarded([&] {
+        need(s && seeds11, "shamx_solver_adjoint_step: null argument");
+        need(s->last.corrector_iterations == 1,
+             "adjoint: the last step reran its corrector (lower the CFL)");
+        for (int k = 0; k < 11; k++) need(seeds11[k] != nullptr, "adjoint: null seed array");
+        adjoint_step(s->stream, s->adjoint_args(), seeds11, s->adj_work);
+        SHAMX_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int shamx_solver_adjoint_density(shamx_solver *s, const double *rho_bar, double *xbar,
+                                 double *ybar, double *zbar) {
+    return guarded([&] {
+        need(s && rho_bar && xbar && ybar && zbar, "shamx_solver_adjoint_density: null argument");
+        adjoint_density(s->stream, s->adjoint_args(), rho_bar, xbar, ybar, zbar, s->adj_work);
+        SHAMX_HIP(hipStreamSynchronize(s->stream));
     });
 }
 

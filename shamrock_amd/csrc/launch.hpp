@@ -64,6 +64,11 @@ namespace shamx {
     void tree_stage_sort(hipStream_t s, TreeBuffers &t);
     void tree_stage_finish(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
                            const f64 *z, const f64 *h, u32 reduction_level, f64 htol);
+    // the two halves of tree_stage_finish: leaf reduction + Karras tree (synchronises once for
+    // the leaf count), then leaf boxes, max h * htol and the internal cells
+    void tree_stage_karras(hipStream_t s, TreeBuffers &t, u32 reduction_level);
+    void tree_stage_aabb(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
+                         const f64 *z, const f64 *h, f64 htol);
 
     // ------------------------------------------------------------------ neigh.hip
     struct NeighBuffers {
@@ -137,6 +142,7 @@ namespace shamx {
         f64 mass, alpha_u, alpha_AV, beta_AV;
         f64 C_cour, C_force; // CFL factors (cfl * multiplier)
         int mixed;           // 1: FP32 pair sums with Kahan compensation
+        int varying = 0;     // 1: per-particle alpha from the FP64 records (varying AV)
     };
     // bytes of one force record (128 FP64, 64 mixed)
     size_t force_record_bytes(int mixed);
@@ -168,6 +174,20 @@ namespace shamx {
                    u64 *red, DevArray<u64> &part);
     // max over n values as an ordered key (device)
     void max_key_f64(hipStream_t s, u32 n, const f64 *v, u64 *key);
+    // varying AV (Cullen & Dehnen 2010): div v, curl v, d(div v)/dt over the force rows of the
+    // n local particles (sorted layout), the alpha update from the saved sound speed into
+    // alpha_new (local), and alpha of every merged particle into the FP64 records.
+    // sorted_acc: merged accelerations in sorted order; work6: 5 local scratch arrays.
+    void cd10_alpha(hipStream_t s, int kernel, u32 n, u32 nm, Csr c, const void *rec,
+                    const u32 *sort_index, const u32 *ghost_src, const f64 *const *sorted_acc,
+                    const f64 *cs_saved, const f64 *h, const f64 *alpha, f64 dt, f64 mass,
+                    f64 alpha_min, f64 alpha_max, f64 sigma_decay, f64 *const *work6,
+                    f64 *alpha_new);
+    // per-particle CFL dt = min(C_cour h / vsig, C_force sqrt(h / |a|)) and its minimum as an
+    // ordered key (ComputeCFLCourant.hpp, ComputeCFLForce.hpp:50-62)
+    void cfl_dt(hipStream_t s, u32 n, const f64 *h, const f64 *vsig, const f64 *ax,
+                const f64 *ay, const f64 *az, f64 C_cour, f64 C_force, f64 *dt_cfl,
+                u64 *dt_min_key, DevArray<u64> &part);
     // periodic image selection (BasicSPHGhosts.cpp:260-579): returns G (synchronises) and
     // fills src ids + appends positions/h of the images after the n local entries.
     u32 periodic_ghosts(hipStream_t s, u32 n, f64 *x, f64 *y, f64 *z, f64 *h, const f64 *box6,
@@ -179,5 +199,29 @@ namespace shamx {
         int n;
     };
     void gather_ghosts(hipStream_t s, u32 n, u32 G, const u32 *ghost_src, FieldPtrs fields);
+
+    // ------------------------------------------------------------------ adjoint.hip
+    // inputs of the adjoint of one step, all from the solver after that step (sorted layout)
+    struct AdjointArgs {
+        u32 n, nm;
+        int kernel;
+        f64 dt, mass, hfact, gamma, alpha_u, alpha_AV, beta_AV;
+        int eos_kind;
+        f64 cs_iso;
+        const u32 *sort_index, *ghost_src, *lpos, *cnt, *offset, *ids, *fcnt, *fids;
+        const void *frec;    // FP64 force records of the step's force evaluation
+        const double4 *ps;   // (x, y, z, final h), sorted
+        const f64 *h, *omega; // final h and Omega, local particles
+    };
+    // seeds (11 device arrays of n: x y z vx vy vz u ax ay az du) at the end of the step are
+    // replaced by the seeds at its start
+    void adjoint_step(hipStream_t s, const AdjointArgs &A, f64 *const *seeds,
+                      DevArray<f64> &work);
+    // adds to (gx, gy, gz) the position gradient of sum_a rho_bar_a rho_h(h_a(x))
+    void adjoint_density(hipStream_t s, const AdjointArgs &A, const f64 *rho_bar, f64 *gx,
+                         f64 *gy, f64 *gz, DevArray<f64> &work);
+    void adjoint_gather_sorted(hipStream_t s, u32 nm, u32 N, const u32 *sort_index,
+                               const u32 *ghost_src, int nf, const f64 *const *in,
+                               f64 *const *out);
 
 } // namespace shamx

@@ -497,6 +497,9 @@ namespace shamx {
                 const f64 omega_a_rho_a_inv = 1 / (A.omega * rho_a);
                 const f64 ia_fac = A.inv_sub;
                 const f64 ca     = A.hR2;
+                // varying AV: per-particle alpha in the record's last slot
+                // (NodeUpdateDerivsVaryingAlphaAV.cpp:60-150)
+                const f64 alpha_a = fa.varying ? A.pad : fa.alpha_AV;
                 (void) h_a;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
                 // group-uniform trip count; the sums take the terms in list order
@@ -517,8 +520,9 @@ namespace shamx {
                             f64 rux = dx * inv, ruy = dy * inv, ruz = dz * inv;
                             f64 v_ab_r_ab = vabx * rux + vaby * ruy + vabz * ruz;
                             f64 abs_v     = fabs(v_ab_r_ab);
-                            f64 vsig_a = fa.alpha_AV * cs_a + fa.beta_AV * abs_v;
-                            f64 vsig_b = fa.alpha_AV * B.cs + fa.beta_AV * abs_v;
+                            f64 vsig_a = alpha_a * cs_a + fa.beta_AV * abs_v;
+                            f64 vsig_b = (fa.varying ? B.pad : fa.alpha_AV) * B.cs +
+                                         fa.beta_AV * abs_v;
                             f64 rho_avg = (rho_a + rho_b) * 0.5;
                             f64 vsig_u  = sqrt(fabs(P_a - B.P) / rho_avg);
                             f64 qa_ab = fmax(-0.5 * rho_a * vsig_a * v_ab_r_ab, 0.);
@@ -648,6 +652,153 @@ namespace shamx {
                 part[blockIdx.x]             = key;
                 part[gridDim.x + blockIdx.x] = npair;
             }
+        }
+
+        // ------------------------------------------------------------- varying AV (CD10)
+        __device__ inline void cross3(const f64 a[3], const f64 b[3], f64 c[3]) {
+            c[0] = a[1] * b[2] - a[2] * b[1];
+            c[1] = a[2] * b[0] - a[0] * b[2];
+            c[2] = a[0] * b[1] - a[1] * b[0];
+        }
+
+        // div v and curl v (DiffOperator.cpp:24-268) over the force rows (their criterion is the
+        // operators' own), one lane per particle, sums in row order
+        template<class K>
+        __global__ void __launch_bounds__(BLK)
+            k_divv_curlv(u32 n, Rows r, const FRec *__restrict rec, f64 mass,
+                         f64 *__restrict divv, f64 *__restrict curlx, f64 *__restrict curly,
+                         f64 *__restrict curlz) {
+            using KK = Kern<K, f64>;
+            u32 j    = xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
+            if (j >= n) return;
+            u32 row = r.sel[j], a = r.outi[row];
+            const FRec A = rec[row];
+            f64 inv_rho_omega_a = 1. / (A.omega * A.rho);
+            f64 sum_nabla_v = 0, sc[3] = {0, 0, 0};
+            u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+            for (u32 k = k0; k < k1; k++) {
+                const FRec B = rec[r.ids[k]];
+                f64 dr[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+                f64 rab2  = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+                if (rab2 > A.hR2 && rab2 > B.hR2) continue;
+                f64 rab     = sqrt(rab2);
+                f64 v_ab[3] = {A.vx - B.vx, A.vy - B.vy, A.vz - B.vz};
+                f64 ru[3]   = {dr[0] / rab, dr[1] / rab, dr[2] / rab};
+                if (rab < 1e-9) ru[0] = ru[1] = ru[2] = 0;
+                f64 dW      = KK::dW(rab, A.h);
+                f64 dWa[3]  = {dW * ru[0], dW * ru[1], dW * ru[2]};
+                sum_nabla_v += mass * (v_ab[0] * dWa[0] + v_ab[1] * dWa[1] + v_ab[2] * dWa[2]);
+                f64 c[3];
+                cross3(v_ab, dWa, c);
+                for (int t = 0; t < 3; t++) sc[t] = sc[t] + mass * c[t];
+            }
+            divv[a]  = -inv_rho_omega_a * sum_nabla_v;
+            curlx[a] = -inv_rho_omega_a * sc[0];
+            curly[a] = -inv_rho_omega_a * sc[1];
+            curlz[a] = -inv_rho_omega_a * sc[2];
+        }
+
+        // d(div v)/dt (DiffOperatorDtDivv.cpp:89-175) with the merged accelerations in sorted
+        // order; 3x3 inverse and products as in matrix_legacy.hpp:25-95 (rows stored)
+        template<class K>
+        __global__ void __launch_bounds__(BLK)
+            k_dtdivv(u32 n, Rows r, const FRec *__restrict rec, const f64 *__restrict sax,
+                     const f64 *__restrict say, const f64 *__restrict saz, f64 mass,
+                     f64 *__restrict dtdivv) {
+            using KK = Kern<K, f64>;
+            u32 j    = xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
+            if (j >= n) return;
+            u32 row = r.sel[j], a = r.outi[row];
+            const FRec A = rec[row];
+            const f64 aa[3] = {sax[row], say[row], saz[row]};
+            f64 R[3][3] = {}, Rv[3][3] = {}, Ra[3][3] = {};
+            u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+            for (u32 k = k0; k < k1; k++) {
+                u32 b        = r.ids[k];
+                const FRec B = rec[b];
+                f64 rr[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+                f64 rab2  = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+                if (rab2 > A.hR2 && rab2 > B.hR2) continue;
+                f64 rab     = sqrt(rab2);
+                f64 v_ab[3] = {A.vx - B.vx, A.vy - B.vy, A.vz - B.vz};
+                f64 a_ab[3] = {aa[0] - sax[b], aa[1] - say[b], aa[2] - saz[b]};
+                f64 ru[3]   = {rr[0] / rab, rr[1] / rab, rr[2] / rab};
+                if (rab < 1e-9) ru[0] = ru[1] = ru[2] = 0;
+                f64 dW = KK::dW(rab, A.h);
+                f64 m[3] = {dW * ru[0] * mass, dW * ru[1] * mass, dW * ru[2] * mass};
+                for (int i = 0; i < 3; i++)
+                    for (int c = 0; c < 3; c++) {
+                        R[i][c]  = R[i][c] - rr[i] * m[c];
+                        Rv[i][c] = Rv[i][c] - v_ab[c] * m[i];
+                        Ra[i][c] = Ra[i][c] - a_ab[c] * m[i];
+                    }
+            }
+            // inverse (matrix_legacy.hpp:25-50): aij = R[i][j] (element j of stored row i)
+            f64 a00 = R[0][0], a10 = R[1][0], a20 = R[2][0];
+            f64 a01 = R[0][1], a11 = R[1][1], a21 = R[2][1];
+            f64 a02 = R[0][2], a12 = R[1][2], a22 = R[2][2];
+            f64 det = (-a02 * a11 * a20 + a01 * a12 * a20 + a02 * a10 * a21 - a00 * a12 * a21 -
+                       a01 * a10 * a22 + a00 * a11 * a22);
+            f64 iv[3][3] = {{(-a12 * a21 + a11 * a22) / det, (a02 * a21 - a01 * a22) / det,
+                             (-a02 * a11 + a01 * a12) / det},
+                            {(a12 * a20 - a10 * a22) / det, (-a02 * a20 + a00 * a22) / det,
+                             (a02 * a10 - a00 * a12) / det},
+                            {(-a11 * a20 + a10 * a21) / det, (a01 * a20 - a00 * a21) / det,
+                             (-a01 * a10 + a00 * a11) / det}};
+            // products (matrix_legacy.hpp:53-95): out[i][c] = sum_k iv[i][k] B[k][c], k order
+            auto prod = [&](const f64 B[3][3], f64 O[3][3]) {
+                for (int i = 0; i < 3; i++)
+                    for (int c = 0; c < 3; c++)
+                        O[i][c] = iv[i][0] * B[0][c] + iv[i][1] * B[1][c] + iv[i][2] * B[2][c];
+            };
+            f64 dv[3][3], da[3][3];
+            prod(Rv, dv);
+            prod(Ra, da);
+            f64 div_a = da[0][0] + da[1][1] + da[2][2];
+            f64 tens  = dv[0][0] * dv[0][0] + dv[1][0] * dv[0][1] + dv[2][0] * dv[0][2] +
+                       dv[0][1] * dv[1][0] + dv[1][1] * dv[1][1] + dv[2][1] * dv[1][2] +
+                       dv[0][2] * dv[2][0] + dv[1][2] * dv[2][1] + dv[2][2] * dv[2][2];
+            dtdivv[a] = div_a - tens;
+        }
+
+        // alpha update (UpdateViscosity.cpp:120-219), local particles
+        __global__ void k_alpha_cd10(u32 n, f64 dt, const f64 *__restrict cs,
+                                     const f64 *__restrict h, const f64 *__restrict alpha,
+                                     const f64 *__restrict divv, const f64 *__restrict curlx,
+                                     const f64 *__restrict curly, const f64 *__restrict curlz,
+                                     const f64 *__restrict dtdivv, f64 alpha_min, f64 alpha_max,
+                                     f64 sigma_decay, f64 *__restrict alpha_new) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            const f64 eps = 2.220446049250313080847e-16; // DBL_EPSILON
+            f64 cs_a = cs[i], h_a = h[i], alpha_a = alpha[i], divv_a = divv[i];
+            f64 cx = curlx[i], cy = curly[i], cz = curlz[i], dtdivv_a = dtdivv[i];
+            f64 vsig            = cs_a;
+            f64 inv_tau_a       = vsig * sigma_decay / h_a;
+            f64 fact_t          = dt * inv_tau_a;
+            f64 euler_impl_fact = 1 / (1 + fact_t);
+            f64 fac = fmax(-divv_a, 0.);
+            fac *= fac;
+            f64 traceS  = cx * cx + cy * cy + cz * cz;
+            f64 balsara = (fac + traceS > eps) ? fac / (fac + traceS) : 1.;
+            f64 A_a     = balsara * fmax(-dtdivv_a, 0.);
+            f64 temp    = cs_a * cs_a;
+            f64 alpha_loc_a =
+                fmin((cs_a > 0) ? 10 * h_a * h_a * A_a / (temp) : alpha_min, alpha_max);
+            alpha_loc_a   = (temp > eps) ? alpha_loc_a : alpha_min;
+            f64 new_alpha = (alpha_a + alpha_loc_a * fact_t) * euler_impl_fact;
+            if (alpha_loc_a > alpha_a) new_alpha = alpha_loc_a;
+            alpha_new[i] = new_alpha;
+        }
+
+        // alpha of every merged particle (images take their source's) into the force records
+        __global__ void k_record_alpha(u32 nm, u32 N, const u32 *__restrict sort_index,
+                                       const u32 *__restrict ghost_src,
+                                       const f64 *__restrict alpha, FRec *__restrict rec) {
+            u32 s = blockIdx.x * blockDim.x + threadIdx.x;
+            if (s >= nm) return;
+            u32 b    = sort_index[s];
+            rec[s].pad = alpha[b < N ? b : ghost_src[b - N]];
         }
 
         // ------------------------------------------------------------- partial reductions
@@ -781,6 +932,28 @@ namespace shamx {
                 part[gridDim.x + blockIdx.x]     = (u64) __double_as_longlong(vsq);
                 part[2 * gridDim.x + blockIdx.x] = (u64) __double_as_longlong(en);
             }
+        }
+
+        // CFL dt per particle and per-block minima (the expressions of force_store)
+        __global__ void __launch_bounds__(BLK)
+            k_cfl(u32 n, const f64 *__restrict h, const f64 *__restrict vsig,
+                  const f64 *__restrict ax, const f64 *__restrict ay, const f64 *__restrict az,
+                  f64 C_cour, f64 C_force, f64 *__restrict dt_cfl, u64 *__restrict part) {
+            __shared__ u64 lds[BLK / WAVE];
+            u32 i   = blockIdx.x * BLK + threadIdx.x;
+            u64 key = ORDERED_POS_INF;
+            if (i < n) {
+                f64 fx = ax[i], fy = ay[i], fz = az[i], h_a = h[i];
+                f64 dt_c   = C_cour * h_a / vsig[i];
+                f64 abs_a  = sqrt(fx * fx + fy * fy + fz * fz);
+                f64 dt_f   = C_force * sqrt(h_a / abs_a);
+                f64 cfl    = fmin(__builtin_inf(), dt_c);
+                cfl        = fmin(cfl, dt_f);
+                dt_cfl[i]  = cfl;
+                key        = f64_to_ordered(cfl);
+            }
+            key = block_reduce(key, umin64, lds);
+            if (threadIdx.x == 0) part[blockIdx.x] = key;
         }
 
         __global__ void __launch_bounds__(BLK) k_max_key(u32 n, const f64 *__restrict v,
@@ -1067,6 +1240,43 @@ namespace shamx {
         finalize(s, nb, part.data(),
                  RedSpec{3, {RED_MAX_KEY, RED_SUM_F64, RED_SUM_F64}, {0, 1, 2}}, red,
                  part.data() + 3 * (size_t) nb);
+    }
+
+    void cd10_alpha(hipStream_t s, int kernel, u32 n, u32 nm, Csr c, const void *rec,
+                    const u32 *sort_index, const u32 *ghost_src, const f64 *const *sorted_acc,
+                    const f64 *cs_saved, const f64 *h, const f64 *alpha, f64 dt, f64 mass,
+                    f64 alpha_min, f64 alpha_max, f64 sigma_decay, f64 *const *work6,
+                    f64 *alpha_new) {
+        if (n == 0) return;
+        u32 nb = div_up(n, BLK);
+        Rows r = rows_of(c);
+        const FRec *fr = (const FRec *) rec;
+        dispatch_kernel(kernel, [&](auto K) {
+            using KT = decltype(K);
+            k_divv_curlv<KT><<<nb, BLK, 0, s>>>(n, r, fr, mass, work6[0], work6[1], work6[2],
+                                                work6[3]);
+            k_dtdivv<KT><<<nb, BLK, 0, s>>>(n, r, fr, sorted_acc[0], sorted_acc[1],
+                                            sorted_acc[2], mass, work6[4]);
+        });
+        k_alpha_cd10<<<div_up(n, 256), 256, 0, s>>>(n, dt, cs_saved, h, alpha, work6[0],
+                                                    work6[1], work6[2], work6[3], work6[4],
+                                                    alpha_min, alpha_max, sigma_decay,
+                                                    alpha_new);
+        k_record_alpha<<<div_up(nm, 256), 256, 0, s>>>(nm, n, sort_index, ghost_src, alpha_new,
+                                                       (FRec *) rec);
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void cfl_dt(hipStream_t s, u32 n, const f64 *h, const f64 *vsig, const f64 *ax,
+                const f64 *ay, const f64 *az, f64 C_cour, f64 C_force, f64 *dt_cfl,
+                u64 *dt_min_key, DevArray<u64> &part) {
+        if (n == 0) return;
+        u32 nb = div_up(n, BLK);
+        part.resize(part_words(1, nb));
+        k_cfl<<<nb, BLK, 0, s>>>(n, h, vsig, ax, ay, az, C_cour, C_force, dt_cfl, part.data());
+        SHAMX_HIP(hipGetLastError());
+        finalize(s, nb, part.data(), RedSpec{1, {RED_MIN_KEY}, {0}}, dt_min_key,
+                 part.data() + nb);
     }
 
     void max_key_f64(hipStream_t s, u32 n, const f64 *v, u64 *key) {

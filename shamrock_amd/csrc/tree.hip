@@ -514,8 +514,7 @@ namespace shamx {
                                  sizeof(u32), hipMemcpyDeviceToDevice, s));
     }
 
-    void tree_stage_finish(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
-                           const f64 *z, const f64 *h, u32 reduction_level, f64 htol) {
+    void tree_stage_karras(hipStream_t s, TreeBuffers &t, u32 reduction_level) {
         u32 n = t.n_obj;
         // leaf reduction
         u32 g = div_up(n, 256);
@@ -555,6 +554,13 @@ namespace shamx {
         } else {
             fill_u32(s, t.parent.data(), NO_PARENT, 1);
         }
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void tree_stage_aabb(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
+                         const f64 *z, const f64 *h, f64 htol) {
+        u32 L = t.L, I = t.I;
+        if (L == 0) throw std::invalid_argument("tree_stage_aabb: build the tree first");
         k_leaf_boxes<<<div_up(L, 256), 256, 0, s>>>(L, I, t.reduc_map.data(), t.sort_index.data(),
                                                     x, y, z, h, htol, t.nodes.data());
         if (I > 0) {
@@ -572,6 +578,12 @@ namespace shamx {
                                                             t.nodes.data(), err);
         }
         SHAMX_HIP(hipGetLastError());
+    }
+
+    void tree_stage_finish(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
+                           const f64 *z, const f64 *h, u32 reduction_level, f64 htol) {
+        tree_stage_karras(s, t, reduction_level);
+        tree_stage_aabb(s, t, x, y, z, h, htol);
     }
 
     void tree_build(hipStream_t s, TreeBuffers &t, u32 n, const f64 *x, const f64 *y,

@@ -51,7 +51,8 @@ def base_config(kernel: str, box_min, box_max, periodic=True, **kw) -> dict:
                box_max=list(map(float, box_max)), gpart_mass=0.0, gamma=5.0 / 3.0, eos_cs=1.0,
                eos_cs0=1.0, eos_q=0.5, eos_r0=1.0, alpha_u=1.0, alpha_AV=1.0, beta_AV=2.0,
                htol_up_coarse=1.1, htol_up_fine=1.1, epsilon_h=1e-6, cfl_cour=0.3,
-               cfl_force=0.3, cfl_multiplier_stiffness=2.0, mixed_precision=0)
+               cfl_force=0.3, cfl_multiplier_stiffness=2.0, mixed_precision=0, av_kind=0,
+               alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1)
     cfg.update(kw)
     return cfg
 
@@ -110,6 +111,122 @@ def random_box(n: int, seed: int = 0, kernel: str = "M4", periodic=True, jitter=
     cfg = base_config(kernel, (0, 0, 0), (1, 1, 1), periodic=periodic,
                       gpart_mass=1.0 / pos.shape[0])
     return cfg, fields_from(pos, dr, u, v)
+
+
+def plummer(n: int, seed: int = 0, a: float = 1.0, M: float = 1.0, r_max: float = 30.0,
+            u: float = 0.05, kernel: str = "M4") -> tuple[dict, dict]:
+    """Plummer sphere (scale radius a, mass M) truncated at r_max, free boundaries, zero
+    velocities, adiabatic gamma = 5/3 with u = 0.05.  Radii by the inverse of the enclosed-mass
+    fraction r^3 / (r^2 + a^2)^(3/2), drawn inside the truncation; isotropic directions; h from
+    the analytic density, h = hfact (m / rho)^(1/3)."""
+    rng = np.random.default_rng(seed)
+    f_max = r_max**3 / (r_max**2 + a * a) ** 1.5
+    X = rng.uniform(0, f_max, n)
+    r = a / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = np.ascontiguousarray(r[:, None] * d)
+    m = M / n
+    rho = 3 * M / (4 * math.pi * a**3) * (1 + r * r / (a * a)) ** -2.5
+    h = KERNEL_HFACT[kernel] * (m / rho) ** (1.0 / 3.0)
+    lo, hi = pos.min(0) - 1, pos.max(0) + 1
+    cfg = base_config(kernel, lo, hi, periodic=False, gpart_mass=m)
+    return cfg, fields_from(pos, h, u)
+
+
+def kelvin_helmholtz(n_side: int = 64, kernel: str = "M4", phase: float = 0.0,
+                     seed: int = 0) -> tuple[dict, dict]:
+    """Kelvin-Helmholtz shear layers in the periodic box [-0.5, 0.5)^3: a dense band |y| < 0.25
+    (density 2, lattice spacing dr) between layers of density 1 (spacing dr 2^(1/3)), equal
+    particle masses, pressure 2.5 everywhere, v_x = -0.5 in the band and +0.5 outside, and the
+    seeded perturbation v_y = 0.01 sin(4 pi (x + phase)) (+ 1e-4 white noise)."""
+    rng = np.random.default_rng(seed)
+    n_d = int(round(n_side * (4.0 / 3.0) ** (1.0 / 3.0)))   # band: twice the number density
+    dr_d = 1.0 / n_d
+    n_o = int(round(n_d / 2 ** (1.0 / 3.0)))
+    dr_o = 1.0 / n_o
+    band = lattice_cubic(dr_d, (-0.5, -0.25, -0.5), (0.5, 0.25, 0.5))
+    outer = lattice_cubic(dr_o, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    outer = outer[np.abs(outer[:, 1]) >= 0.25]
+    pos = np.ascontiguousarray(np.concatenate([band, outer]))
+    n = pos.shape[0]
+    m = (2.0 * 0.5 + 1.0 * 0.5) / n
+    dense = np.abs(pos[:, 1]) < 0.25
+    rho = np.where(dense, 2.0, 1.0)
+    gamma = 5.0 / 3.0
+    u = 2.5 / ((gamma - 1) * rho)
+    v = np.zeros((n, 3))
+    v[:, 0] = np.where(dense, -0.5, 0.5)
+    v[:, 1] = 0.01 * np.sin(4 * math.pi * (pos[:, 0] + phase)) + 1e-4 * rng.normal(size=n)
+    h = np.where(dense, dr_d, dr_o) * KERNEL_HFACT[kernel]
+    cfg = base_config(kernel, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), gpart_mass=m)
+    return cfg, fields_from(pos, h, u, v)
+
+
+def hcp_box_dim(dr: float, xcnt: int, ycnt: int, zcnt: int) -> tuple[float, float, float]:
+    """Model.get_box_dim_fcc_3d (generators.hpp:31-51): extent of xcnt x ycnt x zcnt HCP cells."""
+    i, j, k = xcnt, ycnt, zcnt
+    r = (float(2 * i + ((j + k) % 2)), math.sqrt(3.0) * (j + (1.0 / 3.0) * (k % 2)),
+         2 * math.sqrt(6.0) * k / 3)
+    return r[0] * dr, r[1] * dr, r[2] * dr
+
+
+def hcp_lattice(dr: float, box_min, box_max) -> np.ndarray:
+    """HCP lattice points inside the half-open box (crystalLattice.hpp LatticeHCP: generator,
+    get_box_index_bounds, index ranges [min, max); GeneratorLatticeHCP.hpp:83-106 keeps the
+    points with Patch::is_in_patch_converted).  Emitted with i fastest (the reference's
+    discontinuous iterator only permutes the same points)."""
+    bmin = np.asarray(box_min, float)
+    bmax = np.asarray(box_max, float)
+    s3, s6 = math.sqrt(3.0), math.sqrt(6.0)
+    cmin = [bmin[0] / 2.0, bmin[1] / s3, bmin[2] / (2 * s6 / 3)]
+    cmax = [bmax[0] / 2.0, bmax[1] / s3, bmax[2] / (2 * s6 / 3)]
+    lo = [int(c / dr) - 1 for c in cmin]
+    hi = [int(c / dr) + 1 for c in cmax]
+    k, j, i = np.meshgrid(np.arange(lo[2], hi[2]), np.arange(lo[1], hi[1]),
+                          np.arange(lo[0], hi[0]), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    x = (2 * i + (np.abs(j + k) % 2)).astype(np.float64)
+    y = s3 * (j + (1.0 / 3.0) * (np.abs(k) % 2))
+    z = 2 * s6 * k.astype(np.float64) / 3
+    p = np.stack([dr * x, dr * y, dr * z], axis=1)
+    keep = np.all((p >= bmin) & (p < bmax), axis=1)
+    return np.ascontiguousarray(p[keep])
+
+
+def sod_tube(resol: int = 128, kernel: str = "M6") -> tuple[dict, dict]:
+    """The reference's Sod-tube CI run (examples/tests_ci/sod_tube_sph.py): HCP lattices of
+    spacing dr (left, rho = 1, P = 1) and 2 dr (right, rho = 0.125, P = 0.1) in the periodic box
+    [-xs, xs] x [-ys/2, ys/2] x [-zs/2, zs/2], gamma = 1.4, Cullen & Dehnen viscosity
+    (alpha in [0, 1], sigma 0.1, alpha_u 1, beta 2), CFL 0.1, particle mass = total mass / N."""
+    gamma = 1.4
+    rho_g, rho_d = 1, 0.125
+    fact = (rho_g / rho_d) ** (1.0 / 3.0)
+    P_g, P_d = 1, 0.1
+    u_g = P_g / ((gamma - 1) * rho_g)
+    u_d = P_d / ((gamma - 1) * rho_d)
+    xs, ys, zs = hcp_box_dim(1, resol, 24, 24)
+    dr = 1 / xs
+    xs, ys, zs = hcp_box_dim(dr, resol, 24, 24)
+    bmin, bmax = (-xs, -ys / 2, -zs / 2), (xs, ys / 2, zs / 2)
+    p1 = hcp_lattice(dr, (-xs, -ys / 2, -zs / 2), (0, ys / 2, zs / 2))
+    p2 = hcp_lattice(dr * fact, (0, -ys / 2, -zs / 2), (xs, ys / 2, zs / 2))
+    pos = np.ascontiguousarray(np.concatenate([p1, p2]))
+    h = np.concatenate([np.full(len(p1), dr), np.full(len(p2), dr * fact)])
+    u = np.zeros(pos.shape[0])
+    left = np.all((pos >= np.array([-xs, -ys / 2, -zs / 2])) &
+                  (pos < np.array([0, ys / 2, zs / 2])), axis=1)
+    right = np.all((pos >= np.array([0, -ys / 2, -zs / 2])) &
+                   (pos < np.array([xs, ys / 2, zs / 2])), axis=1)
+    u[left] = u_g
+    u[right] = u_d
+    vol_b = xs * ys * zs
+    totmass = (rho_d * vol_b) + (rho_g * vol_b)
+    pmass = totmass / pos.shape[0]
+    cfg = base_config(kernel, bmin, bmax, gpart_mass=pmass, gamma=gamma, cfl_cour=0.1,
+                      cfl_force=0.1, av_kind=1, alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1,
+                      alpha_u=1.0, beta_AV=2.0)
+    return cfg, fields_from(pos, h, u)
 
 
 WORKLOADS = {

@@ -12,7 +12,8 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 
-FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du"]
+FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du", "alpha_AV",
+          "soundspeed"]
 
 
 def make_config(cfg: dict) -> _lib.ShamxConfig:
@@ -22,6 +23,9 @@ def make_config(cfg: dict) -> _lib.ShamxConfig:
     for k in ("periodic", "eos_kind", "tree_reduction_level", "h_iter_per_subcycles",
               "h_max_subcycles", "mixed_precision"):
         setattr(c, k, int(cfg[k]))
+    c.av_kind = int(cfg.get("av_kind", 0))
+    for k, d in (("alpha_min", 0.0), ("alpha_max", 1.0), ("sigma_decay", 0.1)):
+        setattr(c, k, float(cfg.get(k, d)))
     for i in range(3):
         c.box_min[i] = float(cfg["box_min"][i])
         c.box_max[i] = float(cfg["box_max"][i])
@@ -56,6 +60,37 @@ class GpuSolver:
         a = np.zeros(self.n)
         check(lib().shamx_solver_get_field(self.h, FIELDS.index(name), a.ctypes.data, 0))
         return a
+
+    def get_device(self, name, out=None):
+        """Device copy of a field as a torch tensor (float64, cuda)."""
+        import torch
+
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        check(lib().shamx_solver_get_field(self.h, FIELDS.index(name), out.data_ptr(), 1))
+        return out
+
+    def snapshot(self) -> dict:
+        """Device copy of the whole state (fields and time) for an exact replay."""
+        st = {k: self.get_device(k) for k in FIELDS}
+        st["_time"] = self.time()
+        return st
+
+    def restore(self, st: dict):
+        for k in FIELDS:
+            self.set_device(k, st[k])
+        tm = st["_time"]
+        self.set_time(tm["t"], tm["dt"], tm["cfl_mult"])
+
+    def adjoint_step(self, seeds):
+        """Seeds (11 cuda float64 tensors: x y z vx vy vz u ax ay az du) at the end of the last
+        step become those at its start (in place)."""
+        arr = (C.c_void_p * 11)(*[t.data_ptr() for t in seeds])
+        check(lib().shamx_solver_adjoint_step(self.h, arr))
+
+    def adjoint_density(self, rho_bar, xbar, ybar, zbar):
+        check(lib().shamx_solver_adjoint_density(self.h, rho_bar.data_ptr(), xbar.data_ptr(),
+                                                 ybar.data_ptr(), zbar.data_ptr()))
 
     def time(self):
         a = np.zeros(3)

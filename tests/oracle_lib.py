@@ -35,6 +35,8 @@ class OrcConfig(C.Structure):
         ("htol_up_coarse", C.c_double), ("htol_up_fine", C.c_double), ("epsilon_h", C.c_double),
         ("cfl_cour", C.c_double), ("cfl_force", C.c_double),
         ("cfl_multiplier_stiffness", C.c_double),
+        ("av_kind", C.c_int32), ("alpha_min", C.c_double), ("alpha_max", C.c_double),
+        ("sigma_decay", C.c_double),
     ]
 
 
@@ -93,6 +95,10 @@ def lib():
         L.orc_iterate_h.argtypes = [C.c_int, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint32, _f64p,
                                     _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                     C.c_double]
+        L.orc_cd10_operators.argtypes = ([C.c_int, C.c_uint32, _u32p, _u32p, _u32p, C.c_uint32] +
+                                         [_f64p] * 11 + [C.c_double] + [_f64p] * 5)
+        L.orc_cd10_alpha.argtypes = ([C.c_uint32, C.c_double] + [_f64p] * 8 +
+                                     [C.c_double] * 3 + [_f64p])
         L.orc_periodic_ghosts.argtypes = [C.c_uint32, _f64p, _f64p, _f64p, _f64p, C.c_double,
                                           C.c_uint32, _u32p, _f64p, C.POINTER(C.c_uint32)]
         L.orc_neigh_build.restype = C.c_void_p
@@ -255,6 +261,31 @@ def iterate_h(kernel, cnt, offset, ids, pos, h_old, h_new, eps, mass, h_evol_max
                              h_new, eps, mass, h_evol_max, h_evol_iter_max))
 
 
+def cd10_operators(kernel, cache, pos, vel, acc, h, omega, mass):
+    """div v, curl v and d(div v)/dt of the cache rows (DiffOperator.cpp:24-268,
+    DiffOperatorDtDivv.cpp:89-175)."""
+    x, y, z = soa(pos)
+    vx, vy, vz = soa(vel)
+    ax, ay, az = soa(acc)
+    n = cache["cnt"].size
+    out = [np.zeros(n) for _ in range(5)]
+    ids = np.ascontiguousarray(cache["ids"], np.uint32)
+    _chk(lib().orc_cd10_operators(KERNELS[kernel], n, np.ascontiguousarray(cache["cnt"], np.uint32),
+                                  np.ascontiguousarray(cache["offset"], np.uint32), ids, ids.size,
+                                  x, y, z, vx, vy, vz, ax, ay, az, _f(h), _f(omega), mass, *out))
+    return dict(zip(("divv", "curlx", "curly", "curlz", "dtdivv"), out))
+
+
+def cd10_alpha(dt, cs, h, alpha, divv, curl, dtdivv, alpha_min, alpha_max, sigma_decay):
+    """Cullen & Dehnen alpha update (UpdateViscosity.cpp:120-219)."""
+    cs = _f(cs)
+    out = np.zeros(cs.size)
+    _chk(lib().orc_cd10_alpha(cs.size, dt, cs, _f(h), _f(alpha), _f(divv), _f(curl[0]),
+                              _f(curl[1]), _f(curl[2]), _f(dtdivv), alpha_min, alpha_max,
+                              sigma_decay, out))
+    return out
+
+
 def periodic_ghosts(pos, box6, int_range):
     x, y, z = soa(pos)
     cap = 27 * x.size
@@ -290,7 +321,8 @@ def neigh_cache(pos, h, n_local, level=3, htol=1.1, Rkern=2.0):
 
 
 # ---------------------------------------------------------------- full solver
-FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du"]
+FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du", "alpha_AV",
+          "soundspeed"]
 
 
 class Solver:
@@ -323,6 +355,10 @@ class Solver:
         c.cfl_cour = cfg.get("cfl_cour", 0.3)
         c.cfl_force = cfg.get("cfl_force", 0.3)
         c.cfl_multiplier_stiffness = cfg.get("cfl_multiplier_stiffness", 2.0)
+        c.av_kind = int(cfg.get("av_kind", 0))
+        c.alpha_min = cfg.get("alpha_min", 0.0)
+        c.alpha_max = cfg.get("alpha_max", 1.0)
+        c.sigma_decay = cfg.get("sigma_decay", 0.1)
         self.cfg = c
         self.n = int(np.asarray(fields["x"]).size)
         self.h = lib().orc_solver_new(C.byref(c), self.n)

@@ -111,12 +111,20 @@ def test_radix_sort_stable(ctx, n, bits, seed):
     np.testing.assert_array_equal(k.cpu().numpy().view(np.uint32), keys[ref])
 
 
-def _tree_gpu(ctx, pos, h, level, htol):
+def _tree_gpu(ctx, pos, h, level, htol, split=False):
+    """Tree through shamx_tree_build, or (split) through its two halves
+    shamx_tree_reduce_karras + shamx_tree_aabb_hmax."""
     n = pos.shape[0]
     x, y, z = (dev(pos[:, k], np.float64) for k in range(3))
     hh = dev(h, np.float64)
-    check(lib().shamx_tree_build(ctx, stream(), n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
-                                 hh.data_ptr(), level, htol))
+    if split:
+        check(lib().shamx_tree_reduce_karras(ctx, stream(), n, x.data_ptr(), y.data_ptr(),
+                                             z.data_ptr(), level))
+        check(lib().shamx_tree_aabb_hmax(ctx, stream(), x.data_ptr(), y.data_ptr(),
+                                         z.data_ptr(), hh.data_ptr(), htol))
+    else:
+        check(lib().shamx_tree_build(ctx, stream(), n, x.data_ptr(), y.data_ptr(),
+                                     z.data_ptr(), hh.data_ptr(), level, htol))
     s = np.zeros(4, np.uint32)
     check(lib().shamx_tree_sizes(ctx, s.ctypes.data_as(C.POINTER(C.c_uint32))))
     n_obj, L, I = (int(v) for v in s[:3])
@@ -160,9 +168,10 @@ def _same_modulo_ties(codes_sorted, perm_a, perm_b):
     return True
 
 
-@pytest.mark.parametrize("case", ["lattice", "clustered"])
-def test_tree_build_bitexact(ctx, case):
-    say(f"[tree_build] case={case}")
+@pytest.mark.parametrize("case,split", [("lattice", False), ("clustered", False),
+                                        ("clustered", True)])
+def test_tree_build_bitexact(ctx, case, split):
+    say(f"[tree_build] case={case} split={split}")
     rng = np.random.default_rng(7)
     if case == "lattice":
         pos = setups.lattice_cubic(1 / 24, (0, 0, 0), (1, 1, 1))
@@ -172,7 +181,7 @@ def test_tree_build_bitexact(ctx, case):
         d = rng.normal(size=(60_000, 3))
         pos = r * d / np.linalg.norm(d, axis=1, keepdims=True)
     h = rng.uniform(0.01, 0.05, pos.shape[0])
-    g = _tree_gpu(ctx, pos, h, 3, 1.1)
+    g = _tree_gpu(ctx, pos, h, 3, 1.1, split=split)
     bb = orc.bbox(pos)
     t = orc.Tree(pos, bb, 3, bits=32)
     n = pos.shape[0]
@@ -187,6 +196,28 @@ def test_tree_build_bitexact(ctx, case):
     np.testing.assert_array_equal(g["aabb_min"], t.aabb_min)
     np.testing.assert_array_equal(g["aabb_max"], t.aabb_max)
     np.testing.assert_array_equal(g["rint"], t.field_max(h) * 1.1)
+
+
+def test_cfl_closed_form():
+    """shamx_cfl: dt_i = min(C_cour h / vsig, C_force sqrt(h / |a|)) and the minimum, bit for
+    bit against the same IEEE operations in numpy (ComputeCFLCourant / ComputeCFLForce)."""
+    say("[cfl]")
+    rng = np.random.default_rng(5)
+    n = 100_003
+    h = rng.uniform(1e-3, 1e-1, n)
+    vsig = rng.uniform(0.1, 10, n)
+    a = rng.normal(size=(3, n))
+    a[:, 17] = 0  # |a| = 0: the force limit is +inf, the Courant one applies
+    out = dev(np.zeros(n))
+    dmin = np.zeros(1)
+    check(lib().shamx_cfl(stream(), n, dev(h).data_ptr(), dev(vsig).data_ptr(),
+                          dev(a[0]).data_ptr(), dev(a[1]).data_ptr(), dev(a[2]).data_ptr(), 0.3,
+                          0.25, out.data_ptr(), _lib.dptr(dmin)))
+    absa = np.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])
+    with np.errstate(divide="ignore"):
+        want = np.minimum(0.3 * h / vsig, 0.25 * np.sqrt(h / absa))
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
+    assert dmin[0] == want.min()
 
 
 def _neigh_gpu(ctx, pos, h, n_local, htol, Rkern):
@@ -322,13 +353,21 @@ def _compare_transient(g, o, exact_sort=True):
 
 
 @pytest.mark.parametrize("workload,steps", [("sound_wave_16", 4), ("sedov_20", 3),
-                                            ("random_box_4096", 3)])
+                                            ("random_box_4096", 3), ("plummer_2000", 3),
+                                            ("kelvin_helmholtz_14", 3)])
 def test_full_step_parity(workload, steps):
+    """Whole steps vs the oracle.  Plummer: free boundaries, h over a decade (a realization the
+    reference's subcycled Newton iteration converges on); Kelvin-Helmholtz: two lattices of
+    different spacing, shear flow."""
     name, n = workload.rsplit("_", 1)
     if name == "sound_wave":
         cfg, f = setups.sound_wave(int(n))
     elif name == "sedov":
         cfg, f = setups.sedov(int(n))
+    elif name == "plummer":
+        cfg, f = setups.plummer(int(n), seed=1, r_max=3.0)
+    elif name == "kelvin_helmholtz":
+        cfg, f = setups.kelvin_helmholtz(int(n))
     else:
         cfg, f = setups.random_box(int(n), seed=3)
     g = GpuSolver(cfg, f)
@@ -345,6 +384,38 @@ def test_full_step_parity(workload, steps):
         tg, to = g.time(), o.time()
         assert abs(tg["dt"] - to["dt"]) <= RTOL * abs(to["dt"])
         assert tg["cfl_mult"] == to["cfl_mult"]
+    g.close()
+
+
+@pytest.mark.parametrize("workload,steps", [("sod_16", 5), ("kelvin_helmholtz_14", 4)])
+def test_full_step_parity_varying_av(workload, steps):
+    """Cullen & Dehnen varying viscosity (div v, curl v, d(div v)/dt, alpha update, per-particle
+    alpha in the force) over whole steps vs the oracle, alpha and the saved sound speed
+    included.  The first step runs with the sound speed 0 (alpha stays at its initial value);
+    from the second on the source and decay terms are live."""
+    name, n = workload.rsplit("_", 1)
+    if name == "sod":
+        cfg, f = setups.sod_tube(int(n))
+    else:
+        cfg, f = setups.kelvin_helmholtz(int(n))
+        cfg = dict(cfg, av_kind=1, alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1)
+    g = GpuSolver(cfg, f)
+    o = orc.Solver(cfg, f)
+    alive = 0
+    for step in range(steps):
+        say(f"[varying_av] {workload} step {step}")
+        lg = g.evolve_once()
+        lo = o.evolve_once()
+        assert lg["subcycles"] == lo["subcycles"] and lg["h_iterations"] == lo["h_iterations"]
+        assert lg["n_pairs"] == lo["n_pairs"]
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du", "alpha_AV",
+                  "soundspeed"):
+            assert relerr(g.get(k), o.get(k)) <= RTOL, (k, relerr(g.get(k), o.get(k)))
+        alive += int(np.count_nonzero(o.get("alpha_AV")) > 0)
+        tg, to = g.time(), o.time()
+        assert abs(tg["dt"] - to["dt"]) <= RTOL * abs(to["dt"])
+        assert tg["cfl_mult"] == to["cfl_mult"]
+    assert alive >= 1, "alpha never left its initial value: the update was not exercised"
     g.close()
 
 

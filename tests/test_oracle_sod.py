@@ -1,0 +1,41 @@
+"""The oracle on the reference's Sod-tube CI workload (examples/tests_ci/sod_tube_sph.py): whole
+steps with the Cullen & Dehnen viscosity, HCP lattices at resolution 128 (82944 particles), run to
+t = 0.245 with the last step clipped, L2 errors vs the exact Riemann solution against the
+reference's expected values (relative 1e-11, :114-121).  About 7 minutes on 8 cores, so it runs
+only with SHAMX_SLOW=1; tests/golden/sod_oracle_run.py prints the same comparison.
+"""
+import json
+import os
+
+import pytest
+
+import oracle_lib as orc
+from shamrock_amd import analysis, setups
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+pytestmark = [pytest.mark.slow,
+              pytest.mark.skipif(os.environ.get("SHAMX_SLOW") != "1",
+                                 reason="several minutes of CPU; set SHAMX_SLOW=1")]
+
+
+def test_oracle_sod_tube_ci_l2():
+    fx = json.load(open(os.path.join(HERE, "golden", "sod_tube_ci.json")))
+    cfg, f = setups.sod_tube(fx["resol"])
+    s = orc.Solver(cfg, f)
+    target = fx["t_target"]
+    steps = 0
+    while s.time()["t"] < target:
+        tm = s.time()
+        if tm["t"] + tm["dt"] > target:
+            s.set_time(dt=target - tm["t"])
+        s.evolve_once()
+        steps += 1
+        assert steps < 5000
+    out = {k: s.get(k) for k in ("x", "y", "z", "vx", "vy", "vz", "h", "u")}
+    sod = analysis.SodTube(1.4, 1, 1, 0.125, 0.1)
+    rho, v, P = analysis.sod_l2(sod, out, cfg["gpart_mass"], 1.0, 1.4, target)
+    got = dict(rho=rho, vx=v[0], vy=v[1], vz=v[2], P=P)
+    for k, val in got.items():
+        e = fx["expect_" + k]
+        assert abs(val - e) <= fx["tol"] * abs(e), (k, val, e)
