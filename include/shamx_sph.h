@@ -61,6 +61,10 @@ typedef struct shamx_config {
     int32_t av_kind;              /* 0: AVConfig::Constant, 1: AVConfig::VaryingCD10 */
     double alpha_min, alpha_max, sigma_decay; /* VaryingCD10 (AVConfig.hpp); alpha_u, beta_AV
                                                  above apply to both kinds */
+    double constant_G;     /* gravitational constant of the sink forces (SolverConfig.hpp:469-477:
+                              6.6743015e-11 with no unit system, 1 in code units with G = 1) */
+    double sink_softening; /* Plummer softening of the sink-gas force; 0 = the reference's
+                              unsoftened G M d / |d|^3 (SinkParticlesUpdate.cpp:127-141) */
 } shamx_config;
 
 /* Per-step log (reference TimestepLog Solver.hpp:43-54 + per-phase timers).  Times in s. */
@@ -188,14 +192,28 @@ typedef struct shamx_solver shamx_solver;
 
 int shamx_solver_create(const shamx_config *cfg, uint32_t n, shamx_solver **out);
 int shamx_solver_destroy(shamx_solver *s);
-/* field ids: 0 x 1 y 2 z 3 vx 4 vy 5 vz 6 ax 7 ay 8 az 9 hpart 10 uint 11 duint
- * (SolverConfig::set_layout, SolverConfig.cpp:24-121); `on_device` selects a device or host
- * source/destination of n doubles */
+/* The solver runs on its own stream.  Entries that take device pointers (set/get_field with
+ * on_device, the adjoint entries) first wait for the whole device, so their inputs may come
+ * from work on any stream; they return once the solver's stream has finished with them.
+ * field ids: 0 x 1 y 2 z 3 vx 4 vy 5 vz 6 ax 7 ay 8 az 9 hpart 10 uint 11 duint
+ * 12 alpha_AV 13 soundspeed 14 divv 15 curlv.x 16 curlv.y 17 curlv.z 18 dtdivv (varying AV)
+ * 19 axyz_ext.x 20 axyz_ext.y 21 axyz_ext.z (SolverConfig::set_layout, SolverConfig.cpp:24-121);
+ * `on_device` selects a device or host source/destination of n doubles */
 int shamx_solver_set_field(shamx_solver *s, int field, const double *src, int on_device);
 int shamx_solver_get_field(shamx_solver *s, int field, double *dst, int on_device);
 /* time state edges (Solver.hpp:92-148): tdc = {t, dt, cfl_multiplier} */
 int shamx_solver_get_time(shamx_solver *s, double *tdc3);
 int shamx_solver_set_time(shamx_solver *s, const double *tdc3);
+/* sinks: Model::add_sink (Model.hpp:183-200, sink_edges_helper.hpp:234-248) and the records of
+ * get_sinks (pySPHModel.cpp:1047-1065), 17 doubles per sink: pos[3], velocity[3],
+ * sph_acceleration[3], ext_acceleration[3], mass, angular_momentum[3], accretion_radius.
+ * Sink gravity (SinkParticlesUpdate.cpp:23-200, ExternalForces.cpp:48-360) runs in every step
+ * once a sink exists; accretion (SinkParticlesFlagAccreteHard & co.) is not part of this
+ * library.  set_sinks replaces the whole set (restart from saved records). */
+int shamx_solver_add_sink(shamx_solver *s, double mass, const double *pos3, const double *vel3,
+                          double accretion_radius);
+int shamx_solver_get_sinks(shamx_solver *s, uint32_t *count, double *out17);
+int shamx_solver_set_sinks(shamx_solver *s, uint32_t count, const double *in17);
 /* Solver::evolve_once (Solver.cpp:1942-3272) */
 int shamx_solver_evolve_once(shamx_solver *s, shamx_step_log *log);
 /* stream used by the solver (hipStream_t as void*) */

@@ -367,6 +367,29 @@ int orc_cd10_alpha(uint32_t n, double dt, const double *cs, const double *h,
     });
 }
 
+// ---- sink-gas gravity on explicit arrays ---------------------------------------------------
+// sinks as 17-double records (get_sinks layout); ext_out (3 n) starts from 0; acc_sph of each
+// sink written back into its record
+int orc_sink_gas(uint32_t n, const double *x, const double *y, const double *z, uint32_t ns,
+                 double *sinks17, double G, double gpart_mass, double soft, double *ext_out) {
+    return guard([&] {
+        std::vector<Sink> s(ns);
+        for (uint32_t i = 0; i < ns; i++) {
+            const double *o = sinks17 + 17 * (size_t) i;
+            s[i].pos        = {o[0], o[1], o[2]};
+            s[i].mass       = o[12];
+            s[i].racc       = o[16];
+        }
+        std::fill(ext_out, ext_out + 3 * (size_t) n, 0.);
+        sink_gas(n, x, y, z, s, G, gpart_mass, soft, ext_out, ext_out + n, ext_out + 2 * (size_t) n);
+        for (uint32_t i = 0; i < ns; i++) {
+            sinks17[17 * (size_t) i + 6] = s[i].acc_sph.x;
+            sinks17[17 * (size_t) i + 7] = s[i].acc_sph.y;
+            sinks17[17 * (size_t) i + 8] = s[i].acc_sph.z;
+        }
+    });
+}
+
 // ---- periodic ghosts ---------------------------------------------------------------------
 int orc_periodic_ghosts(uint32_t n, const double *x, const double *y, const double *z,
                         const double *box6, double int_range, uint32_t cap, uint32_t *out_src,
@@ -411,13 +434,17 @@ static int with_solver(void *h, int kernel, F &&f) {
 
 extern "C" {
 // field ids: 0 x 1 y 2 z 3 vx 4 vy 5 vz 6 ax 7 ay 8 az 9 h 10 u 11 du 12 alpha_AV
-// 13 soundspeed (saved sound speed of the varying AV)
+// 13 soundspeed (saved sound speed of the varying AV) 14 divv 15-17 curlv 18 dtdivv
+// 19-21 axyz_ext
 int orc_solver_field(void *h, int kernel, int field, double *buf, int set) {
     return with_solver(h, kernel, [&](auto &s) {
-        std::vector<double> *f[14] = {&s.x,  &s.y,  &s.z, &s.vx, &s.vy,       &s.vz,
-                                      &s.ax, &s.ay, &s.az, &s.h, &s.u, &s.du, &s.alpha_av,
-                                      &s.soundspeed};
-        if (field < 0 || field > 13) throw std::invalid_argument("field id");
+        std::vector<double> *f[22] = {&s.x,     &s.y,     &s.z,     &s.vx,        &s.vy,
+                                      &s.vz,    &s.ax,    &s.ay,    &s.az,        &s.h,
+                                      &s.u,     &s.du,    &s.alpha_av, &s.soundspeed, &s.divv,
+                                      &s.curlx, &s.curly, &s.curlz, &s.dtdivv,    &s.axe,
+                                      &s.aye,   &s.aze};
+        if (field < 0 || field > 21) throw std::invalid_argument("field id");
+        if (f[field]->size() != s.N) f[field]->assign(s.N, 0.);
         if (set) std::copy(buf, buf + s.N, f[field]->begin());
         else std::copy(f[field]->begin(), f[field]->end(), buf);
     });
@@ -433,6 +460,39 @@ int orc_solver_time(void *h, int kernel, double *tdc, int set) {
             tdc[0] = s.t;
             tdc[1] = s.dt;
             tdc[2] = s.cfl_mult;
+        }
+    });
+}
+// Model.hpp:183-200 (add_sink) and pySPHModel.cpp:1047-1065 (get_sinks): 17 doubles per sink,
+// pos, velocity, sph_acceleration, ext_acceleration, mass, angular_momentum, accretion_radius
+int orc_solver_add_sink(void *h, int kernel, const double *d17) {
+    return with_solver(h, kernel, [&](auto &s) {
+        Sink k{};
+        k.pos  = {d17[0], d17[1], d17[2]};
+        k.vel  = {d17[3], d17[4], d17[5]};
+        k.mass = d17[12];
+        k.racc = d17[16];
+        s.sinks.push_back(k);
+    });
+}
+int orc_solver_get_sinks(void *h, int kernel, uint32_t *count, double *out) {
+    return with_solver(h, kernel, [&](auto &s) {
+        *count = (uint32_t) s.sinks.size();
+        if (!out) return;
+        for (size_t i = 0; i < s.sinks.size(); i++) {
+            const Sink &k = s.sinks[i];
+            double *o     = out + 17 * i;
+            vec3 v[4]     = {k.pos, k.vel, k.acc_sph, k.acc_ext};
+            for (int j = 0; j < 4; j++) {
+                o[3 * j]     = v[j].x;
+                o[3 * j + 1] = v[j].y;
+                o[3 * j + 2] = v[j].z;
+            }
+            o[12] = k.mass;
+            o[13] = k.angmom.x;
+            o[14] = k.angmom.y;
+            o[15] = k.angmom.z;
+            o[16] = k.racc;
         }
     });
 }

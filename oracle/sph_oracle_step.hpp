@@ -28,7 +28,43 @@ namespace orc {
         // Dehnen 2010 (AVConfig.hpp VaryingCD10: alpha_min, alpha_max, sigma_decay)
         int32_t av_kind;
         double alpha_min, alpha_max, sigma_decay;
+        // gravitational constant of the sink forces (SolverConfig.hpp:469-477) and a Plummer
+        // softening of the sink-gas force (0 = the reference's unsoftened force)
+        double constant_G, sink_softening;
     };
+
+    // one sink (sink_edges_helper.hpp:234-248)
+    struct Sink {
+        vec3 pos, vel, acc_sph, acc_ext;
+        f64 mass;
+        vec3 angmom;
+        f64 racc;
+    };
+
+    // sink-gas gravity: ExternalForces.cpp:48-70 (axyz_ext reset by the caller) and
+    // SinkParticlesUpdate.cpp:69-150 (per sink: the particles gain -G M d/|d|^3, zero inside
+    // the accretion radius; the sink's acc_sph becomes the sum of G m d/|d|^3).  soft > 0 is a
+    // Plummer softening of the distance cubed (no counterpart in the reference).
+    inline void sink_gas(u32 n, const f64 *x, const f64 *y, const f64 *z,
+                         std::vector<Sink> &sinks, f64 G, f64 gpart_mass, f64 soft, f64 *axe,
+                         f64 *aye, f64 *aze) {
+        for (auto &s : sinks) {
+            vec3 acc{0, 0, 0};
+            for (u32 a = 0; a < n; a++) {
+                vec3 delta{x[a] - s.pos.x, y[a] - s.pos.y, z[a] - s.pos.z};
+                f64 r2     = dot(delta, delta);
+                f64 d      = std::sqrt(r2);
+                f64 ds     = (soft == 0) ? d : std::sqrt(r2 + soft * soft);
+                vec3 force = G * delta / (ds * ds * ds);
+                if (d < s.racc) force = {0, 0, 0};
+                acc    = acc + force * gpart_mass;
+                axe[a] = axe[a] + (-force.x) * s.mass;
+                aye[a] = aye[a] + (-force.y) * s.mass;
+                aze[a] = aze[a] + (-force.z) * s.mass;
+            }
+            s.acc_sph = acc;
+        }
+    }
 
     struct StepStats {
         double t_total, t_ghost, t_tree, t_neigh, t_hiter, t_omega, t_eos, t_force, t_corr, t_cfl;
@@ -51,6 +87,9 @@ namespace orc {
         std::vector<f64> x, y, z, vx, vy, vz, ax, ay, az, h, u, du;
         // varying AV: per-particle alpha and the sound speed saved at the end of each step
         std::vector<f64> alpha_av, soundspeed;
+        // sinks and the external acceleration of the particles (axyz_ext)
+        std::vector<Sink> sinks;
+        std::vector<f64> axe, aye, aze;
         f64 t = 0, dt = 0, cfl_mult = 1e-2; // Solver.hpp:124-147 time-state edges
         StepStats stats{};
 
@@ -230,10 +269,39 @@ namespace orc {
             stats.t_omega += to.sec();
         }
 
+        // sink-sink gravity: SinkParticlesUpdate.cpp:155-200 (compute_ext_forces)
+        void sink_ext_forces() {
+            const f64 eps = 1e-9;
+            for (auto &s : sinks) s.acc_ext = {0, 0, 0};
+            for (size_t i = 0; i < sinks.size(); i++) {
+                vec3 sum{0, 0, 0};
+                for (size_t j = 0; j < sinks.size(); j++) {
+                    vec3 rij = sinks[i].pos - sinks[j].pos;
+                    f64 r    = std::sqrt(dot(rij, rij));
+                    sum      = sum - (cfg.constant_G * sinks[j].mass) * rij / (r * r * r + eps);
+                }
+                sinks[i].acc_ext = sum;
+            }
+        }
+        void sink_gas_forces() {
+            axe.assign(N, 0.);
+            aye.assign(N, 0.);
+            aze.assign(N, 0.);
+            sink_gas(N, x.data(), y.data(), z.data(), sinks, cfg.constant_G, cfg.gpart_mass,
+                     cfg.sink_softening, axe.data(), aye.data(), aze.data());
+        }
+
         void evolve_once() {
             stats = StepStats{};
             Timer tstep;
             f64 t_cur = t, dtc = dt;
+
+            // sink predictor: SinkParticlesUpdate.cpp:23-46 (Solver.cpp:2017)
+            if (!sinks.empty()) {
+                sink_ext_forces();
+                for (auto &s : sinks) s.vel = s.vel + (dtc / 2) * (s.acc_sph + s.acc_ext);
+                for (auto &s : sinks) s.pos = s.pos + dtc * s.vel;
+            }
 
             // leapfrog predictor: Solver.cpp:390-524, 582-608; ForwardEuler.hpp:62-69
             f64 dt_half = dtc / 2;
@@ -249,6 +317,12 @@ namespace orc {
                 vy[i] = vy[i] + dt_half * ay[i];
                 vz[i] = vz[i] + dt_half * az[i];
                 u[i]  = u[i] + dt_half * du[i];
+            }
+            // sink-sink forces at the drifted sink positions, then the sink-gas forces
+            // (Solver.cpp:2033-2035), before the position boundary
+            if (!sinks.empty()) {
+                sink_ext_forces();
+                sink_gas_forces();
             }
             // apply_position_boundary: Solver.cpp:938-981
             if (cfg.periodic) {
@@ -291,6 +365,13 @@ namespace orc {
                                              ax.data(), ay.data(), az.data(), du.data(),
                                              cfg.gpart_mass, av,
                                              cfg.av_kind == 1 ? malpha.data() : nullptr);
+                // add_ext_forces: ExternalForces.cpp:324-360
+                if (!sinks.empty())
+                    for (u32 i = 0; i < N; i++) {
+                        ax[i] = ax[i] + axe[i];
+                        ay[i] = ay[i] + aye[i];
+                        az[i] = az[i] + aze[i];
+                    }
                 stats.t_force += tf.sec();
                 Timer tc;
                 // leapfrog_corrector: integrators.cpp:87-119 with hdt = dt/2
@@ -320,6 +401,8 @@ namespace orc {
                     cfl_mult /= 2;
                 } else {
                     rerun = false;
+                    // sink corrector: SinkParticlesUpdate.cpp:48-67 (Solver.cpp:2620)
+                    for (auto &s : sinks) s.vel = s.vel + (dtc / 2) * (s.acc_sph + s.acc_ext);
                     // write back the updated alpha (Solver.cpp:2622-2650)
                     if (cfg.av_kind == 1) alpha_av = alpha_new;
                     Timer tl;

@@ -35,6 +35,7 @@ class ShamxConfig(C.Structure):
         ("cfl_multiplier_stiffness", C.c_double),
         ("mixed_precision", C.c_int32), ("av_kind", C.c_int32),
         ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("sigma_decay", C.c_double),
+        ("constant_G", C.c_double), ("sink_softening", C.c_double),
     ]
 
 
@@ -96,6 +97,9 @@ SIGNATURES = {
     "shamx_solver_get_field": (_i, [_P, _i, _P, _i]),
     "shamx_solver_get_time": (_i, [_P, _dp]),
     "shamx_solver_set_time": (_i, [_P, _dp]),
+    "shamx_solver_add_sink": (_i, [_P, C.c_double, _dp, _dp, C.c_double]),
+    "shamx_solver_get_sinks": (_i, [_P, C.POINTER(C.c_uint32), _P]),
+    "shamx_solver_set_sinks": (_i, [_P, C.c_uint32, _P]),
     "shamx_solver_evolve_once": (_i, [_P, C.POINTER(ShamxStepLog)]),
     "shamx_solver_stream": (_i, [_P, C.POINTER(_P)]),
     "shamx_solver_sizes": (_i, [_P, C.POINTER(C.c_uint64)]),

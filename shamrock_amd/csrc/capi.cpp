@@ -123,7 +123,16 @@ struct shamx_solver {
     DevArray<f64> ax, ay, az, du, ax_old, ay_old, az_old, du_old, eps, h_old, vsig, dtc;
     // varying AV (Cullen & Dehnen 2010): alpha and the sound speed saved at the end of the
     // previous step (local), the updated alpha, 5 operator arrays (local), sorted accelerations
-    DevArray<f64> alpha_av, soundspeed, alpha_new, cd10_work, sacc;
+    DevArray<f64> alpha_av, soundspeed, alpha_new, sacc;
+    DevArray<f64> divv, curlx, curly, curlz, dtdivv; // the operators of the last force pass
+    // sinks (sink_edges_helper.hpp:234-248), host-side like the reference's synchronized data,
+    // and the particles' external acceleration (axyz_ext, 3 N)
+    struct Sink {
+        f64 pos[3], vel[3], acc_sph[3], acc_ext[3], mass, angmom[3], racc;
+    };
+    std::vector<Sink> sinks;
+    DevArray<f64> axe, aye, aze;
+    DevArray<u64> sink_red;
     DevArray<u32> ghost_src, gwork;
     DevArray<u8> ghost_oid;
     DevArray<unsigned char> gscratch;
@@ -136,9 +145,11 @@ struct shamx_solver {
     PhaseTimer timer;
 
     DevArray<f64> *field(int id) {
-        DevArray<f64> *f[14] = {&x, &y, &z, &vx, &vy, &vz, &ax,       &ay,
-                                &az, &h, &u, &du, &alpha_av, &soundspeed};
-        need(id >= 0 && id < 14, "field id must be in [0, 13]");
+        DevArray<f64> *f[22] = {&x,     &y,     &z,      &vx,       &vy,         &vz,
+                                &ax,    &ay,    &az,     &h,        &u,          &du,
+                                &alpha_av, &soundspeed, &divv, &curlx, &curly, &curlz,
+                                &dtdivv, &axe,  &aye,    &aze};
+        need(id >= 0 && id < 22, "field id must be in [0, 21]");
         return f[id];
     }
 
@@ -147,11 +158,13 @@ struct shamx_solver {
         cap = n + n / 3 + 1024;
         for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega, &P, &cs}) a->resize(cap);
         for (auto *a : {&ax, &ay, &az, &du, &ax_old, &ay_old, &az_old, &du_old, &eps, &h_old,
-                        &vsig, &dtc, &alpha_av, &soundspeed, &alpha_new})
+                        &vsig, &dtc, &alpha_av, &soundspeed, &alpha_new, &divv, &curlx, &curly,
+                        &curlz, &dtdivv, &axe, &aye, &aze})
             a->resize(std::max<u32>(n, 1));
         red.resize(8);
         for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega, &ax, &ay, &az, &du,
-                        &alpha_av, &soundspeed})
+                        &alpha_av, &soundspeed, &divv, &curlx, &curly, &curlz, &dtdivv, &axe,
+                        &aye, &aze})
             fill_f64(stream, a->data(), 0., a->capacity());
         SHAMX_HIP(hipStreamSynchronize(stream));
     }
@@ -266,6 +279,47 @@ struct shamx_solver {
         timer.end(stream, PH_OMEGA, b);
     }
 
+    // sink-sink gravity: SinkParticlesUpdate.cpp:155-200
+    void sink_ext_forces() {
+        const f64 eps = 1e-9;
+        for (auto &s : sinks) s.acc_ext[0] = s.acc_ext[1] = s.acc_ext[2] = 0;
+        for (size_t i = 0; i < sinks.size(); i++) {
+            f64 sum[3] = {0, 0, 0};
+            for (size_t j = 0; j < sinks.size(); j++) {
+                f64 r[3];
+                for (int k = 0; k < 3; k++) r[k] = sinks[i].pos[k] - sinks[j].pos[k];
+                f64 rs = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                f64 gm = cfg.constant_G * sinks[j].mass;
+                for (int k = 0; k < 3; k++) sum[k] = sum[k] - gm * r[k] / (rs * rs * rs + eps);
+            }
+            for (int k = 0; k < 3; k++) sinks[i].acc_ext[k] = sum[k];
+        }
+    }
+    // sink-gas gravity: axyz_ext of the particles (ExternalForces.cpp:48-70) and the sinks'
+    // acceleration from the gas (SinkParticlesUpdate.cpp:69-150)
+    void sink_gas_forces() {
+        u32 S = (u32) sinks.size();
+        sink_red.resize(3 * (size_t) S);
+        for (u32 k = 0; k < S; k++) {
+            const Sink &sk = sinks[k];
+            sink_gas(stream, N, x.data(), y.data(), z.data(),
+                     SinkArgs{sk.pos[0], sk.pos[1], sk.pos[2], sk.mass, sk.racc, cfg.constant_G,
+                              cfg.gpart_mass, cfg.sink_softening},
+                     k == 0, axe.data(), aye.data(), aze.data(),
+                     sink_red.data() + 3 * (size_t) k, part);
+        }
+        std::vector<u64> r(3 * (size_t) S);
+        SHAMX_HIP(hipMemcpyAsync(r.data(), sink_red.data(), r.size() * 8, hipMemcpyDeviceToHost,
+                                 stream));
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        for (u32 k = 0; k < S; k++)
+            for (int c = 0; c < 3; c++) std::memcpy(&sinks[k].acc_sph[c], &r[3 * k + c], 8);
+    }
+    void sink_kick(f64 dt) { // SinkParticlesUpdate.cpp:42-44, 63-65
+        for (auto &s : sinks)
+            for (int k = 0; k < 3; k++) s.vel[k] = s.vel[k] + (dt / 2) * (s.acc_sph[k] + s.acc_ext[k]);
+    }
+
     void evolve_once() {
         need(N > 0, "evolve_once on an empty particle set");
         timer.reset();
@@ -275,10 +329,26 @@ struct shamx_solver {
         f64 t_cur = t, dtc_ = dt;
         f64 box6[6] = {cfg.box_min[0], cfg.box_min[1], cfg.box_min[2],
                        cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
+        const bool has_sinks = !sinks.empty();
+        if (has_sinks) { // sink predictor: SinkParticlesUpdate.cpp:23-46 (Solver.cpp:2017)
+            sink_ext_forces();
+            sink_kick(dtc_);
+            for (auto &s : sinks)
+                for (int k = 0; k < 3; k++) s.pos[k] = s.pos[k] + dtc_ * s.vel[k];
+        }
         int b = timer.begin(stream);
+        // with sinks, the sink forces see the drifted positions before the periodic wrap
+        // (Solver.cpp:2033-2040)
+        int wrap_now = cfg.periodic && !has_sinks;
         predictor(stream, N, dtc_, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
-                  ax.data(), ay.data(), az.data(), u.data(), du.data(), cfg.periodic, nullptr_box());
+                  ax.data(), ay.data(), az.data(), u.data(), du.data(), wrap_now,
+                  wrap_now ? nullptr_box() : nullptr);
         timer.end(stream, PH_CORR, b);
+        if (has_sinks) {
+            sink_ext_forces();
+            sink_gas_forces();
+            if (cfg.periodic) wrap_positions(stream, N, x.data(), y.data(), z.data(), nullptr_box());
+        }
         (void) box6;
         prestep();
 
@@ -307,13 +377,12 @@ struct shamx_solver {
                 b = timer.begin(stream);
                 u32 Nm = N + G;
                 sacc.resize(3 * (size_t) std::max<u32>(Nm, 1));
-                cd10_work.resize(5 * (size_t) N);
                 const f64 *acc_in[3] = {ax.data(), ay.data(), az.data()};
                 f64 *acc_out[3] = {sacc.data(), sacc.data() + Nm, sacc.data() + 2 * (size_t) Nm};
                 adjoint_gather_sorted(stream, Nm, N, tree.sort_index.data(), ghost_src.data(), 3,
                                       acc_in, acc_out);
-                f64 *w5[5];
-                for (int k = 0; k < 5; k++) w5[k] = cd10_work.data() + k * (size_t) N;
+                f64 *w5[5] = {divv.data(), curlx.data(), curly.data(), curlz.data(),
+                              dtdivv.data()};
                 cd10_alpha(stream, cfg.kernel, N, Nm, csr_force(), frec.data(),
                            tree.sort_index.data(), ghost_src.data(), acc_out, soundspeed.data(),
                            h.data(), alpha_av.data(), dtc_, cfg.gpart_mass, cfg.alpha_min,
@@ -329,6 +398,11 @@ struct shamx_solver {
                          cfg.cfl_force * cfl_mult,
                          cfg.mixed_precision,
                          cfg.av_kind == 1};
+            if (has_sinks) {
+                fa.ext_x = axe.data();
+                fa.ext_y = aye.data();
+                fa.ext_z = aze.data();
+            }
             compute_force(stream, cfg.kernel, N, csr_force(), frec.data(), x.data(), y.data(),
                           z.data(), vx.data(), vy.data(), vz.data(), h.data(), omega.data(),
                           u.data(), P.data(), cs.data(), ax.data(), ay.data(), az.data(),
@@ -362,6 +436,7 @@ struct shamx_solver {
             } else {
                 rerun    = false;
                 next_cfl = dt_min;
+                if (has_sinks) sink_kick(dtc_); // sink corrector (Solver.cpp:2620)
                 // the updated alpha is kept once the step is accepted (Solver.cpp:2622-2650)
                 if (cfg.av_kind == 1)
                     SHAMX_HIP(hipMemcpyAsync(alpha_av.data(), alpha_new.data(), N * sizeof(f64),
@@ -473,6 +548,7 @@ struct shamx_solver {
              "adjoint: available for the adiabatic and isothermal equations of state");
         need(!cfg.mixed_precision, "adjoint: the step must run in FP64");
         need(cfg.av_kind == 0, "adjoint: available for the constant artificial viscosity");
+        need(sinks.empty(), "adjoint: not available with sinks");
         f64 hfact = 0;
         dispatch_kernel(cfg.kernel, [&](auto K) { hfact = decltype(K)::hfactd; });
         return AdjointArgs{N,
@@ -831,6 +907,8 @@ int shamx_solver_destroy(shamx_solver *s) {
 int shamx_solver_set_field(shamx_solver *s, int field, const double *src, int on_device) {
     return guarded([&] {
         auto *f = s->field(field);
+        // a device source may still be being written on another stream
+        if (on_device) SHAMX_HIP(hipDeviceSynchronize());
         SHAMX_HIP(hipMemcpyAsync(f->data(), src, (size_t) s->N * sizeof(f64),
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                                  s->stream));
@@ -841,6 +919,7 @@ int shamx_solver_set_field(shamx_solver *s, int field, const double *src, int on
 int shamx_solver_get_field(shamx_solver *s, int field, double *dst, int on_device) {
     return guarded([&] {
         auto *f = s->field(field);
+        if (on_device) SHAMX_HIP(hipDeviceSynchronize());
         SHAMX_HIP(hipMemcpyAsync(dst, f->data(), (size_t) s->N * sizeof(f64),
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                                  s->stream));
@@ -864,6 +943,62 @@ int shamx_solver_set_time(shamx_solver *s, const double *tdc3) {
     });
 }
 
+int shamx_solver_add_sink(shamx_solver *s, double mass, const double *pos3, const double *vel3,
+                          double accretion_radius) {
+    return guarded([&] {
+        need(s && pos3 && vel3, "shamx_solver_add_sink: null argument");
+        shamx_solver::Sink k{};
+        for (int c = 0; c < 3; c++) {
+            k.pos[c] = pos3[c];
+            k.vel[c] = vel3[c];
+        }
+        k.mass = mass;
+        k.racc = accretion_radius;
+        s->sinks.push_back(k);
+    });
+}
+
+int shamx_solver_get_sinks(shamx_solver *s, uint32_t *count, double *out17) {
+    return guarded([&] {
+        need(s && count, "shamx_solver_get_sinks: null argument");
+        *count = (uint32_t) s->sinks.size();
+        if (!out17) return;
+        for (size_t i = 0; i < s->sinks.size(); i++) {
+            const auto &k = s->sinks[i];
+            double *o     = out17 + 17 * i;
+            for (int c = 0; c < 3; c++) {
+                o[c]      = k.pos[c];
+                o[3 + c]  = k.vel[c];
+                o[6 + c]  = k.acc_sph[c];
+                o[9 + c]  = k.acc_ext[c];
+                o[13 + c] = k.angmom[c];
+            }
+            o[12] = k.mass;
+            o[16] = k.racc;
+        }
+    });
+}
+
+int shamx_solver_set_sinks(shamx_solver *s, uint32_t count, const double *in17) {
+    return guarded([&] {
+        need(s && (count == 0 || in17), "shamx_solver_set_sinks: null argument");
+        s->sinks.assign(count, shamx_solver::Sink{});
+        for (uint32_t i = 0; i < count; i++) {
+            auto &k         = s->sinks[i];
+            const double *o = in17 + 17 * (size_t) i;
+            for (int c = 0; c < 3; c++) {
+                k.pos[c]     = o[c];
+                k.vel[c]     = o[3 + c];
+                k.acc_sph[c] = o[6 + c];
+                k.acc_ext[c] = o[9 + c];
+                k.angmom[c]  = o[13 + c];
+            }
+            k.mass = o[12];
+            k.racc = o[16];
+        }
+    });
+}
+
 int shamx_solver_evolve_once(shamx_solver *s, shamx_step_log *log) {
     return guarded([&] {
         s->evolve_once();
@@ -877,6 +1012,7 @@ int shamx_solver_adjoint_step(shamx_solver *s, double *const *seeds11) {
         need(s->last.corrector_iterations == 1,
              "adjoint: the last step reran its corrector (lower the CFL)");
         for (int k = 0; k < 11; k++) need(seeds11[k] != nullptr, "adjoint: null seed array");
+        SHAMX_HIP(hipDeviceSynchronize()); // seeds may still be being written on another stream
         adjoint_step(s->stream, s->adjoint_args(), seeds11, s->adj_work);
         SHAMX_HIP(hipStreamSynchronize(s->stream));
     });
@@ -886,6 +1022,7 @@ int shamx_solver_adjoint_density(shamx_solver *s, const double *rho_bar, double 
                                  double *ybar, double *zbar) {
     return guarded([&] {
         need(s && rho_bar && xbar && ybar && zbar, "shamx_solver_adjoint_density: null argument");
+        SHAMX_HIP(hipDeviceSynchronize()); // inputs may still be being written on another stream
         adjoint_density(s->stream, s->adjoint_args(), rho_bar, xbar, ybar, zbar, s->adj_work);
         SHAMX_HIP(hipStreamSynchronize(s->stream));
     });

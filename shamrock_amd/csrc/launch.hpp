@@ -143,6 +143,9 @@ namespace shamx {
         f64 C_cour, C_force; // CFL factors (cfl * multiplier)
         int mixed;           // 1: FP32 pair sums with Kahan compensation
         int varying = 0;     // 1: per-particle alpha from the FP64 records (varying AV)
+        // external acceleration added to the hydro force (axyz += axyz_ext,
+        // ExternalForces.cpp:324-360) before the CFL criterion; null when there is none
+        const f64 *ext_x = nullptr, *ext_y = nullptr, *ext_z = nullptr;
     };
     // bytes of one force record (128 FP64, 64 mixed)
     size_t force_record_bytes(int mixed);
@@ -183,6 +186,17 @@ namespace shamx {
                     const f64 *cs_saved, const f64 *h, const f64 *alpha, f64 dt, f64 mass,
                     f64 alpha_min, f64 alpha_max, f64 sigma_decay, f64 *const *work6,
                     f64 *alpha_new);
+    // one sink's gravity on n particles (SinkParticlesUpdate.cpp:69-150): ext += -G M d/|d|^3
+    // (ext starts from 0 when first), zero inside the accretion radius, with an optional
+    // Plummer softening (0 = the reference's force); sum_out (device, 3 f64 as u64 words)
+    // receives sum_a G m d/|d|^3, the sink's acceleration from the gas.
+    // periodic wrap of positions into box6 (integrators.cpp:206-236)
+    void wrap_positions(hipStream_t s, u32 n, f64 *x, f64 *y, f64 *z, const f64 *box6);
+    struct SinkArgs {
+        f64 x, y, z, mass, racc, G, gpart_mass, soft;
+    };
+    void sink_gas(hipStream_t s, u32 n, const f64 *x, const f64 *y, const f64 *z, SinkArgs sk,
+                  int first, f64 *ex, f64 *ey, f64 *ez, u64 *sum_out, DevArray<u64> &part);
     // per-particle CFL dt = min(C_cour h / vsig, C_force sqrt(h / |a|)) and its minimum as an
     // ordered key (ComputeCFLCourant.hpp, ComputeCFLForce.hpp:50-62)
     void cfl_dt(hipStream_t s, u32 n, const f64 *h, const f64 *vsig, const f64 *ax,

@@ -456,6 +456,11 @@ namespace shamx {
             o.ay_old[a] = o.ay[a];
             o.az_old[a] = o.az[a];
             o.du_old[a] = o.du[a];
+            if (fa.ext_x) { // ExternalForces.cpp:324-360
+                fx = fx + fa.ext_x[a];
+                fy = fy + fa.ext_y[a];
+                fz = fz + fa.ext_z[a];
+            }
             o.ax[a]     = fx;
             o.ay[a]     = fy;
             o.az[a]     = fz;
@@ -801,6 +806,41 @@ namespace shamx {
             rec[s].pad = alpha[b < N ? b : ghost_src[b - N]];
         }
 
+        // ------------------------------------------------------------- sink-gas gravity
+        __global__ void __launch_bounds__(BLK)
+            k_sink_gas(u32 n, const f64 *__restrict x, const f64 *__restrict y,
+                       const f64 *__restrict z, SinkArgs sk, int first, f64 *__restrict ex,
+                       f64 *__restrict ey, f64 *__restrict ez, u64 *__restrict part) {
+            u32 i  = blockIdx.x * BLK + threadIdx.x;
+            f64 sx = 0, sy = 0, sz = 0;
+            if (i < n) {
+                f64 dx = x[i] - sk.x, dy = y[i] - sk.y, dz = z[i] - sk.z;
+                f64 r2 = dx * dx + dy * dy + dz * dz;
+                f64 d  = sqrt(r2);
+                f64 ds = (sk.soft == 0) ? d : sqrt(r2 + sk.soft * sk.soft);
+                f64 d3 = ds * ds * ds;
+                f64 fx = sk.G * dx / d3, fy = sk.G * dy / d3, fz = sk.G * dz / d3;
+                if (d < sk.racc) fx = fy = fz = 0;
+                sx = fx * sk.gpart_mass;
+                sy = fy * sk.gpart_mass;
+                sz = fz * sk.gpart_mass;
+                f64 bx = first ? 0. : ex[i], by = first ? 0. : ey[i], bz = first ? 0. : ez[i];
+                ex[i] = bx + (-fx) * sk.mass;
+                ey[i] = by + (-fy) * sk.mass;
+                ez[i] = bz + (-fz) * sk.mass;
+            }
+            __shared__ f64 ldf[3][BLK / WAVE];
+            auto add = [](f64 p, f64 q) { return p + q; };
+            sx = block_reduce(sx, add, ldf[0]);
+            sy = block_reduce(sy, add, ldf[1]);
+            sz = block_reduce(sz, add, ldf[2]);
+            if (threadIdx.x == 0) {
+                part[blockIdx.x]                 = (u64) __double_as_longlong(sx);
+                part[gridDim.x + blockIdx.x]     = (u64) __double_as_longlong(sy);
+                part[2 * gridDim.x + blockIdx.x] = (u64) __double_as_longlong(sz);
+            }
+        }
+
         // ------------------------------------------------------------- partial reductions
         // slot k of the nb block partials is reduced in a fixed order (fixed chunks, then one
         // block), so the result, an FP64 sum included, does not depend on scheduling
@@ -895,6 +935,17 @@ namespace shamx {
             vy[i] = v1;
             vz[i] = v2;
             u[i]  = uu;
+        }
+
+        // periodic wrap alone (integrators.cpp:206-236), for steps whose sink forces must see
+        // the drifted positions before the wrap
+        __global__ void k_wrap(u32 n, f64 *__restrict x, f64 *__restrict y, f64 *__restrict z,
+                               const f64 *__restrict box6) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            x[i] = pos_mod(x[i], box6[0], box6[3] - box6[0]);
+            y[i] = pos_mod(y[i], box6[1], box6[4] - box6[1]);
+            z[i] = pos_mod(z[i], box6[2], box6[5] - box6[2]);
         }
 
         __global__ void __launch_bounds__(BLK)
@@ -1240,6 +1291,26 @@ namespace shamx {
         finalize(s, nb, part.data(),
                  RedSpec{3, {RED_MAX_KEY, RED_SUM_F64, RED_SUM_F64}, {0, 1, 2}}, red,
                  part.data() + 3 * (size_t) nb);
+    }
+
+    void wrap_positions(hipStream_t s, u32 n, f64 *x, f64 *y, f64 *z, const f64 *box6) {
+        if (n == 0) return;
+        k_wrap<<<div_up(n, 256), 256, 0, s>>>(n, x, y, z, box6);
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void sink_gas(hipStream_t s, u32 n, const f64 *x, const f64 *y, const f64 *z, SinkArgs sk,
+                  int first, f64 *ex, f64 *ey, f64 *ez, u64 *sum_out, DevArray<u64> &part) {
+        if (n == 0) {
+            fill_u64(s, sum_out, 0, 3);
+            return;
+        }
+        u32 nb = div_up(n, BLK);
+        part.resize(part_words(3, nb));
+        k_sink_gas<<<nb, BLK, 0, s>>>(n, x, y, z, sk, first, ex, ey, ez, part.data());
+        SHAMX_HIP(hipGetLastError());
+        finalize(s, nb, part.data(), RedSpec{3, {RED_SUM_F64, RED_SUM_F64, RED_SUM_F64}, {0, 1, 2}},
+                 sum_out, part.data() + 3 * (size_t) nb);
     }
 
     void cd10_alpha(hipStream_t s, int kernel, u32 n, u32 nm, Csr c, const void *rec,

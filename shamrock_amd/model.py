@@ -16,10 +16,37 @@ from typing import Callable
 import numpy as np
 
 from .setups import KERNEL_HFACT, KERNEL_RKERN, base_config, kernel_W, lattice_cubic
-from .solver import GpuSolver
+from .solver import FIELDS, G_SI, GpuSolver, sink_dict
 
-_VEC_FIELDS = {"xyz": ("x", "y", "z"), "vxyz": ("vx", "vy", "vz"), "axyz": ("ax", "ay", "az")}
-_SCAL_FIELDS = {"hpart": "h", "uint": "u", "duint": "du"}
+_VEC_FIELDS = {"xyz": ("x", "y", "z"), "vxyz": ("vx", "vy", "vz"), "axyz": ("ax", "ay", "az"),
+               "axyz_ext": ("axe", "aye", "aze"), "curlv": ("curlx", "curly", "curlz")}
+_SCAL_FIELDS = {"hpart": "h", "uint": "u", "duint": "du", "alpha_AV": "alpha_AV",
+                "divv": "divv", "dtdivv": "dtdivv", "soundspeed": "soundspeed"}
+# fields of the layout (SolverConfig.cpp:24-121) for the two viscosities
+_LAYOUT_CONSTANT = ("xyz", "vxyz", "axyz", "axyz_ext", "hpart", "uint", "duint")
+_LAYOUT_CD10 = _LAYOUT_CONSTANT + ("alpha_AV", "divv", "dtdivv", "curlv", "soundspeed")
+
+
+class UnitSystem:
+    """shamunits::UnitSystem (UnitSystem.hpp:148-156): SI value of each code unit."""
+
+    def __init__(self, unit_time=1.0, unit_length=1.0, unit_mass=1.0, unit_current=1.0,
+                 unit_temperature=1.0, unit_qte=1.0, unit_lumint=1.0):
+        self.unit_time, self.unit_length, self.unit_mass = unit_time, unit_length, unit_mass
+        self.unit_current, self.unit_temperature = unit_current, unit_temperature
+        self.unit_qte, self.unit_lumint = unit_qte, unit_lumint
+
+
+class Constants:
+    """shamunits::Constants: physical constants expressed in a unit system."""
+
+    def __init__(self, units: UnitSystem | None = None):
+        self.units = units or UnitSystem()
+
+    def G(self):
+        """6.6743015e-11 m^3 kg^-1 s^-2 (Constants.hpp:130) in code units."""
+        u = self.units
+        return G_SI * u.unit_mass * u.unit_time ** 2 / u.unit_length ** 3
 
 
 @dataclasses.dataclass
@@ -75,8 +102,29 @@ class SolverConfig:
         self.two_stage_search = True
 
     # artificial viscosity (AVConfig.hpp:49-55)
-    def set_artif_viscosity_Constant(self, alpha_u, alpha_AV, beta_AV):
-        self.d.update(alpha_u=float(alpha_u), alpha_AV=float(alpha_AV), beta_AV=float(beta_AV))
+    def set_artif_viscosity_Constant(self, *, alpha_u, alpha_AV, beta_AV):
+        self.d.update(av_kind=0, alpha_u=float(alpha_u), alpha_AV=float(alpha_AV),
+                      beta_AV=float(beta_AV))
+
+    def set_artif_viscosity_VaryingCD10(self, *, alpha_min, alpha_max, sigma_decay, alpha_u,
+                                        beta_AV):
+        """Cullen & Dehnen 2010 (pySPHModel.cpp:170-186, AVConfig VaryingCD10)."""
+        self.d.update(av_kind=1, alpha_min=float(alpha_min), alpha_max=float(alpha_max),
+                      sigma_decay=float(sigma_decay), alpha_u=float(alpha_u),
+                      beta_AV=float(beta_AV))
+
+    # units: G of the sink forces (SolverConfig.hpp:469-477)
+    def set_units(self, units: UnitSystem):
+        self.units = units
+        self.d["constant_G"] = Constants(units).G()
+
+    def get_constant_G(self):
+        return self.d["constant_G"]
+
+    def set_sink_softening(self, eps):
+        """Plummer softening of the sink-gas force: an extension of this library (the
+        reference's sink force is unsoftened); 0 restores the reference's force."""
+        self.d["sink_softening"] = float(eps)
 
     def set_boundary_periodic(self):
         self.d["periodic"] = 1
@@ -147,8 +195,8 @@ class SPHModel:
         self.kernel = kernel
         self.name = f"SPHModel_f64_3_{kernel}"
         self.cfg = SolverConfig(kernel)
-        self.host = {k: np.zeros(0) for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az",
-                                              "h", "u", "du")}
+        self.host = {k: np.zeros(0) for k in FIELDS}
+        self.sinks = np.zeros((0, 17))
         self.gpu: GpuSolver | None = None
         self.t, self.dt, self.cfl_mult = 0.0, 0.0, 1e-2
         self.logs: list[dict] = []
@@ -186,6 +234,7 @@ class SPHModel:
         if self.gpu is not None:
             for k in self.host:
                 self.host[k] = self.gpu.get(k)
+            self.sinks = self.gpu.sink_records()
             tm = self.gpu.time()
             self.t, self.dt, self.cfl_mult = tm["t"], tm["dt"], tm["cfl_mult"]
 
@@ -255,7 +304,24 @@ class SPHModel:
         return f.sum(axis=0)
 
     def collect_data(self) -> dict:
-        return {k: self._field_get(k) for k in ("xyz", "vxyz", "axyz", "hpart", "uint", "duint")}
+        """pyShamrockCtx.cpp:97-128: every field of the layout, (N, 3) or (N,) float64."""
+        names = _LAYOUT_CD10 if self.cfg.d.get("av_kind", 0) == 1 else _LAYOUT_CONSTANT
+        return {k: self._field_get(k) for k in names}
+
+    # --- sinks ---------------------------------------------------------------------------
+    def add_sink(self, mass, pos, velocity, accretion_radius):
+        """Model::add_sink (Model.hpp:183-200)."""
+        self._sync_from_gpu()
+        rec = np.zeros(17)
+        rec[0:3], rec[3:6], rec[12], rec[16] = pos, velocity, mass, accretion_radius
+        self.sinks = np.concatenate([self.sinks, rec[None]])
+        if self.gpu is not None:
+            self.gpu.set_sink_records(self.sinks)
+
+    def get_sinks(self) -> list:
+        """pySPHModel.cpp:1047-1065"""
+        self._sync_from_gpu()
+        return [sink_dict(r) for r in self.sinks]
 
     # --- mass / density ------------------------------------------------------------------
     def set_particle_mass(self, m):
@@ -332,6 +398,7 @@ class SPHModel:
                                    "Please set it using either model.set_particle_mass(pmass) "
                                    "or cfg.set_particle_mass(pmass)")
             self.gpu = GpuSolver(self.cfg.d, self.host)
+            self.gpu.set_sink_records(self.sinks)
             self.gpu.set_time(self.t, self.dt, self.cfl_mult)
         return self.gpu
 

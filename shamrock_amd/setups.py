@@ -52,7 +52,8 @@ def base_config(kernel: str, box_min, box_max, periodic=True, **kw) -> dict:
                eos_cs0=1.0, eos_q=0.5, eos_r0=1.0, alpha_u=1.0, alpha_AV=1.0, beta_AV=2.0,
                htol_up_coarse=1.1, htol_up_fine=1.1, epsilon_h=1e-6, cfl_cour=0.3,
                cfl_force=0.3, cfl_multiplier_stiffness=2.0, mixed_precision=0, av_kind=0,
-               alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1)
+               alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1, constant_G=6.6743015e-11,
+               sink_softening=0.0)
     cfg.update(kw)
     return cfg
 
@@ -132,6 +133,55 @@ def plummer(n: int, seed: int = 0, a: float = 1.0, M: float = 1.0, r_max: float 
     lo, hi = pos.min(0) - 1, pos.max(0) + 1
     cfg = base_config(kernel, lo, hi, periodic=False, gpart_mass=m)
     return cfg, fields_from(pos, h, u)
+
+
+def knn_h(pos, kernel: str) -> np.ndarray:
+    """Half the distance to the k-th nearest particle, k = (4 pi / 3) (hfact R)^3 (the
+    kernel's neighbour count), scaled by R / 2: the support radius R h then holds k particles."""
+    from scipy.spatial import cKDTree
+    R = KERNEL_RKERN[kernel]
+    k = int(round(4.0 / 3.0 * math.pi * (KERNEL_HFACT[kernel] * R) ** 3))
+    k = max(1, min(k, pos.shape[0] - 1))
+    d, _ = cKDTree(pos).query(pos, k=k + 1)
+    return d[:, -1] / R
+
+
+def disc(n: int, seed: int = 0, r_in: float = 1.0, r_out: float = 10.0, m_disc: float = 0.01,
+         h_over_r: float = 0.05, q: float = 0.25, alpha_AV: float = 0.1, kernel: str = "M4",
+         softening: float = 0.1, mixed: bool = False) -> tuple[dict, dict]:
+    """Gas disc around a central sink (G = M = 1): surface density proportional to 1/r on
+    [r_in, r_out] (radii uniform in r), locally isothermal LP07 sound speed
+    c_s = h_over_r (r / r_in)^(-q) (spherical r, ComputeEos.cpp:726-798), Gaussian vertical
+    profile of scale height H = c_s / Omega_K, Keplerian azimuthal velocity about the sink,
+    constant viscosity alpha_AV (beta 2, alpha_u 1), free boundaries.  h starts at 0.6 times the
+    k-nearest-neighbour estimate: in a disc thinner than the kernel the density sum falls more
+    slowly than rho_h(h) beyond the root, so the reference's Newton iteration
+    (density.hpp:22-41) only converges from below.  The sink (mass 1 at rest at the origin, accretion
+    radius 0.1) is in cfg["sinks"]; its force on the gas is softened by `softening` (a choice of
+    this library: the reference's sink force has no softening)."""
+    rng = np.random.default_rng(seed)
+    G = M = 1.0
+    r = rng.uniform(r_in, r_out, n)
+    phi = rng.uniform(0, 2 * math.pi, n)
+    cs = h_over_r * (r / r_in) ** (-q)
+    omega_k = np.sqrt(G * M / r**3)
+    H = cs / omega_k
+    zz = rng.normal(0, 1, n) * H
+    pos = np.ascontiguousarray(np.stack([r * np.cos(phi), r * np.sin(phi), zz], 1))
+    m = m_disc / n
+    h = 0.6 * knn_h(pos, kernel)
+    vk = np.sqrt(G * M / r)
+    v = np.stack([-vk * np.sin(phi), vk * np.cos(phi), np.zeros(n)], 1)
+    gamma = 5.0 / 3.0
+    u = cs**2 / (gamma * (gamma - 1))
+    lo, hi = pos.min(0) - 1, pos.max(0) + 1
+    cfg = base_config(kernel, lo, hi, periodic=False, gpart_mass=m, gamma=gamma, eos_kind=2,
+                      eos_cs0=h_over_r, eos_q=q, eos_r0=r_in, alpha_AV=alpha_AV, beta_AV=2.0,
+                      alpha_u=1.0, constant_G=G, sink_softening=softening,
+                      mixed_precision=int(mixed))
+    cfg["sinks"] = [dict(mass=M, pos=(0.0, 0.0, 0.0), velocity=(0.0, 0.0, 0.0),
+                         accretion_radius=0.1)]
+    return cfg, fields_from(pos, h, u, v)
 
 
 def kelvin_helmholtz(n_side: int = 64, kernel: str = "M4", phase: float = 0.0,

@@ -12,8 +12,19 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 
+# gravitational constant with no unit system set (shamunits Constants.hpp:130)
+G_SI = 6.6743015e-11
+
 FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du", "alpha_AV",
-          "soundspeed"]
+          "soundspeed", "divv", "curlx", "curly", "curlz", "dtdivv", "axe", "aye", "aze"]
+
+
+def sink_dict(r) -> dict:
+    """One 17-double sink record with the keys of get_sinks (pySPHModel.cpp:1047-1065)."""
+    r = np.asarray(r, float)
+    return dict(pos=tuple(r[0:3]), velocity=tuple(r[3:6]), sph_acceleration=tuple(r[6:9]),
+                ext_acceleration=tuple(r[9:12]), mass=float(r[12]),
+                angular_momentum=tuple(r[13:16]), accretion_radius=float(r[16]))
 
 
 def make_config(cfg: dict) -> _lib.ShamxConfig:
@@ -24,7 +35,8 @@ def make_config(cfg: dict) -> _lib.ShamxConfig:
               "h_max_subcycles", "mixed_precision"):
         setattr(c, k, int(cfg[k]))
     c.av_kind = int(cfg.get("av_kind", 0))
-    for k, d in (("alpha_min", 0.0), ("alpha_max", 1.0), ("sigma_decay", 0.1)):
+    for k, d in (("alpha_min", 0.0), ("alpha_max", 1.0), ("sigma_decay", 0.1),
+                 ("constant_G", G_SI), ("sink_softening", 0.0)):
         setattr(c, k, float(cfg.get(k, d)))
     for i in range(3):
         c.box_min[i] = float(cfg["box_min"][i])
@@ -48,6 +60,8 @@ class GpuSolver:
         self.h = h
         for k, v in fields.items():
             self.set(k, v)
+        for s in cfg.get("sinks", ()):
+            self.add_sink(s["mass"], s["pos"], s["velocity"], s["accretion_radius"])
 
     def set(self, name, v):
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (self.n,)))
@@ -74,11 +88,14 @@ class GpuSolver:
         """Device copy of the whole state (fields and time) for an exact replay."""
         st = {k: self.get_device(k) for k in FIELDS}
         st["_time"] = self.time()
+        st["_sinks"] = self.sink_records()
         return st
 
     def restore(self, st: dict):
         for k in FIELDS:
             self.set_device(k, st[k])
+        if "_sinks" in st:
+            self.set_sink_records(st["_sinks"])
         tm = st["_time"]
         self.set_time(tm["t"], tm["dt"], tm["cfl_mult"])
 
@@ -102,6 +119,30 @@ class GpuSolver:
         a = np.array([cur["t"] if t is None else t, cur["dt"] if dt is None else dt,
                       cur["cfl_mult"] if cfl_mult is None else cfl_mult])
         check(lib().shamx_solver_set_time(self.h, _lib.dptr(a)))
+
+    def add_sink(self, mass, pos, velocity, accretion_radius):
+        """Model::add_sink (Model.hpp:183-200)."""
+        p = np.ascontiguousarray(pos, np.float64)
+        v = np.ascontiguousarray(velocity, np.float64)
+        check(lib().shamx_solver_add_sink(self.h, float(mass), _lib.dptr(p), _lib.dptr(v),
+                                          float(accretion_radius)))
+
+    def sink_records(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        check(lib().shamx_solver_get_sinks(self.h, C.byref(n), None))
+        out = np.zeros((n.value, 17))
+        if n.value:
+            check(lib().shamx_solver_get_sinks(self.h, C.byref(n), out.ctypes.data))
+        return out
+
+    def set_sink_records(self, rec):
+        rec = np.ascontiguousarray(rec, np.float64).reshape(-1, 17)
+        check(lib().shamx_solver_set_sinks(self.h, rec.shape[0],
+                                           rec.ctypes.data if rec.size else None))
+
+    def get_sinks(self) -> list:
+        """get_sinks (pySPHModel.cpp:1047-1065): one dict per sink."""
+        return [sink_dict(r) for r in self.sink_records()]
 
     def evolve_once(self) -> dict:
         log = _lib.ShamxStepLog()

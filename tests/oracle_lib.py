@@ -36,7 +36,7 @@ class OrcConfig(C.Structure):
         ("cfl_cour", C.c_double), ("cfl_force", C.c_double),
         ("cfl_multiplier_stiffness", C.c_double),
         ("av_kind", C.c_int32), ("alpha_min", C.c_double), ("alpha_max", C.c_double),
-        ("sigma_decay", C.c_double),
+        ("sigma_decay", C.c_double), ("constant_G", C.c_double), ("sink_softening", C.c_double),
     ]
 
 
@@ -99,6 +99,8 @@ def lib():
                                          [_f64p] * 11 + [C.c_double] + [_f64p] * 5)
         L.orc_cd10_alpha.argtypes = ([C.c_uint32, C.c_double] + [_f64p] * 8 +
                                      [C.c_double] * 3 + [_f64p])
+        L.orc_sink_gas.argtypes = [C.c_uint32, _f64p, _f64p, _f64p, C.c_uint32, _f64p, C.c_double,
+                                   C.c_double, C.c_double, _f64p]
         L.orc_periodic_ghosts.argtypes = [C.c_uint32, _f64p, _f64p, _f64p, _f64p, C.c_double,
                                           C.c_uint32, _u32p, _f64p, C.POINTER(C.c_uint32)]
         L.orc_neigh_build.restype = C.c_void_p
@@ -111,6 +113,8 @@ def lib():
         L.orc_solver_free.argtypes = [C.c_void_p]
         L.orc_solver_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_int]
         L.orc_solver_time.argtypes = [C.c_void_p, C.c_int, _f64p, C.c_int]
+        L.orc_solver_add_sink.argtypes = [C.c_void_p, C.c_int, _f64p]
+        L.orc_solver_get_sinks.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.c_void_p]
         L.orc_solver_evolve_once.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats)]
         L.orc_solver_sizes.argtypes = [C.c_void_p, C.c_int, _u64p]
         L.orc_solver_get_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p]
@@ -286,6 +290,20 @@ def cd10_alpha(dt, cs, h, alpha, divv, curl, dtdivv, alpha_min, alpha_max, sigma
     return out
 
 
+def sink_gas(pos, sinks, G, gpart_mass, soft=0.0):
+    """Sink-gas gravity (SinkParticlesUpdate.cpp:69-150): the particles' external accelerations
+    (n, 3) and each sink's acceleration from the gas (ns, 3)."""
+    x, y, z = soa(pos)
+    rec = np.zeros((len(sinks), 17))
+    for i, s in enumerate(sinks):
+        rec[i, 0:3], rec[i, 12], rec[i, 16] = s["pos"], s["mass"], s["accretion_radius"]
+    rec = np.ascontiguousarray(rec)
+    ext = np.zeros(3 * x.size)
+    _chk(lib().orc_sink_gas(x.size, x, y, z, len(sinks), rec.reshape(-1), G, gpart_mass, soft,
+                            ext))
+    return ext.reshape(3, -1).T.copy(), rec[:, 6:9].copy()
+
+
 def periodic_ghosts(pos, box6, int_range):
     x, y, z = soa(pos)
     cap = 27 * x.size
@@ -322,7 +340,15 @@ def neigh_cache(pos, h, n_local, level=3, htol=1.1, Rkern=2.0):
 
 # ---------------------------------------------------------------- full solver
 FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du", "alpha_AV",
-          "soundspeed"]
+          "soundspeed", "divv", "curlx", "curly", "curlz", "dtdivv", "axe", "aye", "aze"]
+
+
+def sink_dict(r):
+    """One sink record (17 doubles) as the keys of get_sinks (pySPHModel.cpp:1047-1065)."""
+    r = np.asarray(r, float)
+    return dict(pos=r[0:3].copy(), velocity=r[3:6].copy(), sph_acceleration=r[6:9].copy(),
+                ext_acceleration=r[9:12].copy(), mass=float(r[12]),
+                angular_momentum=r[13:16].copy(), accretion_radius=float(r[16]))
 
 
 class Solver:
@@ -359,6 +385,8 @@ class Solver:
         c.alpha_min = cfg.get("alpha_min", 0.0)
         c.alpha_max = cfg.get("alpha_max", 1.0)
         c.sigma_decay = cfg.get("sigma_decay", 0.1)
+        c.constant_G = cfg.get("constant_G", 6.6743015e-11)
+        c.sink_softening = cfg.get("sink_softening", 0.0)
         self.cfg = c
         self.n = int(np.asarray(fields["x"]).size)
         self.h = lib().orc_solver_new(C.byref(c), self.n)
@@ -366,6 +394,8 @@ class Solver:
             raise RuntimeError("oracle: " + lib().orc_last_error().decode())
         for k, v in fields.items():
             self.set(k, v)
+        for s in cfg.get("sinks", ()):
+            self.add_sink(s["mass"], s["pos"], s["velocity"], s["accretion_radius"])
 
     def set(self, name, v):
         a = _f(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.n,)))
@@ -386,6 +416,18 @@ class Solver:
         a = np.array([cur["t"] if t is None else t, cur["dt"] if dt is None else dt,
                       cur["cfl_mult"] if cfl_mult is None else cfl_mult])
         _chk(lib().orc_solver_time(self.h, self.kernel, a, 1))
+
+    def add_sink(self, mass, pos, velocity, accretion_radius):
+        rec = np.zeros(17)
+        rec[0:3], rec[3:6], rec[12], rec[16] = pos, velocity, mass, accretion_radius
+        _chk(lib().orc_solver_add_sink(self.h, self.kernel, rec))
+
+    def get_sinks(self):
+        n = C.c_uint32(0)
+        _chk(lib().orc_solver_get_sinks(self.h, self.kernel, C.byref(n), None))
+        out = np.zeros(17 * max(n.value, 1))
+        _chk(lib().orc_solver_get_sinks(self.h, self.kernel, C.byref(n), out.ctypes.data))
+        return [sink_dict(out[17 * i:17 * i + 17]) for i in range(n.value)]
 
     def evolve_once(self):
         st = StepStats()

@@ -48,7 +48,7 @@ def test_struct_layouts_match_header():
     from shamrock_amd import _lib
 
     # shamx_config: 6 int32 + 6 f64 box + 15 f64 + 2 int32 = 24 + 48 + 120 + 8
-    assert _lib.ShamxConfig.__dict__ and ctypes_size(_lib.ShamxConfig) == 224
+    assert _lib.ShamxConfig.__dict__ and ctypes_size(_lib.ShamxConfig) == 240
     # shamx_step_log: 11 f64 + 8 u32 + 2 u64 + 5 f64
     assert ctypes_size(_lib.ShamxStepLog) == 11 * 8 + 8 * 4 + 2 * 8 + 5 * 8
 

@@ -3,6 +3,11 @@
 The gradient of L = sum_a c_a rho_a(final) w.r.t. the initial velocities and internal energies,
 through several full steps, is compared with central finite differences of the FP64 forward
 (same time steps, h converged to 1e-12 so that the forward map is smooth to that level).
+
+The conductivity signal speed sqrt(|P_a - P_b| / rho_avg) has an unbounded derivative where
+P_a = P_b, which a uniform-pressure start (the Kelvin-Helmholtz setup) puts on every pair of a
+lattice; the finite-difference cases therefore start from internal energies perturbed by 5 %,
+where the step is differentiable with bounded derivatives.
 """
 import sys
 
@@ -36,8 +41,9 @@ def _case(name):
     else:
         cfg, f = setups.kelvin_helmholtz(10)
     cfg = dict(cfg, epsilon_h=1e-12, cfl_cour=0.1, cfl_force=0.1)
+    rng = np.random.default_rng(11)
     v0 = torch.tensor(np.stack([f["vx"], f["vy"], f["vz"]], 1), device=DEV)
-    u0 = torch.tensor(f["u"], device=DEV)
+    u0 = torch.tensor(f["u"] * (1 + 0.05 * rng.standard_normal(f["u"].size)), device=DEV)
     return cfg, f, v0, u0
 
 
@@ -66,14 +72,14 @@ def test_adjoint_matches_central_differences(name):
     fd_v, fd_u = [], []
     with torch.no_grad():
         for i, d in comps:
-            e = 1e-6
+            e = 1e-5
             vp, vm = v0.clone(), v0.clone()
             vp[i, d] += e
             vm[i, d] -= e
             fd_v.append((_loss(cfg, f, vp, u0, c, steps, dts) -
                          _loss(cfg, f, vm, u0, c, steps, dts)).item() / (2 * e))
         for i in comps_u:
-            e = 1e-6
+            e = 1e-5
             up, um = u0.clone(), u0.clone()
             up[i] += e
             um[i] -= e
