@@ -43,10 +43,12 @@ namespace shamx {
             for (int k = 0; k < g.n; k++) g.out[k][s] = g.in[k][src];
         }
 
-        // force-record layout (sph.hip): x y z h vx vy vz u P cs omega rho ...
-        struct FRecView {
-            f64 x, y, z, h, vx, vy, vz, u, P, cs, omega, rho, inv_sub, romega, hR2, pad;
+        // force-record layout (FRec in sph.hip); Omega is read from the merged array
+        struct alignas(16) FRecView {
+            f64 x, y, z, h, vx, vy, vz, u, P, cs, rho, inv_sub, romega, hR2, alpha, rh, rh4,
+                rromega;
         };
+        static_assert(sizeof(FRecView) == 144, "force record layout");
 
         struct ForceGrad {
             f64 *gx, *gy, *gz, *gvx, *gvy, *gvz, *gu, *gh, *gom;
@@ -55,14 +57,14 @@ namespace shamx {
         template<class K>
         __global__ void __launch_bounds__(ABLK)
             k_force_vjp(u32 n, Rows1 r, const FRecView *__restrict rec,
-                        const f64 *__restrict sax, const f64 *__restrict say,
-                        const f64 *__restrict saz, const f64 *__restrict sdu, PairParams pp,
-                        ForceGrad o) {
+                        const f64 *__restrict omega_m, const f64 *__restrict sax,
+                        const f64 *__restrict say, const f64 *__restrict saz,
+                        const f64 *__restrict sdu, PairParams pp, ForceGrad o) {
             u32 j = blockIdx.x * ABLK + threadIdx.x;
             if (j >= n) return;
             u32 row = r.sel[j], a = r.outi[row];
             const FRecView A = rec[row];
-            const PairSide SA{{A.vx, A.vy, A.vz}, A.u, A.h, A.omega};
+            const PairSide SA{{A.vx, A.vy, A.vz}, A.u, A.h, omega_m[a]};
             const f64 seed_a[3] = {sax[row], say[row], saz[row]};
             const f64 sdu_a     = sdu[row];
             f64 gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gu = 0, gh = 0, gom = 0;
@@ -71,7 +73,7 @@ namespace shamx {
                 u32 b = r.ids[k];
                 if (b == row) continue;
                 const FRecView B = rec[b];
-                const PairSide SB{{B.vx, B.vy, B.vz}, B.u, B.h, B.omega};
+                const PairSide SB{{B.vx, B.vy, B.vz}, B.u, B.h, omega_m[r.outi[b]]};
                 f64 rab[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
                 f64 acc[3], du, gr[3];
                 PairGrad g1, g2;
@@ -302,7 +304,7 @@ namespace shamx {
         dispatch_adjoint_kernel(A.kernel, [&](auto K) {
             using KT = decltype(K);
             k_force_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(
-                N, rf, (const FRecView *) A.frec, sf[0], sf[1], sf[2], sf[3], pp, g);
+                N, rf, (const FRecView *) A.frec, A.omega, sf[0], sf[1], sf[2], sf[3], pp, g);
             // Omega bar of the merged particles, then the Omega adjoint
             const f64 *oin[1] = {g.gom};
             f64 *oout[1]      = {s_omb};

@@ -225,7 +225,8 @@ namespace shamx {
         const u32 *sort_index, *ghost_src, *lpos, *cnt, *offset, *ids, *fcnt, *fids;
         const void *frec;    // FP64 force records of the step's force evaluation
         const double4 *ps;   // (x, y, z, final h), sorted
-        const f64 *h, *omega; // final h and Omega, local particles
+        const f64 *h;     // final h, local particles
+        const f64 *omega; // final Omega, merged (local particles then their images)
     };
     // seeds (11 device arrays of n: x y z vx vy vz u ax ay az du) at the end of the step are
     // replaced by the seeds at its start

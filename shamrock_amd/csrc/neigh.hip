@@ -413,7 +413,9 @@ namespace shamx {
                 y, z, h, nb.inv.data(), htol, Rker2, nullptr, nb.offset.data(), nb.ids.data(), 0, 0);
         }
 
-        u32 grow_cap(u32 need) { return need + need / 8 + 16; }
+        // row capacity for a measured maximum: 1/8 headroom, rounded up to 16 entries so that
+        // rows start on 64-byte boundaries
+        u32 grow_cap(u32 need) { return (need + need / 8 + 16 + 15) & ~15u; }
     } // namespace
 
     void neigh_build(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,

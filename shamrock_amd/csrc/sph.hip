@@ -29,6 +29,61 @@ namespace shamx {
         __device__ inline u64 umin64(u64 a, u64 b) { return a < b ? a : b; }
 
         // ------------------------------------------------------------- launch helpers
+        // Orders the LDS accesses of the lanes of one wave around this point: LDS executes one
+        // wave's operations in issue order, the fences keep the compiler from moving them across.
+        __device__ inline void wave_lds_sync() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+
+        // x / y from the correctly rounded reciprocal ry = RN(1 / y): q = RN(x ry) is within one
+        // ulp of x / y and the residual x - q y is exact (one FMA), so RN(q + (x - q y) ry) is
+        // the correctly rounded quotient (Markstein's theorem, IBM J. Res. Dev. 34(1), 1990;
+        // tests/test_division_identity.py checks it on 10^7 adversarial operand pairs).  Valid
+        // while no operand, quotient or residual leaves the normal range; the records mark the
+        // particles outside [2^-200, 2^200] (ry = 0), which take the IEEE division instead.
+        __device__ inline f64 div_rcp(f64 x, f64 y, f64 ry) {
+            f64 q = x * ry;
+            f64 e = __builtin_fma(-q, y, x);
+            return __builtin_fma(e, ry, q);
+        }
+
+        __device__ inline bool rcp_safe(f64 v) { return v > 0x1p-200 && v < 0x1p200; }
+
+        // W, dW/dr and dW/dh of kernel K at one particle's h (sphkernels.hpp:2329-2343): the same
+        // expressions as Kern<K, f64>, the divisions by h, h^3 and h^4 through their
+        // reciprocals.  FAST = false is the plain IEEE division path, for h outside
+        // [2^-200, 2^200]^(1/4) or so (`fast` says which one applies).
+        template<class K>
+        struct KernAt {
+            f64 h, h3, h4, rh, rh3, rh4;
+            bool fast;
+            __device__ explicit KernAt(f64 hh) : h(hh) {
+                h3   = hh * hh * hh;
+                h4   = hh * hh * hh * hh;
+                fast = rcp_safe(hh) && rcp_safe(h3) && rcp_safe(h4);
+                rh   = fast ? 1. / hh : 0.;
+                rh3  = fast ? 1. / h3 : 0.;
+                rh4  = fast ? 1. / h4 : 0.;
+            }
+            template<bool FAST>
+            __device__ f64 W(f64 r) const {
+                if constexpr (FAST)
+                    return div_rcp(Kern<K, f64>::norm * K::f(div_rcp(r, h, rh)), h3, rh3);
+                else
+                    return Kern<K, f64>::W(r, h);
+            }
+            template<bool FAST>
+            __device__ f64 dhW(f64 r) const {
+                if constexpr (FAST) {
+                    f64 q = div_rcp(r, h, rh);
+                    return div_rcp(-(Kern<K, f64>::norm) * (3. * K::f(q) + q * K::df(q)), h4, rh4);
+                } else
+                    return Kern<K, f64>::dhW(r, h);
+            }
+        };
+
         // The dispatcher hands block b to XCD b % 8.  Renumbering so that every XCD walks one
         // contiguous run of logical blocks keeps particles that are close in sorted order, and
         // the neighbour records they share, in the same L2.
@@ -193,23 +248,29 @@ namespace shamx {
                     rho_sum = (f64) rs - (f64) rc;
                     sumdWdh = (f64) ds - (f64) dc;
                 } else {
+                    const KernAt<K> ka(h_a);
                     // group-uniform trip count; terms folded in list order
-                    for (u32 base = k0; base < k1; base += G) {
-                        u32 k  = base + lg;
-                        f64 tw = 0, th = 0;
-                        if (k < k1) {
-                            double3 q = src.pos(r.ids[k]);
-                            f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
-                            f64 rab2 = dx * dx + dy * dy + dz * dz;
-                            if (!(rab2 > dint)) {
-                                f64 rab = sqrt(rab2);
-                                tw      = mass * KK::W(rab, h_a);
-                                th      = mass * KK::dhW(rab, h_a);
+                    auto sweep = [&](auto fast) {
+                        constexpr bool FAST = decltype(fast)::value;
+                        for (u32 base = k0; base < k1; base += G) {
+                            u32 k  = base + lg;
+                            f64 tw = 0, th = 0;
+                            if (k < k1) {
+                                double3 q = src.pos(r.ids[k]);
+                                f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
+                                f64 rab2 = dx * dx + dy * dy + dz * dz;
+                                if (!(rab2 > dint)) {
+                                    f64 rab = sqrt(rab2);
+                                    tw      = mass * ka.template W<FAST>(rab);
+                                    th      = mass * ka.template dhW<FAST>(rab);
+                                }
                             }
+                            rho_sum = ordered_fold<G>(rho_sum, tw);
+                            sumdWdh = ordered_fold<G>(sumdWdh, th);
                         }
-                        rho_sum = ordered_fold<G>(rho_sum, tw);
-                        sumdWdh = ordered_fold<G>(sumdWdh, th);
-                    }
+                    };
+                    if (ka.fast) sweep(std::true_type{});
+                    else sweep(std::false_type{});
                 }
             }
             if (MIXED) {
@@ -255,11 +316,16 @@ namespace shamx {
             k_omega(u32 n, Rows r, Src src, const f64 *__restrict h, f64 *__restrict omega,
                     f64 mass, u32 *__restrict fcnt, u32 *__restrict fids) {
             using KK = Kern<K, f64>;
+            // the kept ids of a row go through a ring in LDS and leave in runs of FL entries
+            // (64 bytes), so every line of the force rows is written whole and at once
+            constexpr u32 RING = 32, FL = 16, PER = FL / G;
+            __shared__ u32 ring[FILTER ? BLK / G : 1][RING];
             const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
             const u32 lg   = threadIdx.x % G;
+            const u32 grp  = FILTER ? threadIdx.x / G : 0;
             const bool act = j < n;
             const f64 Rkern = K::Rkern, Rker2 = K::Rkern * K::Rkern;
-            u32 row = 0, a = 0, fc = 0;
+            u32 row = 0, a = 0, fc = 0, flushed = 0;
             f64 h_a = 0, part_omega_sum = 0;
             if (act) {
                 row        = r.sel ? r.sel[j] : j;
@@ -269,6 +335,7 @@ namespace shamx {
                 f64 dint   = h_a * h_a * Rkern * Rkern;
                 f64 ca     = h_a * h_a * Rker2;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
+                const KernAt<K> ka(h_a);
                 for (u32 base = k0; base < k1; base += G) {
                     u32 k     = base + lg;
                     f64 th    = 0;
@@ -288,7 +355,8 @@ namespace shamx {
                         f64 rab2 = dx * dx + dy * dy + dz * dz;
                         if (!(rab2 > dint)) {
                             f64 rab = sqrt(rab2);
-                            th      = mass * KK::dhW(rab, h_a);
+                            if (ka.fast) th = mass * ka.template dhW<true>(rab);
+                            else th = mass * ka.template dhW<false>(rab);
                         }
                         keep = !(rab2 > ca && rab2 > hb * hb * Rker2);
                     }
@@ -296,9 +364,37 @@ namespace shamx {
                     if constexpr (FILTER) {
                         u64 bal   = __ballot(keep);
                         u32 gbits = (u32) (bal >> (__lane_id() & ~(G - 1))) & ((1u << G) - 1u);
-                        if (keep) fids[k0 + fc + (u32) __popc(gbits & ((1u << lg) - 1u))] = id;
+                        if (keep)
+                            ring[grp][(fc + (u32) __popc(gbits & ((1u << lg) - 1u))) & (RING - 1)] =
+                                id;
                         fc += (u32) __popc(gbits);
+                        if (fc - flushed >= FL) { // uniform in the group
+                            wave_lds_sync();
+                            u32 p = flushed + lg * PER;
+                            u32 *dst = fids + k0 + p;
+                            if constexpr (PER == 4) {
+                                uint4 v = make_uint4(ring[grp][p & (RING - 1)],
+                                                     ring[grp][(p + 1) & (RING - 1)],
+                                                     ring[grp][(p + 2) & (RING - 1)],
+                                                     ring[grp][(p + 3) & (RING - 1)]);
+                                if (((k0 + p) & 3u) == 0) *(uint4 *) dst = v;
+                                else {
+                                    dst[0] = v.x;
+                                    dst[1] = v.y;
+                                    dst[2] = v.z;
+                                    dst[3] = v.w;
+                                }
+                            } else {
+                                for (u32 t = 0; t < PER; t++) dst[t] = ring[grp][(p + t) & (RING - 1)];
+                            }
+                            flushed += FL;
+                            wave_lds_sync();
+                        }
                     }
+                }
+                if constexpr (FILTER) { // the last partial run
+                    wave_lds_sync();
+                    for (u32 p = flushed + lg; p < fc; p += G) fids[k0 + p] = ring[grp][p & (RING - 1)];
                 }
             }
             if (act && lg == 0) {
@@ -353,13 +449,15 @@ namespace shamx {
         __device__ inline f64 inv_sat_positive(f64 v) { return (v >= 1e-9) ? 1. / v : 0.; }
         __device__ inline f64 inv_sat_zero(f64 v) { return (v != 0. && v == v) ? 1. / v : 0.; }
 
-        // one 128 B line per particle: everything the pair loop reads of a neighbour, with the
-        // per-particle factors of UpdateDerivs.cpp:140-240 evaluated once (same IEEE operations
-        // as in the pair loop, so the values are bit-identical)
+        // everything the pair loop reads of a neighbour (144 B), with the per-particle factors
+        // of UpdateDerivs.cpp:140-240 evaluated once (the same IEEE operations as the pair loop,
+        // so the values are bit-identical) and the reciprocals of its three per-particle
+        // divisors (h, h^4, rho Omega) for div_rcp
         struct alignas(16) FRec {
-            f64 x, y, z, h, vx, vy, vz, u, P, cs, omega, rho, inv_sub, romega, hR2, pad;
+            f64 x, y, z, h, vx, vy, vz, u, P, cs, rho, inv_sub, romega, hR2, alpha;
+            f64 rh, rh4, rromega; // 1 / h, 1 / h^4, 1 / (rho Omega); 0: use the IEEE division
         };
-        static_assert(sizeof(FRec) == 128, "force record is one 128 B line");
+        static_assert(sizeof(FRec) == 144, "force record is 144 B");
         // mixed: FP64 positions for r_ab, FP32 fields (64 B)
         struct alignas(16) FRec32 {
             f64 x, y, z;
@@ -374,12 +472,17 @@ namespace shamx {
             FRec r;
             r.x = x, r.y = y, r.z = z, r.h = h;
             r.vx = vx, r.vy = vy, r.vz = vz, r.u = u;
-            r.P = P, r.cs = cs, r.omega = omega;
+            r.P = P, r.cs = cs;
             r.rho     = rho_h(mass, h, (f64) K::hfactd);
             r.inv_sub = inv_sat_zero((r.rho * r.rho) * omega);
             r.romega  = r.rho * omega;
             r.hR2     = h * h * Rker2;
-            r.pad     = 0;
+            r.alpha   = 0;
+            f64 h4    = h * h * h * h;
+            bool fast = rcp_safe(h) && rcp_safe(h4) && rcp_safe(r.romega);
+            r.rh      = fast ? 1. / h : 0.;
+            r.rh4     = fast ? 1. / h4 : 0.;
+            r.rromega = fast ? 1. / r.romega : 0.;
             return r;
         }
         template<class K>
@@ -499,13 +602,14 @@ namespace shamx {
                 const f64 P_a = A.P, u_a = A.u, cs_a = A.cs;
                 const f64 rho_a = A.rho;
                 const f64 rho_a_inv = 1. / rho_a;
-                const f64 omega_a_rho_a_inv = 1 / (A.omega * rho_a);
+                // 1 / (omega_a rho_a): romega is rho_a omega_a, the same product
+                const f64 omega_a_rho_a_inv = 1 / A.romega;
                 const f64 ia_fac = A.inv_sub;
                 const f64 ca     = A.hR2;
-                // varying AV: per-particle alpha in the record's last slot
-                // (NodeUpdateDerivsVaryingAlphaAV.cpp:60-150)
-                const f64 alpha_a = fa.varying ? A.pad : fa.alpha_AV;
-                (void) h_a;
+                // varying AV: per-particle alpha (NodeUpdateDerivsVaryingAlphaAV.cpp:60-150)
+                const f64 alpha_a = fa.varying ? A.alpha : fa.alpha_AV;
+                const f64 h4_a    = h_a * h_a * h_a * h_a;
+                const bool fast_a = A.rh4 != 0;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
                 // group-uniform trip count; the sums take the terms in list order
                 for (u32 base = k0; base < k1; base += G) {
@@ -519,14 +623,32 @@ namespace shamx {
                             npair++;
                             f64 rab   = sqrt(rab2);
                             f64 rho_b = B.rho;
-                            f64 Fab_a = KK::dW(rab, A.h), Fab_b = KK::dW(rab, B.h);
+                            // dW = norm f'(r / h) / h^4 (sphkernels.hpp:2329-2343), the two
+                            // divisions by way of the particles' reciprocals
+                            f64 Fab_a, Fab_b;
+                            if (fast_a)
+                                Fab_a = div_rcp(KK::norm * K::df(div_rcp(rab, h_a, A.rh)), h4_a,
+                                                A.rh4);
+                            else
+                                Fab_a = KK::dW(rab, h_a);
+                            // Fab_b / (rho_b omega_b) of the conductivity term alike
+                            f64 fb_rom;
+                            if (B.rh4 != 0) {
+                                f64 h4_b = B.h * B.h * B.h * B.h;
+                                Fab_b  = div_rcp(KK::norm * K::df(div_rcp(rab, B.h, B.rh)), h4_b,
+                                                 B.rh4);
+                                fb_rom = div_rcp(Fab_b, B.romega, B.rromega);
+                            } else {
+                                Fab_b  = KK::dW(rab, B.h);
+                                fb_rom = Fab_b / B.romega;
+                            }
                             f64 vabx = vxa - B.vx, vaby = vya - B.vy, vabz = vza - B.vz;
                             f64 inv  = inv_sat_positive(rab);
                             f64 rux = dx * inv, ruy = dy * inv, ruz = dz * inv;
                             f64 v_ab_r_ab = vabx * rux + vaby * ruy + vabz * ruz;
                             f64 abs_v     = fabs(v_ab_r_ab);
                             f64 vsig_a = alpha_a * cs_a + fa.beta_AV * abs_v;
-                            f64 vsig_b = (fa.varying ? B.pad : fa.alpha_AV) * B.cs +
+                            f64 vsig_b = (fa.varying ? B.alpha : fa.alpha_AV) * B.cs +
                                          fa.beta_AV * abs_v;
                             f64 rho_avg = (rho_a + rho_b) * 0.5;
                             f64 vsig_u  = sqrt(fabs(P_a - B.P) / rho_avg);
@@ -544,10 +666,12 @@ namespace shamx {
                             tu1    = AV_P_a * (omega_a_rho_a_inv * rho_a_inv) * pmass *
                                   (vabx * nax + vaby * nay + vabz * naz);
                             tu2 = pmass * fa.alpha_u * vsig_u * (u_a - B.u) * 0.5 *
-                                  (Fab_a * omega_a_rho_a_inv + Fab_b / B.romega);
-                            // CFL signal speed: r_ab / |r_ab|, alpha 1, beta 2
-                            f64 r2x = dx / rab, r2y = dy / rab, r2z = dz / rab;
-                            if (rab < 1e-9) r2x = r2y = r2z = 0;
+                                  (Fab_a * omega_a_rho_a_inv + fb_rom);
+                            // CFL signal speed: r_ab / |r_ab|, alpha 1, beta 2.  inv is
+                            // RN(1 / rab) from 1e-9 on and 0 below, where r_ab / |r_ab| is
+                            // taken as 0: div_rcp gives both
+                            f64 r2x = div_rcp(dx, rab, inv), r2y = div_rcp(dy, rab, inv),
+                                r2z = div_rcp(dz, rab, inv);
                             f64 vdot = vabx * r2x + vaby * r2y + vabz * r2z;
                             vsig_max = fmax(vsig_max, 1.0 * cs_a + 2.0 * fabs(vdot));
                         }
@@ -678,7 +802,7 @@ namespace shamx {
             if (j >= n) return;
             u32 row = r.sel[j], a = r.outi[row];
             const FRec A = rec[row];
-            f64 inv_rho_omega_a = 1. / (A.omega * A.rho);
+            f64 inv_rho_omega_a = 1. / A.romega; // 1 / (omega_a rho_a)
             f64 sum_nabla_v = 0, sc[3] = {0, 0, 0};
             u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
             for (u32 k = k0; k < k1; k++) {
@@ -803,7 +927,7 @@ namespace shamx {
             u32 s = blockIdx.x * blockDim.x + threadIdx.x;
             if (s >= nm) return;
             u32 b    = sort_index[s];
-            rec[s].pad = alpha[b < N ? b : ghost_src[b - N]];
+            rec[s].alpha = alpha[b < N ? b : ghost_src[b - N]];
         }
 
         // ------------------------------------------------------------- sink-gas gravity
