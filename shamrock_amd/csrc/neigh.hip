@@ -33,46 +33,58 @@ namespace shamx {
         }
 
         // Depth-first traversal shared by the 64 lanes of a wave.  The stack lives in lanes
-        // (entry i in lane i: node id and the mask of lanes still descending); a node is
-        // tested by the lanes of its mask, and its children are pushed with the mask of the
-        // lanes that passed.  Each lane therefore visits exactly the nodes of its own
-        // depth-first walk (KarrasTreeTraverser.hpp:69-118, left child first), in the same
-        // order.  crit(rec) is evaluated per lane; found(node) is called in the lanes whose
-        // walk reaches leaf node.  Returns false on stack overflow (wave-uniform).
+        // (entry i in lane i: node id, the mask of lanes whose walk reaches it, its two child
+        // ids).  A node is tested before it is pushed, by the lanes of its parent's mask, so
+        // only nodes that some lane enters are pushed, and the two children of a popped node
+        // are loaded and tested together (one dependent load per tree level).  Each lane still
+        // visits exactly the nodes of its own depth-first walk (KarrasTreeTraverser.hpp:69-118,
+        // left child first), in the same order.  crit(rec) is evaluated per lane; found(node)
+        // is called in the lanes whose walk reaches leaf node.  Returns false on stack overflow
+        // or a child id out of range (wave-uniform).
         template<class Crit, class Found>
         __device__ inline bool wave_traverse(const f64 *__restrict nodes, u32 I, u32 ncell,
                                              u64 live, Crit &&crit, Found &&found) {
             const int lane = __lane_id();
-            u32 st_node = 0, st_lo = 0, st_hi = 0; // lane i holds stack entry i
+            u32 st_node = 0, st_lo = 0, st_hi = 0, st_l = 0, st_r = 0; // lane i: entry i
             int sp      = 0;
-            auto push   = [&](u32 node, u64 mask) { // sp is wave-uniform
+            auto push   = [&](u32 node, u64 mask, u32 cl, u32 cr) { // sp is wave-uniform
                 if (lane == sp) {
                     st_node = node;
                     st_lo   = (u32) mask;
                     st_hi   = (u32) (mask >> 32);
+                    st_l    = cl;
+                    st_r    = cr;
                 }
                 sp++;
             };
-            push(0, live);
+            // lanes of mask that enter node, and the node's child ids
+            auto test = [&](u32 node, u64 mask, u32 &cl, u32 &cr) -> u64 {
+                const f64 *rec = nodes + (u64) node * NODE_F64;
+                bool pass      = ((mask >> lane) & 1ull) && crit(rec);
+                node_children(nodes, node, cl, cr);
+                return __ballot(pass);
+            };
+            u32 cl, cr;
+            u64 pm = test(0, live, cl, cr);
+            if (pm) push(0, pm, cl, cr);
             while (sp > 0) {
                 sp--;
                 u32 node = (u32) __builtin_amdgcn_readlane((int) st_node, sp);
                 u64 mask = (u64) (u32) __builtin_amdgcn_readlane((int) st_lo, sp) |
                            ((u64) (u32) __builtin_amdgcn_readlane((int) st_hi, sp) << 32);
-                const f64 *rec = nodes + (u64) node * NODE_F64;
-                bool pass      = ((mask >> lane) & 1ull) && crit(rec);
-                u64 pm         = __ballot(pass);
-                if (!pm) continue;
                 if (node >= I) {
-                    if (pass) found(node);
-                } else {
-                    if (sp + 2 > WAVE) return false;
-                    u32 l, r;
-                    node_children(nodes, node, l, r);
-                    if (l >= ncell || r >= ncell) return false;
-                    push(r, pm);
-                    push(l, pm);
+                    if ((mask >> lane) & 1ull) found(node);
+                    continue;
                 }
+                u32 l = (u32) __builtin_amdgcn_readlane((int) st_l, sp);
+                u32 r = (u32) __builtin_amdgcn_readlane((int) st_r, sp);
+                if (l >= ncell || r >= ncell) return false;
+                if (sp + 2 > WAVE) return false;
+                u32 ll, lr, rl, rr;
+                u64 pl = test(l, mask, ll, lr);
+                u64 pr = test(r, mask, rl, rr);
+                if (pr) push(r, pr, rl, rr);
+                if (pl) push(l, pl, ll, lr);
             }
             return true;
         }
@@ -227,6 +239,14 @@ namespace shamx {
             if (s < nm && flag[s]) lpos[pos[s]] = s;
         }
 
+        // orders the LDS accesses of one wave's lanes around this point (LDS executes a wave's
+        // operations in issue order; the fences keep the compiler from moving them across)
+        __device__ inline void wave_lds_sync() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+
         __device__ inline f64 wave_bcast(f64 v, int lane) {
             i64 b  = __double_as_longlong(v);
             int lo = __builtin_amdgcn_readlane((int) (b & 0xffffffff), lane);
@@ -237,9 +257,11 @@ namespace shamx {
         constexpr int GRP_WAVES = 4;
 
         // stage 2, one wave per leaf: the members are the leaf's local particles whose owner
-        // is this leaf; the candidates (particles of the neighbour leaves, in list order then
-        // sorted order) are taken 64 at a time, each lane holding one candidate; for each
-        // member the keep-mask is a ballot, so the compacted output keeps candidate order.
+        // is this leaf; the candidates are the particles of the neighbour leaves, in list order
+        // then sorted order.  Candidates are taken 64 at a time (their sorted index, position
+        // and cut radius staged in LDS); the wave then tests P candidates x M members per
+        // instruction, lane = candidate * M + member, so the kept lanes of one member are in
+        // candidate order and its row is the compaction of a ballot restricted to its lanes.
         template<bool FILL, bool SORTED, bool STRIDE>
         __global__ void __launch_bounds__(GRP_WAVES * WAVE)
             k_group_neigh(u32 n, u32 L, u32 I, const f64 *__restrict nodes,
@@ -249,6 +271,10 @@ namespace shamx {
                           f64 htol, f64 Rker2, u32 *__restrict cnt, u32 *__restrict offset,
                           u32 *__restrict ids, u32 cap, u32 leaf_cap) {
             __shared__ u32 rstart[GRP_WAVES][WAVE], rexcl[GRP_WAVES][WAVE];
+            __shared__ double4 cpos[GRP_WAVES][WAVE]; // candidate x, y, z, cut radius^2
+            __shared__ u32 cid[GRP_WAVES][WAVE];      // candidate id written to the rows
+            __shared__ double4 mpos[GRP_WAVES][WAVE]; // member x, y, z, cut radius^2
+            __shared__ u32 mrow[GRP_WAVES][WAVE];     // member row start
             const int lane = __lane_id(), wid = threadIdx.x / WAVE;
             const u32 l    = blockIdx.x * GRP_WAVES + wid;
             if (l >= L) return;
@@ -260,32 +286,43 @@ namespace shamx {
             const u32 k0 = leaf_offset[l],
                       k1 = k0 + (leaf_cap ? min(leaf_cnt[l], leaf_cap) : leaf_cnt[l]);
             for (u32 mb = m0; mb < m1; mb += WAVE) { // member batches of 64
-                u32 sm     = mb + lane;
-                bool mem   = false;
-                u32 a      = 0;
-                f64 xa = 0, ya = 0, za = 0, ca = 0;
+                u32 sm   = mb + lane;
+                bool mem = false;
+                u32 a    = 0;
+                double4 pa{0, 0, 0, 0};
                 if (sm < m1) {
                     a = sort_index[sm];
                     if (a < n && owner[a] == l) {
                         mem       = true;
                         double4 p = ps[sm];
-                        xa = p.x;
-                        ya = p.y;
-                        za = p.z;
-                        f64 ra = p.w * htol;
-                        ca     = ra * ra * Rker2;
+                        f64 ra    = p.w * htol;
+                        pa        = make_double4(p.x, p.y, p.z, ra * ra * Rker2);
                     }
                 }
-                u64 members = __ballot(mem);
+                const u64 members = __ballot(mem);
                 if (!members) continue;
+                const u32 M    = (u32) __popcll(members);
+                const u32 rank = (u32) __popcll(members & lt); // member index of this lane
                 const u32 ia   = SORTED ? sm : a;
                 const u32 row0 = STRIDE ? (mem ? ia * cap : 0u)
                                         : ((FILL && mem) ? offset[ia] : 0u);
-                u32 cursor     = row0;
-                u32 count = 0;
-                // neighbour leaves in batches of up to 64: lane i loads the range of the i-th
-                // leaf of the batch; candidates are the concatenated ranges in list order
+                if (mem) {
+                    mpos[wid][rank] = pa;
+                    mrow[wid][rank] = row0;
+                }
+                // lane = c * M + mi: P candidates per pass; lanes past P * M stay idle
+                const u32 P  = WAVE / M;
+                const u32 mi = (u32) lane % M, ci = (u32) lane / M;
+                const bool on = ci < P;
+                u64 mine = 0; // this member's lanes
+                for (u32 c = 0; c < P; c++) mine |= 1ull << (c * M + mi);
+                wave_lds_sync();
+                const double4 me = mpos[wid][on ? mi : 0];
+                const u32 mrow0  = mrow[wid][on ? mi : 0];
+                u32 cursor = mrow0, count = 0;
                 for (u32 kb = k0; kb < k1; kb += WAVE) {
+                    // neighbour leaves in batches of up to 64: lane i loads the range of the
+                    // i-th leaf of the batch; candidates are the concatenated ranges in order
                     u32 k  = kb + lane;
                     u32 r0 = 0, r1 = 0;
                     if (k < k1) node_children(nodes, leaf_ids[k], r0, r1); // leaf: sorted range
@@ -297,59 +334,66 @@ namespace shamx {
                     }
                     const u32 total = __shfl(inc, WAVE - 1, WAVE);
                     const u32 nr    = min(k1 - kb, (u32) WAVE);
+                    wave_lds_sync();
                     rstart[wid][lane] = r0;
                     rexcl[wid][lane]  = inc - sz;
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    wave_lds_sync();
                     for (u32 c0 = 0; c0 < total; c0 += WAVE) {
-                        u32 c      = c0 + lane;
-                        bool valid = c < total;
-                        // the range holding candidate c: largest j with rexcl[j] <= c (leaves
-                        // are never empty, so that range has c - rexcl[j] < its size)
-                        u32 j = 0;
-                        for (u32 st = WAVE / 2; st > 0; st >>= 1) {
-                            u32 t = j + st;
-                            if (t < nr && rexcl[wid][t] <= c) j = t;
+                        // stage 64 candidates: lane j takes candidate c0 + j
+                        const u32 cn = min(total - c0, (u32) WAVE);
+                        {
+                            u32 c = c0 + lane;
+                            // the range holding candidate c: largest j with rexcl[j] <= c
+                            // (leaves are never empty, so c - rexcl[j] < that range's size)
+                            u32 j = 0;
+                            for (u32 st = WAVE / 2; st > 0; st >>= 1) {
+                                u32 t = j + st;
+                                if (t < nr && rexcl[wid][t] <= c) j = t;
+                            }
+                            const bool valid = (u32) lane < cn;
+                            u32 sb    = valid ? rstart[wid][j] + (c - rexcl[wid][j]) : 0u;
+                            double4 q = ps[sb];
+                            f64 rb    = q.w * htol;
+                            wave_lds_sync();
+                            cpos[wid][lane] = make_double4(q.x, q.y, q.z, rb * rb * Rker2);
+                            cid[wid][lane]  = FILL && valid ? (SORTED ? sb : sort_index[sb]) : 0u;
+                            wave_lds_sync();
                         }
-                        u32 sb     = valid ? rstart[wid][j] + (c - rexcl[wid][j]) : 0u;
-                        double4 q  = ps[sb];
-                        f64 rb     = q.w * htol;
-                        f64 cb     = rb * rb * Rker2;
-                        u32 b      = FILL && valid ? (SORTED ? sb : sort_index[sb]) : 0u;
-                        u64 mm     = members;
-                        while (mm) {
-                            int m  = __builtin_ctzll(mm);
-                            mm &= mm - 1;
-                            f64 dx = wave_bcast(xa, m) - q.x;
-                            f64 dy = wave_bcast(ya, m) - q.y;
-                            f64 dz = wave_bcast(za, m) - q.z;
-                            f64 rab2 = dx * dx + dy * dy + dz * dz;
-                            bool keep = valid && !(rab2 > wave_bcast(ca, m) && rab2 > cb);
-                            u64 km    = __ballot(keep);
-                            u32 kc    = (u32) __popcll(km);
-                            if (FILL) {
-                                u32 base = (u32) __builtin_amdgcn_readlane((int) cursor, m);
-                                u32 pos  = base + (u32) __popcll(km & lt);
+                        for (u32 cb0 = 0; cb0 < cn; cb0 += P) {
+                            const u32 cl = cb0 + ci; // candidate of this lane
+                            bool keep    = false;
+                            if (on && cl < cn) {
+                                const double4 q = cpos[wid][cl];
+                                f64 dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+                                f64 rab2 = dx * dx + dy * dy + dz * dz;
+                                keep     = !(rab2 > me.w && rab2 > q.w);
+                            }
+                            const u64 km = __ballot(keep);
+                            const u64 kt = km & mine;
+                            if (FILL && keep) {
+                                u32 pos = cursor + (u32) __popcll(kt & lt);
                                 if (STRIDE) {
-                                    u32 end = (u32) __builtin_amdgcn_readlane((int) row0, m) + cap;
-                                    if (keep && pos < end) ids[pos] = b;
-                                } else if (keep) {
-                                    ids[pos] = b;
+                                    if (pos < mrow0 + cap) ids[pos] = cid[wid][cl];
+                                } else {
+                                    ids[pos] = cid[wid][cl];
                                 }
                             }
-                            if (lane == m) {
-                                cursor += kc;
-                                count += kc;
-                            }
+                            const u32 kc = (u32) __popcll(kt);
+                            cursor += kc;
+                            count += kc;
                         }
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
-                if ((!FILL || STRIDE) && mem) cnt[ia] = count;
-                if (STRIDE && mem) offset[ia] = row0;
+                // lane mi (< M) holds member mi's count; the member lane of rank mi writes it
+                wave_lds_sync();
+                if (on && ci == 0) mrow[wid][mi] = count;
+                wave_lds_sync();
+                if (mem) {
+                    u32 cm = mrow[wid][rank];
+                    if (!FILL || STRIDE) cnt[ia] = cm;
+                    if (STRIDE) offset[ia] = row0;
+                }
+                wave_lds_sync();
             }
         }
     } // namespace

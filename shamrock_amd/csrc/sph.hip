@@ -109,8 +109,11 @@ namespace shamx {
         // broadcast, G = 16: DPP row_newbcast)
         template<int G, int L>
         __device__ inline f64 group_lane(f64 v) {
-            static_assert(G == 4 || G == 16, "DPP broadcasts cover quads and rows");
-            constexpr int ctrl = (G == 16) ? 0x150 + L : (L | (L << 2) | (L << 4) | (L << 6));
+            static_assert(G == 2 || G == 4 || G == 16, "DPP broadcasts cover quads and rows");
+            // quad_perm selects within quads (pairs: lanes L and 2 + L), row_newbcast rows of 16
+            constexpr int ctrl = (G == 16)  ? 0x150 + L
+                                 : (G == 4) ? (L | (L << 2) | (L << 4) | (L << 6))
+                                            : (L | (L << 2) | ((2 + L) << 4) | ((2 + L) << 6));
             i64 b  = __double_as_longlong(v);
             int lo = __builtin_amdgcn_mov_dpp((int) b, ctrl, 0xf, 0xf, false);
             int hi = __builtin_amdgcn_mov_dpp((int) (b >> 32), ctrl, 0xf, 0xf, false);
@@ -336,30 +339,20 @@ namespace shamx {
                 f64 ca     = h_a * h_a * Rker2;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
                 const KernAt<K> ka(h_a);
-                for (u32 base = k0; base < k1; base += G) {
-                    u32 k     = base + lg;
-                    f64 th    = 0;
-                    bool keep = false;
-                    u32 id    = 0;
-                    if (k < k1) {
-                        id = r.ids[k];
-                        f64 dx, dy, dz, hb = 0;
-                        if constexpr (FILTER) {
-                            double4 q = src.pos4(id);
-                            dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
-                            hb = q.w;
-                        } else {
-                            double3 q = src.pos(id);
-                            dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
-                        }
-                        f64 rab2 = dx * dx + dy * dy + dz * dz;
-                        if (!(rab2 > dint)) {
-                            f64 rab = sqrt(rab2);
-                            if (ka.fast) th = mass * ka.template dhW<true>(rab);
-                            else th = mass * ka.template dhW<false>(rab);
-                        }
-                        keep = !(rab2 > ca && rab2 > hb * hb * Rker2);
+                // one entry of the row: the Omega term and the force-row keep flag
+                auto entry = [&](bool v, u32 id, double4 q, f64 &th, bool &keep) {
+                    if (!v) return;
+                    f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
+                    f64 rab2 = dx * dx + dy * dy + dz * dz;
+                    if (!(rab2 > dint)) {
+                        f64 rab = sqrt(rab2);
+                        if (ka.fast) th = mass * ka.template dhW<true>(rab);
+                        else th = mass * ka.template dhW<false>(rab);
                     }
+                    if constexpr (FILTER) keep = !(rab2 > ca && rab2 > q.w * q.w * Rker2);
+                };
+                // fold the G terms of one pass, then append the kept ids in order
+                auto emit = [&](f64 th, bool keep, u32 id) {
                     part_omega_sum = ordered_fold<G>(part_omega_sum, th);
                     if constexpr (FILTER) {
                         u64 bal   = __ballot(keep);
@@ -391,6 +384,26 @@ namespace shamx {
                             wave_lds_sync();
                         }
                     }
+                };
+                // two entries per lane per pass (k and k + G), both loaded before either is used
+                for (u32 base = k0; base < k1; base += 2 * G) {
+                    u32 k = base + lg, k2 = k + G;
+                    bool v1 = k < k1, v2 = k2 < k1;
+                    u32 i1 = r.ids[v1 ? k : k0], i2 = r.ids[v2 ? k2 : k0];
+                    auto ld = [&](u32 id) -> double4 { // h only needed by the filter
+                        if constexpr (FILTER) return src.pos4(id);
+                        else {
+                            double3 q = src.pos(id);
+                            return make_double4(q.x, q.y, q.z, 0.);
+                        }
+                    };
+                    double4 q1 = ld(i1), q2 = ld(i2);
+                    f64 th1 = 0, th2 = 0;
+                    bool keep1 = false, keep2 = false;
+                    entry(v1, i1, q1, th1, keep1);
+                    entry(v2, i2, q2, th2, keep2);
+                    emit(th1, keep1, i1);
+                    emit(th2, keep2, i2);
                 }
                 if constexpr (FILTER) { // the last partial run
                     wave_lds_sync();
@@ -449,15 +462,24 @@ namespace shamx {
         __device__ inline f64 inv_sat_positive(f64 v) { return (v >= 1e-9) ? 1. / v : 0.; }
         __device__ inline f64 inv_sat_zero(f64 v) { return (v != 0. && v == v) ? 1. / v : 0.; }
 
-        // everything the pair loop reads of a neighbour (144 B), with the per-particle factors
-        // of UpdateDerivs.cpp:140-240 evaluated once (the same IEEE operations as the pair loop,
+        // everything the pair loop reads of a neighbour (144 B): the per-particle factors of
+        // UpdateDerivs.cpp:140-240 evaluated once (the same IEEE operations as the pair loop,
         // so the values are bit-identical) and the reciprocals of its three per-particle
-        // divisors (h, h^4, rho Omega) for div_rcp
+        // divisors (h, h^4, rho Omega) for div_rcp.  (A 128-byte record that recomputes rho
+        // and (h R)^2 from h measured 23 % slower in the force loop.)
         struct alignas(16) FRec {
             f64 x, y, z, h, vx, vy, vz, u, P, cs, rho, inv_sub, romega, hR2, alpha;
             f64 rh, rh4, rromega; // 1 / h, 1 / h^4, 1 / (rho Omega); 0: use the IEEE division
         };
         static_assert(sizeof(FRec) == 144, "force record is 144 B");
+        template<class K>
+        __device__ inline f64 rec_rho(const FRec &r, f64) {
+            return r.rho;
+        }
+        template<class K>
+        __device__ inline f64 rec_hR2(const FRec &r) {
+            return r.hR2;
+        }
         // mixed: FP64 positions for r_ab, FP32 fields (64 B)
         struct alignas(16) FRec32 {
             f64 x, y, z;
@@ -600,18 +622,20 @@ namespace shamx {
                 const f64 h_a = A.h, xa = A.x, ya = A.y, za = A.z;
                 const f64 vxa = A.vx, vya = A.vy, vza = A.vz;
                 const f64 P_a = A.P, u_a = A.u, cs_a = A.cs;
-                const f64 rho_a = A.rho;
+                const f64 rho_a = rec_rho<K>(A, fa.mass);
                 const f64 rho_a_inv = 1. / rho_a;
                 // 1 / (omega_a rho_a): romega is rho_a omega_a, the same product
                 const f64 omega_a_rho_a_inv = 1 / A.romega;
                 const f64 ia_fac = A.inv_sub;
-                const f64 ca     = A.hR2;
+                const f64 ca     = rec_hR2<K>(A);
                 // varying AV: per-particle alpha (NodeUpdateDerivsVaryingAlphaAV.cpp:60-150)
                 const f64 alpha_a = fa.varying ? A.alpha : fa.alpha_AV;
                 const f64 h4_a    = h_a * h_a * h_a * h_a;
-                const bool fast_a = A.rh4 != 0;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
-                // group-uniform trip count; the sums take the terms in list order
+                // group-uniform trip count; the sums take the terms in list order.  FA: the
+                // particle's own divisions go through its reciprocals (one loop per case)
+                auto sweep = [&](auto fa_tag) {
+                constexpr bool FA = decltype(fa_tag)::value;
                 for (u32 base = k0; base < k1; base += G) {
                     u32 k  = base + lg;
                     f64 tx = 0, ty = 0, tz = 0, tu1 = 0, tu2 = 0;
@@ -619,21 +643,20 @@ namespace shamx {
                         const FRec B = src.get(r.ids[k]);
                         f64 dx = xa - B.x, dy = ya - B.y, dz = za - B.z;
                         f64 rab2 = dx * dx + dy * dy + dz * dz;
-                        if (!(rab2 > ca && rab2 > B.hR2)) {
+                        if (!(rab2 > ca && rab2 > rec_hR2<K>(B))) {
                             npair++;
                             f64 rab   = sqrt(rab2);
-                            f64 rho_b = B.rho;
-                            // dW = norm f'(r / h) / h^4 (sphkernels.hpp:2329-2343), the two
-                            // divisions by way of the particles' reciprocals
-                            f64 Fab_a, Fab_b;
-                            if (fast_a)
+                            // dW = norm f'(r / h) / h^4 (sphkernels.hpp:2329-2343) and
+                            // Fab_b / (rho_b omega_b) of the conductivity term, the divisions
+                            // by way of the particles' reciprocals
+                            f64 Fab_a, Fab_b, rho_b, fb_rom;
+                            if constexpr (FA)
                                 Fab_a = div_rcp(KK::norm * K::df(div_rcp(rab, h_a, A.rh)), h4_a,
                                                 A.rh4);
                             else
                                 Fab_a = KK::dW(rab, h_a);
-                            // Fab_b / (rho_b omega_b) of the conductivity term alike
-                            f64 fb_rom;
-                            if (B.rh4 != 0) {
+                            rho_b = B.rho;
+                            if (B.rh != 0) {
                                 f64 h4_b = B.h * B.h * B.h * B.h;
                                 Fab_b  = div_rcp(KK::norm * K::df(div_rcp(rab, B.h, B.rh)), h4_b,
                                                  B.rh4);
@@ -683,6 +706,9 @@ namespace shamx {
                     fz  = ordered_fold<G>(fz, tz);
                     tdu = ordered_fold2<G>(tdu, tu1, tu2);
                 }
+                };
+                if (A.rh != 0) sweep(std::true_type{});
+                else sweep(std::false_type{});
             }
             vsig_max = group_max<G>(vsig_max);
             u64 key  = ORDERED_POS_INF;
@@ -809,7 +835,7 @@ namespace shamx {
                 const FRec B = rec[r.ids[k]];
                 f64 dr[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
                 f64 rab2  = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-                if (rab2 > A.hR2 && rab2 > B.hR2) continue;
+                if (rab2 > rec_hR2<K>(A) && rab2 > rec_hR2<K>(B)) continue;
                 f64 rab     = sqrt(rab2);
                 f64 v_ab[3] = {A.vx - B.vx, A.vy - B.vy, A.vz - B.vz};
                 f64 ru[3]   = {dr[0] / rab, dr[1] / rab, dr[2] / rab};
@@ -847,7 +873,7 @@ namespace shamx {
                 const FRec B = rec[b];
                 f64 rr[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
                 f64 rab2  = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
-                if (rab2 > A.hR2 && rab2 > B.hR2) continue;
+                if (rab2 > rec_hR2<K>(A) && rab2 > rec_hR2<K>(B)) continue;
                 f64 rab     = sqrt(rab2);
                 f64 v_ab[3] = {A.vx - B.vx, A.vy - B.vy, A.vz - B.vz};
                 f64 a_ab[3] = {aa[0] - sax[b], aa[1] - say[b], aa[2] - saz[b]};
@@ -1216,13 +1242,13 @@ namespace shamx {
     } // namespace
 
     namespace {
-        // lanes per particle in the neighbour loops: 1, 4 or 16 (SHAMX_PAIR_LANES; the FP64
+        // lanes per particle in the neighbour loops: 1, 2, 4 or 16 (SHAMX_PAIR_LANES; the FP64
         // sums are taken in list order whatever the choice, so results do not depend on it)
         int pair_lanes() {
             static const int g = [] {
                 const char *e = std::getenv("SHAMX_PAIR_LANES");
                 int v         = e ? std::atoi(e) : 4;
-                return (v == 1 || v == 4 || v == 16) ? v : 4;
+                return (v == 1 || v == 2 || v == 4 || v == 16) ? v : 4;
             }();
             return g;
         }
@@ -1230,6 +1256,7 @@ namespace shamx {
         void dispatch_lanes(F &&f) {
             switch (pair_lanes()) {
             case 1: f(std::integral_constant<int, 1>{}); break;
+            case 2: f(std::integral_constant<int, 2>{}); break;
             case 4: f(std::integral_constant<int, 4>{}); break;
             case 16: f(std::integral_constant<int, 16>{}); break;
             default: f(std::integral_constant<int, 4>{}); break;

@@ -98,7 +98,9 @@ def test_adjoint_matches_central_differences(name):
     say(f"[adjoint fd] {name}: max |grad v| {scale_v:.3e} rel err {err_v:.2e}; "
         f"max |grad u| {scale_u:.3e} rel err {err_u:.2e}")
     assert scale_v > 0 and scale_u > 0
-    assert err_v < 1e-5 and err_u < 1e-5
+    # the forward map is smooth only to the h tolerance (1e-12 relative): a loss of O(30) then
+    # carries O(3e-11) noise, i.e. O(1e-5) absolute in a difference quotient with step 1e-6
+    assert err_v < 1e-4 and err_u < 1e-4
 
 
 def test_adjoint_independent_of_checkpoint_interval():
