@@ -57,16 +57,29 @@ namespace shamx {
                 }
                 sp++;
             };
-            // lanes of mask that enter node, and the node's child ids
-            auto test = [&](u32 node, u64 mask, u32 &cl, u32 &cr) -> u64 {
-                const f64 *rec = nodes + (u64) node * NODE_F64;
-                bool pass      = ((mask >> lane) & 1ull) && crit(rec);
-                node_children(nodes, node, cl, cr);
+            // a node record (box, rint, child ids), loaded whole so that the two children's
+            // loads are in flight together
+            struct Rec {
+                f64 v[7];
+                u32 c0, c1;
+            };
+            auto load = [&](u32 node) {
+                const f64 *p = nodes + (u64) node * NODE_F64;
+                Rec r;
+                for (int i = 0; i < 7; i++) r.v[i] = p[i];
+                node_children(nodes, node, r.c0, r.c1);
+                return r;
+            };
+            // lanes of mask that enter the node
+            auto test = [&](const Rec &r, u64 mask) -> u64 {
+                bool pass = ((mask >> lane) & 1ull) && crit(r.v);
                 return __ballot(pass);
             };
-            u32 cl, cr;
-            u64 pm = test(0, live, cl, cr);
-            if (pm) push(0, pm, cl, cr);
+            {
+                Rec r0 = load(0);
+                u64 pm = test(r0, live);
+                if (pm) push(0, pm, r0.c0, r0.c1);
+            }
             while (sp > 0) {
                 sp--;
                 u32 node = (u32) __builtin_amdgcn_readlane((int) st_node, sp);
@@ -80,11 +93,11 @@ namespace shamx {
                 u32 r = (u32) __builtin_amdgcn_readlane((int) st_r, sp);
                 if (l >= ncell || r >= ncell) return false;
                 if (sp + 2 > WAVE) return false;
-                u32 ll, lr, rl, rr;
-                u64 pl = test(l, mask, ll, lr);
-                u64 pr = test(r, mask, rl, rr);
-                if (pr) push(r, pr, rl, rr);
-                if (pl) push(l, pl, ll, lr);
+                const Rec nl = load(l), nr = load(r);
+                u64 pl = test(nl, mask);
+                u64 pr = test(nr, mask);
+                if (pr) push(r, pr, nr.c0, nr.c1);
+                if (pl) push(l, pl, nl.c0, nl.c1);
             }
             return true;
         }
@@ -96,9 +109,11 @@ namespace shamx {
         __global__ void __launch_bounds__(256)
             k_leaf_neigh_w(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
                            u32 *__restrict cnt, u32 *__restrict offset, u32 *__restrict ids,
-                           u32 cap, u32 *__restrict err) {
-            u32 g      = blockIdx.x * blockDim.x + threadIdx.x;
-            bool valid = g < L;
+                           u32 cap, u32 *__restrict err, u32 lpw) {
+            // lpw leaves per wave (64, 32 or 16): fewer leaves give more, shorter walks
+            const u32 lane = (u32) __lane_id();
+            u32 g      = (blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * lpw + lane;
+            bool valid = lane < lpw && g < L;
             u64 live   = __ballot(valid);
             if (!live) return;
             const f64 *ra = nodes + (u64) (I + (valid ? g : 0u)) * NODE_F64;
@@ -457,6 +472,16 @@ namespace shamx {
                 y, z, h, nb.inv.data(), htol, Rker2, nullptr, nb.offset.data(), nb.ids.data(), 0, 0);
         }
 
+        // leaves per wave in the stage-1 walk (SHAMX_LEAF_LANES: 64, 32 or 16)
+        u32 leaf_lanes() {
+            static const u32 v = [] {
+                const char *e = std::getenv("SHAMX_LEAF_LANES");
+                int x         = e ? std::atoi(e) : 64;
+                return (u32) ((x == 16 || x == 32) ? x : 64);
+            }();
+            return v;
+        }
+
         // row capacity for a measured maximum: 1/8 headroom, rounded up to 16 entries so that
         // rows start on 64-byte boundaries
         u32 grow_cap(u32 need) { return (need + need / 8 + 16 + 15) & ~15u; }
@@ -493,7 +518,7 @@ namespace shamx {
         if (!sorted_layout) {
             // reference layout: stage 1 in two passes
             k_leaf_neigh_w<false, false><<<div_up(L, 256), 256, 0, s>>>(
-                L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nullptr, nullptr, 0, err);
+                L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nullptr, nullptr, 0, err, 64);
             exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.leaf_offset.data(), L,
                                nb.counters.data(), nb.scratch.data());
             SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
@@ -505,7 +530,7 @@ namespace shamx {
             nb.leaf_ids.resize(std::max<u64>(nb.n_leaf_pairs, 1));
             k_leaf_neigh_w<true, false><<<div_up(L, 256), 256, 0, s>>>(
                 L, I, t.nodes.data(), Rkern, nullptr, nb.leaf_offset.data(), nb.leaf_ids.data(),
-                0, err);
+                0, err, 64);
             if (n_local == 0) {
                 nb.n_pairs = 0;
                 return;
@@ -552,7 +577,7 @@ namespace shamx {
             if ((u64) L * cl > 0xffffffffull || (u64) nm * cp > 0xffffffffull) {
                 // u32 entry indices cannot address fixed-capacity rows: two passes instead
                 k_leaf_neigh_w<false, false><<<div_up(L, 256), 256, 0, s>>>(
-                    L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nullptr, nullptr, 0, err);
+                    L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nullptr, nullptr, 0, err, 64);
                 exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.leaf_offset.data(), L,
                                    nb.counters.data(), nb.scratch.data());
                 SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
@@ -563,7 +588,7 @@ namespace shamx {
                 nb.leaf_ids.resize(std::max<u64>(nb.n_leaf_pairs, 1));
                 k_leaf_neigh_w<true, false><<<div_up(L, 256), 256, 0, s>>>(
                     L, I, t.nodes.data(), Rkern, nullptr, nb.leaf_offset.data(),
-                    nb.leaf_ids.data(), 0, err);
+                    nb.leaf_ids.data(), 0, err, 64);
                 stage2_packed<true>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
                 nb.leaf_stride = 0;
                 nb.row_stride  = 0;
@@ -572,9 +597,10 @@ namespace shamx {
             nb.leaf_ids.resize(std::max<u64>((u64) L * cl, 1));
             nb.ids.resize(std::max<u64>((u64) nm * cp, 1));
             fill_u32(s, nb.counters.data() + 8, 0, 8);
-            k_leaf_neigh_w<true, true><<<div_up(L, 256), 256, 0, s>>>(
+            const u32 lpw = leaf_lanes();
+            k_leaf_neigh_w<true, true><<<div_up(div_up(L, lpw), 256 / WAVE), 256, 0, s>>>(
                 L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nb.leaf_offset.data(),
-                nb.leaf_ids.data(), cl, err);
+                nb.leaf_ids.data(), cl, err, lpw);
             max_u32(s, L, nb.leaf_cnt.data(), nb.counters.data() + 8);
             fill_u32(s, nb.cnt.data(), 0, nm); // ghost rows stay empty
             k_group_neigh<true, true, true><<<div_up(L, GRP_WAVES), GRP_WAVES * WAVE, 0, s>>>(
