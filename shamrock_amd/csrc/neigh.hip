@@ -80,18 +80,41 @@ namespace shamx {
                 u64 pm = test(r0, live);
                 if (pm) push(0, pm, r0.c0, r0.c1);
             }
+            auto entry_mask = [&](int e) {
+                return (u64) (u32) __builtin_amdgcn_readlane((int) st_lo, e) |
+                       ((u64) (u32) __builtin_amdgcn_readlane((int) st_hi, e) << 32);
+            };
             while (sp > 0) {
-                sp--;
-                u32 node = (u32) __builtin_amdgcn_readlane((int) st_node, sp);
-                u64 mask = (u64) (u32) __builtin_amdgcn_readlane((int) st_lo, sp) |
-                           ((u64) (u32) __builtin_amdgcn_readlane((int) st_hi, sp) << 32);
-                if (node >= I) {
+                u32 node = (u32) __builtin_amdgcn_readlane((int) st_node, sp - 1);
+                u64 mask = entry_mask(sp - 1);
+                if (node >= I) { // a leaf on top: this is its turn in every lane's walk
+                    sp--;
                     if ((mask >> lane) & 1ull) found(node);
                     continue;
                 }
-                u32 l = (u32) __builtin_amdgcn_readlane((int) st_l, sp);
-                u32 r = (u32) __builtin_amdgcn_readlane((int) st_r, sp);
+                u32 l = (u32) __builtin_amdgcn_readlane((int) st_l, sp - 1);
+                u32 r = (u32) __builtin_amdgcn_readlane((int) st_r, sp - 1);
                 if (l >= ncell || r >= ncell) return false;
+                // the entry below is expanded in the same step when it is internal too: its
+                // children go under the top entry's, so every walk keeps its order, and four
+                // child records are in flight at once
+                u32 node2 = sp >= 2 ? (u32) __builtin_amdgcn_readlane((int) st_node, sp - 2) : I;
+                if (node2 < I && sp + 2 <= WAVE - 8) {
+                    u64 mask2 = entry_mask(sp - 2);
+                    u32 l2    = (u32) __builtin_amdgcn_readlane((int) st_l, sp - 2);
+                    u32 r2    = (u32) __builtin_amdgcn_readlane((int) st_r, sp - 2);
+                    if (l2 >= ncell || r2 >= ncell) return false;
+                    sp -= 2;
+                    const Rec nl = load(l), nr = load(r), nl2 = load(l2), nr2 = load(r2);
+                    u64 pl = test(nl, mask), pr = test(nr, mask);
+                    u64 pl2 = test(nl2, mask2), pr2 = test(nr2, mask2);
+                    if (pr2) push(r2, pr2, nr2.c0, nr2.c1);
+                    if (pl2) push(l2, pl2, nl2.c0, nl2.c1);
+                    if (pr) push(r, pr, nr.c0, nr.c1);
+                    if (pl) push(l, pl, nl.c0, nl.c1);
+                    continue;
+                }
+                sp--;
                 if (sp + 2 > WAVE) return false;
                 const Rec nl = load(l), nr = load(r);
                 u64 pl = test(nl, mask);
