@@ -8,8 +8,8 @@
 The compute path is libshamx_sph.so (hand-written HIP for gfx950, C ABI in
 include/shamx_sph.h); see shamrock_amd/model.py for the Python surface.
 """
-from .model import (Context, EvolveUntilResults, SPHModel, SolverConfig, TimestepLog,
-                    get_Model_SPH)
+from .model import (Constants, Context, EvolveUntilResults, SPHModel, SolverConfig,
+                    TimestepLog, UnitSystem, get_Model_SPH)
 
 __all__ = ["Context", "get_Model_SPH", "SPHModel", "SolverConfig", "TimestepLog",
-           "EvolveUntilResults"]
+           "EvolveUntilResults", "UnitSystem", "Constants"]

@@ -9,6 +9,7 @@ by the GPU solver (libshamx_sph.so) and copied back on demand.
 from __future__ import annotations
 
 import dataclasses
+import json
 import math
 import time
 from typing import Callable
@@ -308,6 +309,30 @@ class SPHModel:
         names = _LAYOUT_CD10 if self.cfg.d.get("av_kind", 0) == 1 else _LAYOUT_CONSTANT
         return {k: self._field_get(k) for k in names}
 
+    # --- restart dumps ---------------------------------------------------------------------
+    def dump(self, fname):
+        """Model::dump (Model.hpp:959-991): the whole state needed to continue the run, i.e.
+        the solver config, the time edges (t, dt, CFL multiplier), the sinks and every field of
+        the particles, in one file.  The layout is this library's own (see write_dump), not
+        the reference's MPI patch serialisation."""
+        self._sync_from_gpu()
+        write_dump(fname, self.kernel, self.cfg.d, dict(t=self.t, dt=self.dt,
+                                                         cfl_mult=self.cfl_mult),
+                   self.sinks, self.host, self.step_count)
+
+    def load_from_dump(self, fname):
+        """Model::load_from_dump (Model.hpp:906-957) for files written by dump."""
+        kernel, cfg, tm, sinks, fields, steps = read_dump(fname)
+        if kernel != self.kernel:
+            raise ValueError(f"dump holds a {kernel} model, this model is {self.kernel}")
+        self._invalidate()
+        self.cfg = SolverConfig(kernel)
+        self.cfg.d.update(cfg)
+        self.t, self.dt, self.cfl_mult = tm["t"], tm["dt"], tm["cfl_mult"]
+        self.sinks = sinks
+        self.host = {k: fields.get(k, np.zeros(fields["x"].size)) for k in FIELDS}
+        self.step_count = steps
+
     # --- sinks ---------------------------------------------------------------------------
     def add_sink(self, mass, pos, velocity, accretion_radius):
         """Model::add_sink (Model.hpp:183-200)."""
@@ -462,6 +487,53 @@ class SPHModel:
 
     def solver_logs_reset_step_count(self):
         self.step_count = 0
+
+
+_DUMP_MAGIC = b"SHAMXDMP"
+_DUMP_VERSION = 1
+
+
+def write_dump(fname, kernel, cfg, time_state, sinks, fields, step_count=0):
+    """Restart file: 8-byte magic, u64 version, u64 header length, a JSON header (kernel,
+    config, time edges, step count, sink records, field names and particle count), then each
+    field as n little-endian float64 values in header order."""
+    n = int(np.asarray(fields["x"]).size)
+    names = [k for k in FIELDS if k in fields]
+    cfg = {k: (list(v) if isinstance(v, (list, tuple, np.ndarray)) else v)
+           for k, v in cfg.items() if k != "sinks"}
+    head = dict(kernel=kernel, config=cfg, time=dict(time_state), step_count=int(step_count),
+                sinks=np.asarray(sinks, float).reshape(-1, 17).tolist(), n=n, fields=names)
+    hb = json.dumps(head, sort_keys=True).encode()
+    with open(fname, "wb") as f:
+        f.write(_DUMP_MAGIC)
+        f.write(np.array([_DUMP_VERSION, len(hb)], "<u8").tobytes())
+        f.write(hb)
+        for k in names:
+            a = np.ascontiguousarray(fields[k], "<f8")
+            if a.size != n:
+                raise ValueError(f"field {k} has {a.size} values, expected {n}")
+            f.write(a.tobytes())
+
+
+def read_dump(fname):
+    """Inverse of write_dump: (kernel, config, time edges, sink records, fields, step count)."""
+    with open(fname, "rb") as f:
+        if f.read(8) != _DUMP_MAGIC:
+            raise ValueError(f"{fname}: not a dump written by this library")
+        version, hlen = np.frombuffer(f.read(16), "<u8")
+        if int(version) != _DUMP_VERSION:
+            raise ValueError(f"{fname}: dump version {int(version)} is not supported")
+        head = json.loads(f.read(int(hlen)).decode())
+        n = int(head["n"])
+        fields = {}
+        for k in head["fields"]:
+            a = np.frombuffer(f.read(8 * n), "<f8")
+            if a.size != n:
+                raise ValueError(f"{fname}: truncated field {k}")
+            fields[k] = a.astype(np.float64)
+    sinks = np.asarray(head["sinks"], float).reshape(-1, 17)
+    return (head["kernel"], head["config"], head["time"], sinks, fields,
+            int(head.get("step_count", 0)))
 
 
 _KERNELS = ("M4", "M6", "M8", "C2", "C4", "C6")

@@ -1,0 +1,73 @@
+"""Restart dumps of the Python model (Model::dump / load_from_dump, Model.hpp:906-991): the
+config, time edges, sinks and every particle field survive a write / read round trip exactly;
+foreign or truncated files and a kernel mismatch are refused.  The GPU continuation test is in
+tests/test_gpu_dump.py."""
+import numpy as np
+import pytest
+
+import shamrock_amd as shamrock
+from shamrock_amd import model as model_mod
+
+
+def _model():
+    ctx = shamrock.Context()
+    ctx.pdata_layout_new()
+    m = shamrock.get_Model_SPH(context=ctx, vector_type="f64_3", sph_kernel="M4")
+    cfg = m.gen_default_config()
+    cfg.set_artif_viscosity_VaryingCD10(alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1,
+                                        alpha_u=1.0, beta_AV=2.0)
+    cfg.set_boundary_free()
+    cfg.set_eos_locally_isothermalLP07(cs0=0.05, q=0.25, r0=1.0)
+    cfg.set_units(shamrock.UnitSystem(unit_time=5.0, unit_length=2.0, unit_mass=3.0))
+    cfg.set_particle_mass(1e-4)
+    m.set_solver_config(cfg)
+    rng = np.random.default_rng(0)
+    pos = rng.normal(size=(500, 3))
+    m.push_particle(pos, np.full(500, 0.2), rng.uniform(1, 2, 500))
+    m.set_field_value_lambda_f64_3("vxyz", lambda p: (p[1], -p[0], 0.0))
+    m.add_sink(1.0, (0.1, 0.2, 0.3), (0.0, 0.01, 0.0), 0.05)
+    m.set_time(0.25)
+    m.set_next_dt(1e-3)
+    m.set_cfl_multipler(0.5)
+    return ctx, m
+
+
+def test_dump_round_trip(tmp_path):
+    ctx, m = _model()
+    f = str(tmp_path / "state.shamx")
+    m.dump(f)
+    ctx2 = shamrock.Context()
+    ctx2.pdata_layout_new()
+    m2 = shamrock.get_Model_SPH(context=ctx2, vector_type="f64_3", sph_kernel="M4")
+    m2.load_from_dump(f)
+    d1, d2 = ctx.collect_data(), ctx2.collect_data()
+    assert set(d1) == set(d2) and "alpha_AV" in d2 and "axyz_ext" in d2
+    for k in d1:
+        np.testing.assert_array_equal(d1[k], d2[k], err_msg=k)
+    assert (m2.get_time(), m2.get_dt()) == (0.25, 1e-3)
+    assert m2.cfl_mult == 0.5
+    c1, c2 = m.get_current_config().d, m2.get_current_config().d
+    for k in ("av_kind", "eos_kind", "eos_cs0", "eos_q", "constant_G", "gpart_mass", "periodic"):
+        assert c1[k] == c2[k], k
+    assert c2["constant_G"] == pytest.approx(6.6743015e-11 * 3.0 * 25.0 / 8.0, rel=1e-15)
+    s1, s2 = m.get_sinks(), m2.get_sinks()
+    assert len(s2) == 1 and s1[0] == s2[0]
+
+
+def test_dump_refuses_foreign_and_mismatched_files(tmp_path):
+    ctx, m = _model()
+    f = str(tmp_path / "state.shamx")
+    m.dump(f)
+    ctx2 = shamrock.Context()
+    m6 = shamrock.get_Model_SPH(context=ctx2, vector_type="f64_3", sph_kernel="M6")
+    with pytest.raises(ValueError, match="M4"):
+        m6.load_from_dump(f)
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"NOTADUMP" + bytes(64))
+    with pytest.raises(ValueError, match="not a dump"):
+        model_mod.read_dump(str(bad))
+    raw = open(f, "rb").read()
+    cut = tmp_path / "cut.shamx"
+    cut.write_bytes(raw[:-8])
+    with pytest.raises(ValueError, match="truncated"):
+        model_mod.read_dump(str(cut))
