@@ -88,6 +88,80 @@ def copy_probe(nbytes=512 << 20):
     log(f"[bench] copy probe: {nbytes} B read + {nbytes} B written")
 
 
+TREE_BYTES = (132, 68, 182)  # per merged particle, per leaf, per internal cell
+
+
+def scale_probe(n=16_777_216, reps=5):
+    """Sort and tree kernels at a size where they are bandwidth-bound: the clustered Plummer
+    workload (16,777,216 particles, h over three decades), timed with HIP events on the
+    stream they run on.  Sort: 68 B per key (30-bit keys, 8-bit digits).  Tree (bbox, Morton,
+    leaf reduction, Karras, cell boxes and h-max, without the sort): 132 B per particle + 68 B
+    per leaf + 182 B per internal cell."""
+    import ctypes as C
+
+    import torch
+
+    from shamrock_amd import _lib, setups
+    from shamrock_amd._lib import check, lib
+
+    pos, hh = setups.plummer_positions(n, seed=1, r_max=30.0)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    x, y, z, h = dev(pos[:, 0]), dev(pos[:, 1]), dev(pos[:, 2]), dev(hh)
+    del pos, hh
+    ctx = C.c_void_p()
+    check(lib().shamx_ctx_create(C.byref(ctx)))
+    s = torch.cuda.current_stream()
+    sp = C.c_void_p(s.cuda_stream)
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    perm = torch.empty(n, dtype=torch.int32, device="cuda")
+    codes = torch.empty(n, dtype=torch.int32, device="cuda")
+    bbox = torch.empty(6, dtype=torch.float64, device="cuda")
+    check(lib().shamx_bbox(ctx, sp, n, x.data_ptr(), y.data_ptr(), z.data_ptr(), bbox.data_ptr()))
+    check(lib().shamx_morton(ctx, sp, n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
+                             bbox.data_ptr(), codes.data_ptr()))
+
+    def timed(fn):
+        fn()  # warm (buffers grown)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ms = []
+        for _ in range(reps):
+            t0.record(s)
+            fn()
+            t1.record(s)
+            t1.synchronize()
+            ms.append(t0.elapsed_time(t1))
+        return sorted(ms)[len(ms) // 2] / 1e3
+
+    def sort():
+        keys.copy_(codes)
+        check(lib().shamx_sort(ctx, sp, n, 30, keys.data_ptr(), perm.data_ptr()))
+
+    t_copy = timed(lambda: keys.copy_(codes))
+    t_sort = timed(sort) - t_copy
+    t_rk = timed(lambda: check(lib().shamx_tree_reduce_karras(
+        ctx, sp, n, x.data_ptr(), y.data_ptr(), z.data_ptr(), 3)))
+    t_ab = timed(lambda: check(lib().shamx_tree_aabb_hmax(
+        ctx, sp, x.data_ptr(), y.data_ptr(), z.data_ptr(), h.data_ptr(), 1.1)))
+    sz = (C.c_uint32 * 4)()
+    check(lib().shamx_tree_sizes(ctx, sz))
+    L, I = int(sz[1]), int(sz[2])
+    lib().shamx_ctx_destroy(ctx)
+    t_tree = t_rk - t_sort + t_ab
+    sort_b = SORT_BYTES_PER_KEY * n
+    tree_b = TREE_BYTES[0] * n + TREE_BYTES[1] * L + TREE_BYTES[2] * I
+    out = {}
+    for k, b, t, alg in (("sort", sort_b, t_sort, f"{SORT_BYTES_PER_KEY} B x {n} keys"),
+                         ("tree", tree_b, t_tree,
+                          f"132 B x {n} + 68 B x {L} leaves + 182 B x {I} cells")):
+        gbs = b / t / 1e9
+        out[f"{k}_plummer_16M"] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                                   "avg_ms": t * 1e3, "algorithmic": alg}
+    log(f"[bench] scale probe n={n}: sort {t_sort * 1e3:.3f} ms, tree {t_tree * 1e3:.3f} ms "
+        f"(L={L}, I={I})")
+    return out
+
+
 def load_traffic(workload):
     """HBM bytes per launch measured by rocprofv3 (profiles/traffic.json), for this workload."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -142,11 +216,13 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 512 MiB device copy (counter calibration)")
+    ap.add_argument("--no-scale-probe", action="store_true",
+                    help="skip the sort / tree timing at 16,777,216 clustered particles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU design "
-                                   "(patch decomposition with ghost exchange) is described in "
-                                   "DESIGN.md and not built", "n_gpus": args.gpus}))
+                                   "(patch decomposition with ghost exchange) is written up "
+                                   "in the design notes and not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     from shamrock_amd import _lib, setups
@@ -221,6 +297,8 @@ def main():
         "kernels": kernels,
     }
     g64.close()
+    if not args.no_scale_probe:
+        kernels.update(scale_probe())
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline (oracle, {args.cpu_threads} threads)")
         out["cpu_baseline"] = cpu_baseline(cfg, fields, args.cpu_threads)

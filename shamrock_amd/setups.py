@@ -114,12 +114,9 @@ def random_box(n: int, seed: int = 0, kernel: str = "M4", periodic=True, jitter=
     return cfg, fields_from(pos, dr, u, v)
 
 
-def plummer(n: int, seed: int = 0, a: float = 1.0, M: float = 1.0, r_max: float = 30.0,
-            u: float = 0.05, kernel: str = "M4") -> tuple[dict, dict]:
-    """Plummer sphere (scale radius a, mass M) truncated at r_max, free boundaries, zero
-    velocities, adiabatic gamma = 5/3 with u = 0.05.  Radii by the inverse of the enclosed-mass
-    fraction r^3 / (r^2 + a^2)^(3/2), drawn inside the truncation; isotropic directions; h from
-    the analytic density, h = hfact (m / rho)^(1/3)."""
+def plummer_positions(n: int, seed: int = 0, a: float = 1.0, M: float = 1.0,
+                      r_max: float = 30.0, kernel: str = "M4"):
+    """Positions (n, 3) and smoothing lengths of the Plummer sphere of `plummer`."""
     rng = np.random.default_rng(seed)
     f_max = r_max**3 / (r_max**2 + a * a) ** 1.5
     X = rng.uniform(0, f_max, n)
@@ -127,9 +124,19 @@ def plummer(n: int, seed: int = 0, a: float = 1.0, M: float = 1.0, r_max: float 
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     pos = np.ascontiguousarray(r[:, None] * d)
-    m = M / n
     rho = 3 * M / (4 * math.pi * a**3) * (1 + r * r / (a * a)) ** -2.5
-    h = KERNEL_HFACT[kernel] * (m / rho) ** (1.0 / 3.0)
+    h = KERNEL_HFACT[kernel] * ((M / n) / rho) ** (1.0 / 3.0)
+    return pos, h
+
+
+def plummer(n: int, seed: int = 0, a: float = 1.0, M: float = 1.0, r_max: float = 30.0,
+            u: float = 0.05, kernel: str = "M4") -> tuple[dict, dict]:
+    """Plummer sphere (scale radius a, mass M) truncated at r_max, free boundaries, zero
+    velocities, adiabatic gamma = 5/3 with u = 0.05.  Radii by the inverse of the enclosed-mass
+    fraction r^3 / (r^2 + a^2)^(3/2), drawn inside the truncation; isotropic directions; h from
+    the analytic density, h = hfact (m / rho)^(1/3)."""
+    pos, h = plummer_positions(n, seed, a, M, r_max, kernel)
+    m = M / n
     lo, hi = pos.min(0) - 1, pos.max(0) + 1
     cfg = base_config(kernel, lo, hi, periodic=False, gpart_mass=m)
     return cfg, fields_from(pos, h, u)
