@@ -43,6 +43,8 @@ namespace shamx {
         DevArray<u32> sort_index;      // n_obj permutation (map_morton_id_to_obj_id)
         DevArray<u32> keys_alt, vals_alt;
         DevArray<u32> split_a, split_b;
+        DevArray<u32> red_cnt, red_pos; // leaf reduction: flags before i, flagged indices
+        DevArray<u32> big_leaves;       // leaves too large for the lane-group box kernel
         DevArray<u32> reduc_map;       // L + 2
         DevArray<u32> leaf_codes;      // L
         DevArray<u32> lchild, rchild, endrange; // I
