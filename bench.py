@@ -107,7 +107,6 @@ def scale_probe(n=16_777_216, reps=5):
     pos, hh = setups.plummer_positions(n, seed=1, r_max=30.0)
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
     x, y, z, h = dev(pos[:, 0]), dev(pos[:, 1]), dev(pos[:, 2]), dev(hh)
-    del pos, hh
     ctx = C.c_void_p()
     check(lib().shamx_ctx_create(C.byref(ctx)))
     s = torch.cuda.current_stream()
@@ -159,7 +158,47 @@ def scale_probe(n=16_777_216, reps=5):
                                    "avg_ms": t * 1e3, "algorithmic": alg}
     log(f"[bench] scale probe n={n}: sort {t_sort * 1e3:.3f} ms, tree {t_tree * 1e3:.3f} ms "
         f"(L={L}, I={I})")
+    del x, y, z, h, keys, perm, codes
+    torch.cuda.empty_cache()
+    out["pass_plummer_16M"] = plummer_pass(n, pos, hh)
     return out
+
+
+def plummer_pass(n, pos, hh, reps=3):
+    """The clustered workload's timed sequence on the solver: Morton sort, tree, neighbour
+    cache, one density (h subcycle + Omega) and force pass, FP64 and mixed; phase times from
+    the solver's HIP events, medians over reps passes after one warm pass."""
+    import torch
+
+    from shamrock_amd import setups
+    from shamrock_amd.solver import GpuSolver
+
+    res = {}
+    for mixed in (0, 1):
+        cfg, f = setups.plummer(n, seed=1, r_max=30.0, pos_h=(pos, hh))
+        cfg["mixed_precision"] = mixed
+        g = GpuSolver(cfg, f)
+        del f
+        logs = []
+        for k in range(reps + 1):
+            g.set("h", hh)  # every pass starts from the same h
+            lg = g.density_force()
+            log(f"[bench] plummer pass mixed={mixed} rep {k}: {lg['t_total'] * 1e3:.1f} ms, "
+                f"h iterations {lg['h_iterations']}, pairs {lg['n_pairs']}")
+            if k:
+                logs.append(lg)
+        med = lambda key: float(np.median([lg[key] for lg in logs]))  # noqa: E731
+        ph = {p: med("t_" + p) * 1e3 for p in ("sort", "tree", "neigh", "hiter", "omega",
+                                                "eos", "force")}
+        tot = med("t_total")
+        res["mixed" if mixed else "fp64"] = {
+            "ms": tot * 1e3, "particles_per_s": n / tot, "phases_ms": ph,
+            "h_iterations": int(logs[-1]["h_iterations"]), "n_pairs": int(logs[-1]["n_pairs"]),
+            "n_leaf": int(logs[-1]["n_leaf"])}
+        g.close()
+        del g
+        torch.cuda.empty_cache()
+    return res
 
 
 def load_traffic(workload):
@@ -298,7 +337,9 @@ def main():
     }
     g64.close()
     if not args.no_scale_probe:
-        kernels.update(scale_probe())
+        probe = scale_probe()
+        out["plummer_16M_pass"] = probe.pop("pass_plummer_16M")
+        kernels.update(probe)
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline (oracle, {args.cpu_threads} threads)")
         out["cpu_baseline"] = cpu_baseline(cfg, fields, args.cpu_threads)

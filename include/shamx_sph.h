@@ -216,6 +216,16 @@ int shamx_solver_get_sinks(shamx_solver *s, uint32_t *count, double *out17);
 int shamx_solver_set_sinks(shamx_solver *s, uint32_t count, const double *in17);
 /* Solver::evolve_once (Solver.cpp:1942-3272) */
 int shamx_solver_evolve_once(shamx_solver *s, shamx_step_log *log);
+/* One density and force pass of the step on the current state, without time integration:
+ * ghosts, tree, neighbour cache and ONE smoothing-length subcycle (Solver.cpp:1060-1304; its
+ * unconverged particles keep h clamped to the cache radius rather than failing), Omega, EOS,
+ * the force loop and the CFL reduction (Solver.cpp:2247-2400, UpdateDerivs.cpp:88-288,
+ * Solver.cpp:3095-3124).  ax/ay/az/du are overwritten (previous values kept as the *_old of a
+ * step), h takes the subcycle's values; x, v, u and the time are untouched.  log->next_dt is
+ * the CFL step.  Constant viscosity only (av_kind 0).  The reference runs these phases only
+ * inside evolve_once; this entry exposes them for the clustered-tree timing workload, whose
+ * h iteration does not converge in a full step. */
+int shamx_solver_density_force(shamx_solver *s, shamx_step_log *log);
 /* stream used by the solver (hipStream_t as void*) */
 int shamx_solver_stream(shamx_solver *s, void **stream);
 

@@ -503,6 +503,14 @@ int orc_solver_evolve_once(void *h, int kernel, StepStats *stats) {
     });
 }
 
+// the density/force pass of Solver::density_force (no time integration)
+int orc_solver_density_force(void *h, int kernel, StepStats *stats) {
+    return with_solver(h, kernel, [&](auto &s) {
+        s.density_force();
+        if (stats) *stats = s.stats;
+    });
+}
+
 // transient products of the last step; sizes: [Nm, G, L, I, n_pairs, n_leaf_pairs]
 int orc_solver_sizes(void *h, int kernel, uint64_t *out6) {
     return with_solver(h, kernel, [&](auto &s) {

@@ -222,9 +222,10 @@ namespace orc {
             for (u32 g = 0; g < G; g++) malpha[N + g] = alpha_new[ghost_src[g]];
         }
 
-        // Solver.cpp:1060-1304
-        void prestep() {
-            u32 hstep = 0, hmax = (u32) cfg.h_max_subcycles;
+        // Solver.cpp:1060-1304.  max_sub subcycles; with require_converged false a last
+        // subcycle that did not converge is kept (its clamped h) instead of failing.
+        void prestep(u32 max_sub, bool require_converged) {
+            u32 hstep = 0, hmax = max_sub;
             std::vector<f64> eps(N), h_old(N);
             stats.h_iterations = 0;
             for (; hstep < hmax; hstep++) {
@@ -257,10 +258,10 @@ namespace orc {
                 for (u32 i = 0; i < N; i++) min_eps = std::fmin(min_eps, eps[i]);
                 stats.t_hiter += th.sec();
                 bool converged = (max_eps < cfg.epsilon_h) && !(min_eps < 0);
-                if (!converged) continue;
+                if (!converged && (require_converged || hstep + 1 < hmax)) continue;
                 break;
             }
-            stats.subcycles = hstep + 1;
+            stats.subcycles = std::min(hstep + 1, hmax);
             if (hstep == hmax) throw std::runtime_error("the h iterator is not converged");
             Timer to;
             omega.assign(N, 0.);
@@ -335,7 +336,7 @@ namespace orc {
                 }
             }
 
-            prestep();
+            prestep((u32) cfg.h_max_subcycles, true);
 
             // corrector loop: Solver.cpp:2247-3171
             std::vector<f64> ax_old(N), ay_old(N), az_old(N), du_old(N), eps_v2(N);
@@ -448,6 +449,63 @@ namespace orc {
             t  = t_cur + dtc;
             f64 stiff = cfg.cfl_multiplier_stiffness;
             cfl_mult  = (cfl_mult * stiff + 1.) / (stiff + 1.);
+        }
+
+        // The phases of evolve_once between the predictor and the corrector, on the current
+        // state: sink forces, one h subcycle kept whether or not it converged, Omega, EOS,
+        // forces (+ external), vsig and the CFL step.  Constant viscosity only.
+        void density_force() {
+            if (cfg.av_kind != 0)
+                throw std::runtime_error("the density/force pass takes the constant viscosity only");
+            stats = StepStats{};
+            Timer tstep;
+            if (!sinks.empty()) {
+                sink_ext_forces();
+                sink_gas_forces();
+            }
+            prestep(1, false);
+            merge_fields();
+            Timer te;
+            P.assign(Nm, 0.);
+            cs.assign(Nm, 0.);
+            compute_eos<K>(Nm, mh.data(), mu.data(), mx.data(), my.data(), mz.data(), P.data(),
+                           cs.data(), cfg.gpart_mass, eos());
+            stats.t_eos += te.sec();
+            Timer tf;
+            AVParams av{cfg.alpha_u, cfg.alpha_AV, cfg.beta_AV};
+            update_derivs_constant_av<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(),
+                                         mvx.data(), mvy.data(), mvz.data(), mh.data(),
+                                         momega.data(), mu.data(), P.data(), cs.data(), ax.data(),
+                                         ay.data(), az.data(), du.data(), cfg.gpart_mass, av,
+                                         nullptr);
+            if (!sinks.empty())
+                for (u32 i = 0; i < N; i++) {
+                    ax[i] = ax[i] + axe[i];
+                    ay[i] = ay[i] + aye[i];
+                    az[i] = az[i] + aze[i];
+                }
+            stats.t_force += tf.sec();
+            Timer tl;
+            vsig.assign(N, 0.);
+            compute_vsig<K>(N, neigh.part_cache, mx.data(), my.data(), mz.data(), mvx.data(),
+                            mvy.data(), mvz.data(), mh.data(), cs.data(), vsig.data());
+            f64 C_cour = cfg.cfl_cour * cfl_mult, C_force = cfg.cfl_force * cfl_mult;
+            f64 rank_dt = std::numeric_limits<f64>::infinity();
+            for (u32 i = 0; i < N; i++) {
+                f64 h_a    = mh[i];
+                f64 dt_c   = C_cour * h_a / vsig[i];
+                f64 abs_a  = std::sqrt(dot({ax[i], ay[i], az[i]}, {ax[i], ay[i], az[i]}));
+                f64 dt_f   = C_force * std::sqrt(h_a / abs_a);
+                f64 cfl_dt = std::fmin(std::numeric_limits<f64>::infinity(), dt_c);
+                cfl_dt     = std::fmin(cfl_dt, dt_f);
+                rank_dt    = std::fmin(rank_dt, cfl_dt);
+            }
+            stats.t_cfl += tl.sec();
+            stats.t_total  = tstep.sec();
+            stats.n_merged = Nm;
+            stats.n_leaf   = tree.L;
+            stats.n_pairs  = neigh.part_cache.ids.size();
+            stats.next_dt  = rank_dt;
         }
     };
 

@@ -101,6 +101,7 @@ SIGNATURES = {
     "shamx_solver_get_sinks": (_i, [_P, C.POINTER(C.c_uint32), _P]),
     "shamx_solver_set_sinks": (_i, [_P, C.c_uint32, _P]),
     "shamx_solver_evolve_once": (_i, [_P, C.POINTER(ShamxStepLog)]),
+    "shamx_solver_density_force": (_i, [_P, C.POINTER(ShamxStepLog)]),
     "shamx_solver_stream": (_i, [_P, C.POINTER(_P)]),
     "shamx_solver_sizes": (_i, [_P, C.POINTER(C.c_uint64)]),
     "shamx_solver_get_f64": (_i, [_P, _i, _P]),

@@ -217,8 +217,11 @@ struct shamx_solver {
         G = g;
     }
 
-    void prestep() { // Solver.cpp:1060-1304
-        u32 hstep = 0, hmax = (u32) cfg.h_max_subcycles;
+    // Solver.cpp:1060-1304.  max_sub subcycles (the configured count in a step); with
+    // require_converged false, a last subcycle that did not converge is kept as it is (h of
+    // the unconverged particles clamped to the cache radius) instead of failing the step.
+    void prestep(u32 max_sub, bool require_converged) {
+        u32 hstep = 0, hmax = max_sub;
         last.h_iterations = 0;
         for (; hstep < hmax; hstep++) {
             int b = timer.begin(stream);
@@ -261,10 +264,10 @@ struct shamx_solver {
             }
             timer.end(stream, PH_HITER, b);
             bool converged = (max_eps < cfg.epsilon_h) && !(min_eps < 0);
-            if (!converged) continue;
+            if (!converged && (require_converged || hstep + 1 < hmax)) continue;
             break;
         }
-        last.subcycles = hstep + 1;
+        last.subcycles = std::min(hstep + 1, hmax);
         if (hstep == hmax) throw std::runtime_error("the h iterator is not converged");
         int b = timer.begin(stream);
         // final h of the images and in the sorted records, then Omega fused with the force
@@ -350,7 +353,7 @@ struct shamx_solver {
             if (cfg.periodic) wrap_positions(stream, N, x.data(), y.data(), z.data(), nullptr_box());
         }
         (void) box6;
-        prestep();
+        prestep((u32) cfg.h_max_subcycles, true);
 
         f64 next_cfl = 0;
         u32 corr_iter = 0;
@@ -477,6 +480,78 @@ struct shamx_solver {
         t         = t_cur + dtc_;
         f64 stiff = cfg.cfl_multiplier_stiffness;
         cfl_mult  = (cfl_mult * stiff + 1.) / (stiff + 1.);
+    }
+
+    // One density and force pass on the current state, with no time integration: the
+    // prestep with a single subcycle (Solver.cpp:1060-1304; particles whose h does not
+    // converge in it keep the value clamped to the cache radius), the EOS, the force loop
+    // with the external sink forces (Solver.cpp:2247-2400, UpdateDerivs.cpp:88-288) and the
+    // CFL reduction (Solver.cpp:3095-3124).  a and du are overwritten, their previous values
+    // saved as in a step; positions, velocities, u and the time are left alone.
+    void density_force() {
+        need(N > 0, "density/force pass on an empty particle set");
+        need(cfg.av_kind == 0, "the density/force pass takes the constant viscosity only");
+        timer.reset();
+        last = shamx_step_log{};
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        auto t0              = std::chrono::steady_clock::now();
+        const bool has_sinks = !sinks.empty();
+        if (has_sinks) {
+            sink_ext_forces();
+            sink_gas_forces();
+        }
+        prestep(1, false);
+        int b = timer.begin(stream);
+        FieldPtrs fp{{h.data(), u.data(), vx.data(), vy.data(), vz.data(), omega.data()}, 6};
+        gather_ghosts(stream, N, G, ghost_src.data(), fp);
+        frec.resize(force_record_bytes(cfg.mixed_precision) * (size_t) (N + G));
+        pack_force(stream, cfg.kernel, N + G, tree.sort_index.data(), x.data(), y.data(),
+                   z.data(), h.data(), vx.data(), vy.data(), vz.data(), u.data(), omega.data(),
+                   P.data(), cs.data(), frec.data(), cfg.gpart_mass,
+                   EosArgs{cfg.eos_kind, cfg.gamma, cfg.eos_cs, cfg.eos_cs0, cfg.eos_q,
+                           cfg.eos_r0},
+                   cfg.mixed_precision);
+        timer.end(stream, PH_EOS, b);
+        b = timer.begin(stream);
+        ForceArgs fa{cfg.gpart_mass,           cfg.alpha_u, cfg.alpha_AV, cfg.beta_AV,
+                     cfg.cfl_cour * cfl_mult, cfg.cfl_force * cfl_mult, cfg.mixed_precision,
+                     false};
+        if (has_sinks) {
+            fa.ext_x = axe.data();
+            fa.ext_y = aye.data();
+            fa.ext_z = aze.data();
+        }
+        compute_force(stream, cfg.kernel, N, csr_force(), frec.data(), x.data(), y.data(),
+                      z.data(), vx.data(), vy.data(), vz.data(), h.data(), omega.data(),
+                      u.data(), P.data(), cs.data(), ax.data(), ay.data(), az.data(), du.data(),
+                      ax_old.data(), ay_old.data(), az_old.data(), du_old.data(), vsig.data(),
+                      dtc.data(), fa, red.data() + 3, red.data() + 7, part);
+        timer.end(stream, PH_FORCE, b);
+        u64 k[5];
+        SHAMX_HIP(hipMemcpyAsync(k, red.data() + 3, 40, hipMemcpyDeviceToHost, stream));
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        auto t1 = std::chrono::steady_clock::now();
+        f64 ph[PH_N] = {0};
+        timer.collect(ph);
+        last.t_total       = std::chrono::duration<f64>(t1 - t0).count();
+        last.rate          = f64(N) / last.t_total;
+        last.t_ghost       = ph[PH_GHOST];
+        last.t_tree        = ph[PH_TREE];
+        last.t_sort        = ph[PH_SORT];
+        last.t_neigh       = ph[PH_NEIGH];
+        last.t_hiter       = ph[PH_HITER];
+        last.t_omega       = ph[PH_OMEGA];
+        last.t_eos         = ph[PH_EOS];
+        last.t_force       = ph[PH_FORCE];
+        last.n_local       = N;
+        last.n_merged      = N + G;
+        last.n_leaf        = tree.L;
+        last.n_pairs       = nb.n_pairs;
+        last.n_force_evals = 1;
+        last.n_interacting = k[4];
+        last.next_dt       = ordered_to_f64(k[0]);
+        last.time          = t;
+        last.dt            = dt;
     }
 
     // periodic box for the predictor wrap, kept in device memory
@@ -1002,6 +1077,14 @@ int shamx_solver_set_sinks(shamx_solver *s, uint32_t count, const double *in17) 
 int shamx_solver_evolve_once(shamx_solver *s, shamx_step_log *log) {
     return guarded([&] {
         s->evolve_once();
+        if (log) *log = s->last;
+    });
+}
+
+int shamx_solver_density_force(shamx_solver *s, shamx_step_log *log) {
+    return guarded([&] {
+        need(s != nullptr, "shamx_solver_density_force: null solver");
+        s->density_force();
         if (log) *log = s->last;
     });
 }

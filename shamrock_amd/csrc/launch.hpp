@@ -93,6 +93,7 @@ namespace shamx {
         DevArray<double4> ps;                           // (x, y, z, h) in sorted order
         DevArray<u32> inv;                              // sorted position of each object
         DevArray<u32> mism;                             // owner leaf != own leaf
+        DevArray<u32> nbatch, bstart, item_leaf;        // stage-2 wave items (member batches)
         u32 n_mismatch = 0;
         DevArray<u32> counters;
         DevArray<unsigned char> scratch;

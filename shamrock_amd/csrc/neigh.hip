@@ -294,10 +294,15 @@ namespace shamx {
 
         constexpr int GRP_WAVES = 4;
 
-        // stage 2, one wave per leaf: the members are the leaf's local particles whose owner
-        // is this leaf; the candidates are the particles of the neighbour leaves, in list order
-        // then sorted order.  Candidates are taken 64 at a time (their sorted index, position
-        // and cut radius staged in LDS); the wave then tests P candidates x M members per
+        // stage 2, one wave per member batch: a batch is 64 consecutive sorted positions of a
+        // leaf (item_leaf / bstart map the wave to its leaf and batch), its members the local
+        // particles among them whose owner is this leaf; the candidates are the particles of
+        // the neighbour leaves, in list order then sorted order.  Candidates are taken 64 at a
+        // time; those farther from the members' bounding box than both the members' largest
+        // cut radius and their own are dropped (they cannot pass the pair test: the box
+        // distance bounds every member distance from below, kept with a 1e-12 margin so that
+        // rounding cannot drop a pair), and the rest are compacted into LDS in order (sorted
+        // index, position, cut radius).  The wave then tests P candidates x M members per
         // instruction, lane = candidate * M + member, so the kept lanes of one member are in
         // candidate order and its row is the compaction of a ballot restricted to its lanes.
         template<bool FILL, bool SORTED, bool STRIDE>
@@ -306,133 +311,171 @@ namespace shamx {
                           const u32 *__restrict sort_index, const double4 *__restrict ps,
                           const u32 *__restrict owner, const u32 *__restrict leaf_cnt,
                           const u32 *__restrict leaf_offset, const u32 *__restrict leaf_ids,
-                          f64 htol, f64 Rker2, u32 *__restrict cnt, u32 *__restrict offset,
-                          u32 *__restrict ids, u32 cap, u32 leaf_cap) {
+                          const u32 *__restrict item_leaf, const u32 *__restrict bstart,
+                          const u32 *__restrict n_items, f64 htol, f64 Rker2,
+                          u32 *__restrict cnt, u32 *__restrict offset, u32 *__restrict ids,
+                          u32 cap, u32 leaf_cap) {
             __shared__ u32 rstart[GRP_WAVES][WAVE], rexcl[GRP_WAVES][WAVE];
             __shared__ double4 cpos[GRP_WAVES][WAVE]; // candidate x, y, z, cut radius^2
             __shared__ u32 cid[GRP_WAVES][WAVE];      // candidate id written to the rows
             __shared__ double4 mpos[GRP_WAVES][WAVE]; // member x, y, z, cut radius^2
             __shared__ u32 mrow[GRP_WAVES][WAVE];     // member row start
             const int lane = __lane_id(), wid = threadIdx.x / WAVE;
-            const u32 l    = blockIdx.x * GRP_WAVES + wid;
-            if (l >= L) return;
+            const u32 item = blockIdx.x * GRP_WAVES + wid;
+            if (item >= *n_items) return;
+            const u32 l = item_leaf[item];
             u32 m0, m1;
             node_children(nodes, I + l, m0, m1);
+            const u32 mb = m0 + (item - bstart[l]) * WAVE;
             const u64 lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
             // leaf_cap (fixed-capacity leaf rows): a row that overflowed holds only leaf_cap
             // entries; the build is repeated then, but nothing may be read past the row
             const u32 k0 = leaf_offset[l],
                       k1 = k0 + (leaf_cap ? min(leaf_cnt[l], leaf_cap) : leaf_cnt[l]);
-            for (u32 mb = m0; mb < m1; mb += WAVE) { // member batches of 64
-                u32 sm   = mb + lane;
-                bool mem = false;
-                u32 a    = 0;
-                double4 pa{0, 0, 0, 0};
-                if (sm < m1) {
-                    a = sort_index[sm];
-                    if (a < n && owner[a] == l) {
-                        mem       = true;
-                        double4 p = ps[sm];
-                        f64 ra    = p.w * htol;
-                        pa        = make_double4(p.x, p.y, p.z, ra * ra * Rker2);
-                    }
+            u32 sm   = mb + lane;
+            bool mem = false;
+            u32 a    = 0;
+            double4 pa{0, 0, 0, 0};
+            if (sm < m1) {
+                a = sort_index[sm];
+                if (a < n && owner[a] == l) {
+                    mem       = true;
+                    double4 p = ps[sm];
+                    f64 ra    = p.w * htol;
+                    pa        = make_double4(p.x, p.y, p.z, ra * ra * Rker2);
                 }
-                const u64 members = __ballot(mem);
-                if (!members) continue;
-                const u32 M    = (u32) __popcll(members);
-                const u32 rank = (u32) __popcll(members & lt); // member index of this lane
-                const u32 ia   = SORTED ? sm : a;
-                const u32 row0 = STRIDE ? (mem ? ia * cap : 0u)
-                                        : ((FILL && mem) ? offset[ia] : 0u);
-                if (mem) {
-                    mpos[wid][rank] = pa;
-                    mrow[wid][rank] = row0;
-                }
-                // lane = c * M + mi: P candidates per pass; lanes past P * M stay idle
-                const u32 P  = WAVE / M;
-                const u32 mi = (u32) lane % M, ci = (u32) lane / M;
-                const bool on = ci < P;
-                u64 mine = 0; // this member's lanes
-                for (u32 c = 0; c < P; c++) mine |= 1ull << (c * M + mi);
-                wave_lds_sync();
-                const double4 me = mpos[wid][on ? mi : 0];
-                const u32 mrow0  = mrow[wid][on ? mi : 0];
-                u32 cursor = mrow0, count = 0;
-                for (u32 kb = k0; kb < k1; kb += WAVE) {
-                    // neighbour leaves in batches of up to 64: lane i loads the range of the
-                    // i-th leaf of the batch; candidates are the concatenated ranges in order
-                    u32 k  = kb + lane;
-                    u32 r0 = 0, r1 = 0;
-                    if (k < k1) node_children(nodes, leaf_ids[k], r0, r1); // leaf: sorted range
-                    u32 sz  = r1 - r0;
-                    u32 inc = sz;
-                    for (int o = 1; o < WAVE; o <<= 1) {
-                        u32 t = __shfl_up(inc, o, WAVE);
-                        if (lane >= o) inc += t;
-                    }
-                    const u32 total = __shfl(inc, WAVE - 1, WAVE);
-                    const u32 nr    = min(k1 - kb, (u32) WAVE);
-                    wave_lds_sync();
-                    rstart[wid][lane] = r0;
-                    rexcl[wid][lane]  = inc - sz;
-                    wave_lds_sync();
-                    for (u32 c0 = 0; c0 < total; c0 += WAVE) {
-                        // stage 64 candidates: lane j takes candidate c0 + j
-                        const u32 cn = min(total - c0, (u32) WAVE);
-                        {
-                            u32 c = c0 + lane;
-                            // the range holding candidate c: largest j with rexcl[j] <= c
-                            // (leaves are never empty, so c - rexcl[j] < that range's size)
-                            u32 j = 0;
-                            for (u32 st = WAVE / 2; st > 0; st >>= 1) {
-                                u32 t = j + st;
-                                if (t < nr && rexcl[wid][t] <= c) j = t;
-                            }
-                            const bool valid = (u32) lane < cn;
-                            u32 sb    = valid ? rstart[wid][j] + (c - rexcl[wid][j]) : 0u;
-                            double4 q = ps[sb];
-                            f64 rb    = q.w * htol;
-                            wave_lds_sync();
-                            cpos[wid][lane] = make_double4(q.x, q.y, q.z, rb * rb * Rker2);
-                            cid[wid][lane]  = FILL && valid ? (SORTED ? sb : sort_index[sb]) : 0u;
-                            wave_lds_sync();
-                        }
-                        for (u32 cb0 = 0; cb0 < cn; cb0 += P) {
-                            const u32 cl = cb0 + ci; // candidate of this lane
-                            bool keep    = false;
-                            if (on && cl < cn) {
-                                const double4 q = cpos[wid][cl];
-                                f64 dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
-                                f64 rab2 = dx * dx + dy * dy + dz * dz;
-                                keep     = !(rab2 > me.w && rab2 > q.w);
-                            }
-                            const u64 km = __ballot(keep);
-                            const u64 kt = km & mine;
-                            if (FILL && keep) {
-                                u32 pos = cursor + (u32) __popcll(kt & lt);
-                                if (STRIDE) {
-                                    if (pos < mrow0 + cap) ids[pos] = cid[wid][cl];
-                                } else {
-                                    ids[pos] = cid[wid][cl];
-                                }
-                            }
-                            const u32 kc = (u32) __popcll(kt);
-                            cursor += kc;
-                            count += kc;
-                        }
-                    }
-                }
-                // lane mi (< M) holds member mi's count; the member lane of rank mi writes it
-                wave_lds_sync();
-                if (on && ci == 0) mrow[wid][mi] = count;
-                wave_lds_sync();
-                if (mem) {
-                    u32 cm = mrow[wid][rank];
-                    if (!FILL || STRIDE) cnt[ia] = cm;
-                    if (STRIDE) offset[ia] = row0;
-                }
-                wave_lds_sync();
             }
+            const u64 members = __ballot(mem);
+            if (!members) return;
+            const u32 M    = (u32) __popcll(members);
+            const u32 rank = (u32) __popcll(members & lt); // member index of this lane
+            const u32 ia   = SORTED ? sm : a;
+            const u32 row0 = STRIDE ? (mem ? ia * cap : 0u) : ((FILL && mem) ? offset[ia] : 0u);
+            if (mem) {
+                mpos[wid][rank] = pa;
+                mrow[wid][rank] = row0;
+            }
+            // the members' bounding box and largest cut radius^2
+            constexpr f64 BIG = 1.7976931348623157e308;
+            auto fmn = [](f64 p, f64 q) { return fmin(p, q); };
+            auto fmx = [](f64 p, f64 q) { return fmax(p, q); };
+            const f64 blx = wave_reduce(mem ? pa.x : BIG, fmn), bhx = wave_reduce(mem ? pa.x : -BIG, fmx);
+            const f64 bly = wave_reduce(mem ? pa.y : BIG, fmn), bhy = wave_reduce(mem ? pa.y : -BIG, fmx);
+            const f64 blz = wave_reduce(mem ? pa.z : BIG, fmn), bhz = wave_reduce(mem ? pa.z : -BIG, fmx);
+            const f64 br2 = wave_reduce(mem ? pa.w : 0.0, fmx);
+            // lane = c * M + mi: P candidates per pass; lanes past P * M stay idle
+            const u32 P  = WAVE / M;
+            const u32 mi = (u32) lane % M, ci = (u32) lane / M;
+            const bool on = ci < P;
+            u64 mine = 0; // this member's lanes
+            for (u32 c = 0; c < P; c++) mine |= 1ull << (c * M + mi);
+            wave_lds_sync();
+            const double4 me = mpos[wid][on ? mi : 0];
+            const u32 mrow0  = mrow[wid][on ? mi : 0];
+            u32 cursor = mrow0, count = 0;
+            for (u32 kb = k0; kb < k1; kb += WAVE) {
+                // neighbour leaves in batches of up to 64: lane i loads the range of the i-th
+                // leaf of the batch; candidates are the concatenated ranges in order
+                u32 k  = kb + lane;
+                u32 r0 = 0, r1 = 0;
+                if (k < k1) node_children(nodes, leaf_ids[k], r0, r1); // leaf: sorted range
+                u32 sz  = r1 - r0;
+                u32 inc = sz;
+                for (int o = 1; o < WAVE; o <<= 1) {
+                    u32 t = __shfl_up(inc, o, WAVE);
+                    if (lane >= o) inc += t;
+                }
+                const u32 total = __shfl(inc, WAVE - 1, WAVE);
+                const u32 nr    = min(k1 - kb, (u32) WAVE);
+                wave_lds_sync();
+                rstart[wid][lane] = r0;
+                rexcl[wid][lane]  = inc - sz;
+                wave_lds_sync();
+                for (u32 c0 = 0; c0 < total; c0 += WAVE) {
+                    // stage 64 candidates: lane j takes candidate c0 + j; the ones that can
+                    // reach a member are compacted to the front of cpos / cid
+                    u32 cn;
+                    {
+                        const u32 c = c0 + lane;
+                        // the range holding candidate c: largest j with rexcl[j] <= c
+                        // (leaves are never empty, so c - rexcl[j] < that range's size)
+                        u32 j = 0;
+                        for (u32 st = WAVE / 2; st > 0; st >>= 1) {
+                            u32 t = j + st;
+                            if (t < nr && rexcl[wid][t] <= c) j = t;
+                        }
+                        const bool valid = c < total;
+                        u32 sb    = valid ? rstart[wid][j] + (c - rexcl[wid][j]) : 0u;
+                        double4 q = ps[sb];
+                        f64 rb    = q.w * htol;
+                        f64 rb2   = rb * rb * Rker2;
+                        f64 ex = q.x < blx ? blx - q.x : (q.x > bhx ? q.x - bhx : 0.0);
+                        f64 ey = q.y < bly ? bly - q.y : (q.y > bhy ? q.y - bhy : 0.0);
+                        f64 ez = q.z < blz ? blz - q.z : (q.z > bhz ? q.z - bhz : 0.0);
+                        f64 d2 = (ex * ex + ey * ey + ez * ez) * (1.0 - 1e-12);
+                        const bool near = valid && !(d2 > br2 && d2 > rb2);
+                        const u64 nm    = __ballot(near);
+                        const u32 pos   = (u32) __popcll(nm & lt);
+                        cn              = (u32) __popcll(nm);
+                        wave_lds_sync();
+                        if (near) {
+                            cpos[wid][pos] = make_double4(q.x, q.y, q.z, rb2);
+                            cid[wid][pos]  = FILL ? (SORTED ? sb : sort_index[sb]) : 0u;
+                        }
+                        wave_lds_sync();
+                    }
+                    for (u32 cb0 = 0; cb0 < cn; cb0 += P) {
+                        const u32 cl = cb0 + ci; // candidate of this lane
+                        bool keep    = false;
+                        if (on && cl < cn) {
+                            const double4 q = cpos[wid][cl];
+                            f64 dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+                            f64 rab2 = dx * dx + dy * dy + dz * dz;
+                            keep     = !(rab2 > me.w && rab2 > q.w);
+                        }
+                        const u64 km = __ballot(keep);
+                        const u64 kt = km & mine;
+                        if (FILL && keep) {
+                            u32 pos = cursor + (u32) __popcll(kt & lt);
+                            if (STRIDE) {
+                                if (pos < mrow0 + cap) ids[pos] = cid[wid][cl];
+                            } else {
+                                ids[pos] = cid[wid][cl];
+                            }
+                        }
+                        const u32 kc = (u32) __popcll(kt);
+                        cursor += kc;
+                        count += kc;
+                    }
+                }
+            }
+            // lane mi (< M) holds member mi's count; the member lane of rank mi writes it
+            wave_lds_sync();
+            if (on && ci == 0) mrow[wid][mi] = count;
+            wave_lds_sync();
+            if (mem) {
+                u32 cm = mrow[wid][rank];
+                if (!FILL || STRIDE) cnt[ia] = cm;
+                if (STRIDE) offset[ia] = row0;
+            }
+        }
+
+        // member batches: nb[l] = batches of 64 sorted positions in leaf l (>= 1)
+        __global__ void k_leaf_batches(u32 L, u32 I, const f64 *__restrict nodes,
+                                       u32 *__restrict nbatch) {
+            u32 l = blockIdx.x * blockDim.x + threadIdx.x;
+            if (l >= L) return;
+            u32 m0, m1;
+            node_children(nodes, I + l, m0, m1);
+            nbatch[l] = max(1u, (m1 - m0 + WAVE - 1) / WAVE);
+        }
+        // item -> leaf for the batches of every leaf (bstart: exclusive scan of nbatch)
+        __global__ void k_item_leaf(u32 L, const u32 *__restrict nbatch,
+                                    const u32 *__restrict bstart, u32 *__restrict item_leaf) {
+            u32 l = blockIdx.x * blockDim.x + threadIdx.x;
+            if (l >= L) return;
+            const u32 b0 = bstart[l], nbl = nbatch[l];
+            for (u32 j = 0; j < nbl; j++) item_leaf[b0 + j] = l;
         }
     } // namespace
 
@@ -456,6 +499,23 @@ namespace shamx {
             if (n) k_max_u32<<<std::min<u32>(div_up(n, 256), 1024), 256, 0, s>>>(n, v, out);
         }
 
+        // the stage-2 wave items (one per 64 sorted positions of every leaf): item -> leaf map,
+        // first item of every leaf, item count at counters[7]; returns the launch's block
+        // count, sized for the bound L + nm / 64 (waves past the count return at once)
+        u32 batch_items(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t) {
+            const u32 L = t.L, bound = L + div_up(t.n_obj, (u32) WAVE);
+            nb.nbatch.resize(L);
+            nb.bstart.resize(L);
+            nb.item_leaf.resize(bound);
+            k_leaf_batches<<<div_up(L, 256), 256, 0, s>>>(L, t.I, t.nodes.data(),
+                                                           nb.nbatch.data());
+            exclusive_scan_u32(s, nb.nbatch.data(), nb.bstart.data(), L, nb.counters.data() + 7,
+                               nb.scratch.data());
+            k_item_leaf<<<div_up(L, 256), 256, 0, s>>>(L, nb.nbatch.data(), nb.bstart.data(),
+                                                        nb.item_leaf.data());
+            return div_up(bound, (u32) GRP_WAVES);
+        }
+
         // packed rows, two passes (count, scan, fill); SORTED: rows by sorted position
         template<bool SORTED>
         void stage2_packed(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
@@ -464,12 +524,14 @@ namespace shamx {
             u32 L = t.L, I = t.I;
             u32 rows      = SORTED ? t.n_obj : n_local;
             u32 *mism_len = nb.counters.data() + 5;
-            u32 gblocks   = div_up(L, GRP_WAVES);
+            u32 gblocks   = batch_items(s, nb, t);
+            const u32 *n_items = nb.counters.data() + 7;
             if (SORTED) fill_u32(s, nb.cnt.data(), 0, rows); // ghost rows stay empty
             k_group_neigh<false, SORTED, false><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
-                htol, Rker2, nb.cnt.data(), nullptr, nullptr, 0, 0);
+                nb.item_leaf.data(), nb.bstart.data(), n_items, htol, Rker2, nb.cnt.data(),
+                nullptr, nullptr, 0, 0);
             k_part_neigh<false, SORTED, false><<<div_up(n_local, 128), 128, 0, s>>>(
                 n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
@@ -488,7 +550,8 @@ namespace shamx {
             k_group_neigh<true, SORTED, false><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
-                htol, Rker2, nullptr, nb.offset.data(), nb.ids.data(), 0, 0);
+                nb.item_leaf.data(), nb.bstart.data(), n_items, htol, Rker2, nullptr,
+                nb.offset.data(), nb.ids.data(), 0, 0);
             k_part_neigh<true, SORTED, false><<<div_up(n_local, 128), 128, 0, s>>>(
                 n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
@@ -591,6 +654,7 @@ namespace shamx {
         // both stages in one pass each, rows of fixed capacity.  The capacities come from the
         // previous build; on overflow the counts (exact, they run past the capacity) size a
         // rebuild.  Stage 2 counts are only meaningful once stage 1 fitted.
+        const u32 gblocks = batch_items(s, nb, t);
         if (nb.cap_leaf == 0) nb.cap_leaf = 128;
         if (nb.cap_part == 0) nb.cap_part = 256;
         u32 *mism_len = nb.counters.data() + 5;
@@ -626,10 +690,11 @@ namespace shamx {
                 nb.leaf_ids.data(), cl, err, lpw);
             max_u32(s, L, nb.leaf_cnt.data(), nb.counters.data() + 8);
             fill_u32(s, nb.cnt.data(), 0, nm); // ghost rows stay empty
-            k_group_neigh<true, true, true><<<div_up(L, GRP_WAVES), GRP_WAVES * WAVE, 0, s>>>(
+            k_group_neigh<true, true, true><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
-                htol, Rker2, nb.cnt.data(), nb.offset.data(), nb.ids.data(), cp, cl);
+                nb.item_leaf.data(), nb.bstart.data(), nb.counters.data() + 7, htol, Rker2,
+                nb.cnt.data(), nb.offset.data(), nb.ids.data(), cp, cl);
             k_part_neigh<true, true, true><<<div_up(n_local, 128), 128, 0, s>>>(
                 n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,

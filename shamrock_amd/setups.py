@@ -130,15 +130,17 @@ def plummer_positions(n: int, seed: int = 0, a: float = 1.0, M: float = 1.0,
 
 
 def plummer(n: int, seed: int = 0, a: float = 1.0, M: float = 1.0, r_max: float = 30.0,
-            u: float = 0.05, kernel: str = "M4") -> tuple[dict, dict]:
+            u: float = 0.05, kernel: str = "M4", pos_h=None) -> tuple[dict, dict]:
     """Plummer sphere (scale radius a, mass M) truncated at r_max, free boundaries, zero
-    velocities, adiabatic gamma = 5/3 with u = 0.05.  Radii by the inverse of the enclosed-mass
-    fraction r^3 / (r^2 + a^2)^(3/2), drawn inside the truncation; isotropic directions; h from
-    the analytic density, h = hfact (m / rho)^(1/3)."""
-    pos, h = plummer_positions(n, seed, a, M, r_max, kernel)
+    velocities, isothermal gas at u = 0.05 (EOS isothermal, c_s^2 = (gamma - 1) u with
+    gamma = 5/3).  Radii by the inverse of the enclosed-mass fraction r^3 / (r^2 + a^2)^(3/2),
+    drawn inside the truncation; isotropic directions; h from the analytic density,
+    h = hfact (m / rho)^(1/3).  pos_h: positions and h of plummer_positions already drawn."""
+    pos, h = plummer_positions(n, seed, a, M, r_max, kernel) if pos_h is None else pos_h
     m = M / n
     lo, hi = pos.min(0) - 1, pos.max(0) + 1
-    cfg = base_config(kernel, lo, hi, periodic=False, gpart_mass=m)
+    cfg = base_config(kernel, lo, hi, periodic=False, gpart_mass=m, eos_kind=1,
+                      eos_cs=math.sqrt((5.0 / 3.0 - 1.0) * u))
     return cfg, fields_from(pos, h, u)
 
 

@@ -149,6 +149,14 @@ class GpuSolver:
         check(lib().shamx_solver_evolve_once(self.h, C.byref(log)))
         return log.as_dict()
 
+    def density_force(self) -> dict:
+        """One density and force pass without time integration: tree, neighbour cache, one
+        smoothing-length subcycle (unconverged h kept clamped), Omega, EOS, forces and the CFL
+        step.  Constant viscosity only."""
+        log = _lib.ShamxStepLog()
+        check(lib().shamx_solver_density_force(self.h, C.byref(log)))
+        return log.as_dict()
+
     def stream(self) -> int:
         s = C.c_void_p()
         check(lib().shamx_solver_stream(self.h, C.byref(s)))

@@ -116,6 +116,7 @@ def lib():
         L.orc_solver_add_sink.argtypes = [C.c_void_p, C.c_int, _f64p]
         L.orc_solver_get_sinks.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.c_void_p]
         L.orc_solver_evolve_once.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats)]
+        L.orc_solver_density_force.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats)]
         L.orc_solver_sizes.argtypes = [C.c_void_p, C.c_int, _u64p]
         L.orc_solver_get_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p]
         L.orc_solver_get_u32.argtypes = [C.c_void_p, C.c_int, C.c_int, _u32p]
@@ -432,6 +433,11 @@ class Solver:
     def evolve_once(self):
         st = StepStats()
         _chk(lib().orc_solver_evolve_once(self.h, self.kernel, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in StepStats._fields_}
+
+    def density_force(self):
+        st = StepStats()
+        _chk(lib().orc_solver_density_force(self.h, self.kernel, C.byref(st)))
         return {k: getattr(st, k) for k, _ in StepStats._fields_}
 
     def sizes(self):

@@ -339,11 +339,20 @@ def _compare_transient(g, o, exact_sort=True):
     np.testing.assert_array_equal(g["tree_bbox"], o["tree_bbox"])
     np.testing.assert_array_equal(g["aabb_min"], o["aabb_min"])
     np.testing.assert_array_equal(g["aabb_max"], o["aabb_max"])
-    np.testing.assert_array_equal(g["rint_tree"], o["rint_tree"])
+    if exact_sort:
+        np.testing.assert_array_equal(g["rint_tree"], o["rint_tree"])
+    else:
+        # h from sums whose terms came in another order within runs of equal codes
+        assert relerr(g["rint_tree"], o["rint_tree"]) <= RTOL
     if exact_sort:
         np.testing.assert_array_equal(g["sort_index"], o["sort_index"])
         for k in ("part_cnt", "part_offset", "part_ids"):
             np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+    else:
+        # the oracle's bitonic sort (the reference's) orders equal Morton codes differently
+        # from the stable radix sort: same permutation inside every run of equal codes
+        assert _same_modulo_ties(g["sorted_codes"], g["sort_index"], o["sort_index"])
+        np.testing.assert_array_equal(g["part_cnt"], o["part_cnt"])
     for k in ("mx", "my", "mz"):
         np.testing.assert_array_equal(g[k], o[k], err_msg=k)
     for k in ("mh", "omega", "P", "cs", "momega", "vsig"):
@@ -452,6 +461,37 @@ def test_full_step_parity_sinks(periodic):
         np.testing.assert_allclose(sg["sph_acceleration"], so["sph_acceleration"], rtol=0,
                                    atol=1e-12 * terms.sum(), err_msg="sph_acceleration")
         assert g.time()["dt"] == pytest.approx(o.time()["dt"], rel=RTOL)
+    g.close()
+
+
+@pytest.mark.parametrize("workload", ["plummer_6000", "disc_8000"])
+def test_density_force_pass_parity(workload):
+    """The density/force pass (tree, cache, one h subcycle, Omega, EOS, forces, CFL) vs the
+    oracle's.  Plummer truncated at r = 30: h over three decades and a subcycle that does not
+    converge everywhere (the clamped particles must match too); disc: the sink's external
+    force on the gas.  Run twice, the second pass on the first one's h."""
+    name, n = workload.rsplit("_", 1)
+    if name == "plummer":
+        cfg, f = setups.plummer(int(n), seed=1, r_max=30.0)
+    else:
+        cfg, f = setups.disc(int(n), seed=2)
+    g = GpuSolver(cfg, f)
+    o = orc.Solver(cfg, f)
+    for rep in range(2):
+        say(f"[density_force] {workload} pass {rep}")
+        lg = g.density_force()
+        lo = o.density_force()
+        assert lg["subcycles"] == lo["subcycles"] == 1
+        assert lg["h_iterations"] == lo["h_iterations"]
+        assert lg["n_pairs"] == lo["n_pairs"]
+        # Plummer at r = 30: the 10-bit cells of the 60-wide box hold several particles
+        _compare_transient(g.transient(), o.transient(), exact_sort=(name != "plummer"))
+        for k in ("h", "ax", "ay", "az", "du"):
+            assert relerr(g.get(k), o.get(k)) <= RTOL, (k, relerr(g.get(k), o.get(k)))
+        for k in ("x", "vx", "u"):
+            np.testing.assert_array_equal(g.get(k), f[k])
+        assert lg["next_dt"] == pytest.approx(lo["next_dt"], rel=RTOL)
+        assert g.time() == o.time()
     g.close()
 
 
