@@ -256,7 +256,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 512 MiB device copy (counter calibration)")
     ap.add_argument("--no-scale-probe", action="store_true",
-                    help="skip the sort / tree timing at 16,777,216 clustered particles")
+                    help="skip the clustered workload (16,777,216 Plummer particles)")
+    ap.add_argument("--clustered-only", action="store_true",
+                    help="run only the clustered workload and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU design "
@@ -267,6 +269,9 @@ def main():
     from shamrock_amd import _lib, setups
 
     _lib.lib()
+    if args.clustered_only:
+        print(json.dumps(scale_probe()))
+        return
     cfg, fields = setups.sedov(args.n_side)
     n = fields["x"].size
     workload = f"sedov_{args.n_side}^3_M6_periodic"

@@ -538,7 +538,11 @@ namespace shamx {
                 y, z, h, nb.inv.data(), htol, Rker2, nb.cnt.data(), nullptr, nullptr, 0, 0);
             exclusive_scan_u32(s, nb.cnt.data(), nb.offset.data(), rows, nb.counters.data() + 1,
                                nb.scratch.data());
-            SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
+            // longest leaf list and row (counters[8], [9])
+            fill_u32(s, nb.counters.data() + 8, 0, 2);
+            max_u32(s, L, nb.leaf_cnt.data(), nb.counters.data() + 8);
+            max_u32(s, rows, nb.cnt.data(), nb.counters.data() + 9);
+            SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
                                      hipMemcpyDeviceToHost, s));
             SHAMX_HIP(hipStreamSynchronize(s));
             if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
@@ -655,8 +659,14 @@ namespace shamx {
         // previous build; on overflow the counts (exact, they run past the capacity) size a
         // rebuild.  Stage 2 counts are only meaningful once stage 1 fitted.
         const u32 gblocks = batch_items(s, nb, t);
-        if (nb.cap_leaf == 0) nb.cap_leaf = 128;
-        if (nb.cap_part == 0) nb.cap_part = 256;
+        // first capacities: 128 leaves and 256 entries per row, or the largest multiple of 16
+        // that u32 entry indices can address (a row needing more switches to packed rows)
+        auto fit = [](u32 rows_, u32 cap) {
+            u64 lim = 0xffffffffull / std::max<u32>(rows_, 1);
+            return (u32) std::max<u64>(16, std::min<u64>(cap, lim) & ~15ull);
+        };
+        if (nb.cap_leaf == 0) nb.cap_leaf = fit(L, 128);
+        if (nb.cap_part == 0) nb.cap_part = fit(nm, 256);
         u32 *mism_len = nb.counters.data() + 5;
         for (int attempt = 0;; attempt++) {
             if (attempt == 4) throw std::runtime_error("neighbour cache: capacities do not settle");
@@ -679,6 +689,10 @@ namespace shamx {
                 stage2_packed<true>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
                 nb.leaf_stride = 0;
                 nb.row_stride  = 0;
+                // the measured maxima size the next build, which takes fixed-capacity rows
+                // again if they fit
+                nb.cap_leaf = grow_cap(hc[8]);
+                nb.cap_part = grow_cap(hc[9]);
                 break;
             }
             nb.leaf_ids.resize(std::max<u64>((u64) L * cl, 1));
