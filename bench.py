@@ -201,6 +201,53 @@ def plummer_pass(n, pos, hh, reps=3):
     return res
 
 
+def differentiable_kh64(steps=10, checkpoint_every=5):
+    """The differentiable Kelvin-Helmholtz workload: 64^3 (262,444 particles), M4, 10 FP64 steps,
+    a checkpoint every 5; loss = mean squared difference of the final density from a target
+    made by the plain method from a phase-shifted perturbation; wall time of the forward
+    rollout and of the backward pass (replays + adjoint), gradient w.r.t. the initial
+    velocities.  Start perturbed like the adjoint tests (u by 5 %, v by 1e-3) so one corrector
+    pass per step holds at CFL 0.1."""
+    import time
+
+    import torch
+
+    from shamrock_amd import setups
+    from shamrock_amd.adjoint import rollout_density, rollout_dts
+
+    cfg, f = setups.kelvin_helmholtz(64)
+    cfg = dict(cfg, epsilon_h=1e-12, cfl_cour=0.1, cfl_force=0.1)
+    rng = np.random.default_rng(11)
+    v = np.stack([f["vx"], f["vy"], f["vz"]], 1)
+    v0 = torch.tensor(v + 1e-3 * rng.standard_normal(v.shape), device="cuda")
+    u0 = torch.tensor(f["u"] * (1 + 0.05 * rng.standard_normal(f["u"].size)), device="cuda")
+    dts = rollout_dts(cfg, f, v0, u0, steps)
+    _, ft = setups.kelvin_helmholtz(64, phase=0.1)
+    vt = torch.tensor(np.stack([ft["vx"], ft["vy"], ft["vz"]], 1), device="cuda")
+    with torch.no_grad():
+        target = rollout_density(cfg, f, vt, u0, steps, checkpoint_every=checkpoint_every,
+                                 dts=dts)
+    vv = v0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rho = rollout_density(cfg, f, vv, u0, steps, checkpoint_every=checkpoint_every, dts=dts)
+    loss = ((rho - target) ** 2).mean()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    n = v0.shape[0]
+    log(f"[bench] differentiable KH 64^3 N={n}: forward {t1 - t0:.3f} s, backward "
+        f"{t2 - t1:.3f} s")
+    return {"workload": "kelvin_helmholtz_64^3_M4_periodic", "n_particles": n, "steps": steps,
+            "checkpoint_every": checkpoint_every, "dtype": "f64", "loss": loss.item(),
+            "forward_s": t1 - t0, "backward_s": t2 - t1,
+            "grad_norm": float(vv.grad.norm().item()),
+            "check": "central differences on 64 random components: "
+                     "tests/test_gpu_adjoint.py::test_adjoint_kh64_sixty_four_components"}
+
+
 def load_traffic(workload):
     """HBM bytes per launch measured by rocprofv3 (profiles/traffic.json), for this workload."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -257,6 +304,8 @@ def main():
                     help="also run one 512 MiB device copy (counter calibration)")
     ap.add_argument("--no-scale-probe", action="store_true",
                     help="skip the clustered workload (16,777,216 Plummer particles)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiable Kelvin-Helmholtz workload (64^3)")
     ap.add_argument("--clustered-only", action="store_true",
                     help="run only the clustered workload and print its JSON (profiling)")
     args = ap.parse_args()
@@ -341,6 +390,8 @@ def main():
         "kernels": kernels,
     }
     g64.close()
+    if not args.no_adjoint:
+        out["differentiable_kh64"] = differentiable_kh64()
     if not args.no_scale_probe:
         probe = scale_probe()
         out["plummer_16M_pass"] = probe.pop("pass_plummer_16M")

@@ -94,6 +94,9 @@ namespace shamx {
         DevArray<u32> inv;                              // sorted position of each object
         DevArray<u32> mism;                             // owner leaf != own leaf
         DevArray<u32> nbatch, bstart, item_leaf;        // stage-2 wave items (member batches)
+        // rows past their fixed capacity: flags, counts, positions, list (overflow region)
+        DevArray<u32> oflag, ocnt, ofpos, opos, olist;
+        bool leaf_overflowed = false; // the last build had leaf lists past their capacity
         u32 n_mismatch = 0;
         DevArray<u32> counters;
         DevArray<unsigned char> scratch;

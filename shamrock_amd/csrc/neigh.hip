@@ -14,6 +14,8 @@
 #include "tree_layout.hpp"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 namespace shamx {
 
@@ -128,16 +130,19 @@ namespace shamx {
         // stage 1 count / fill, one lane per leaf, 64 consecutive leaves per wave.
         // STRIDE: single pass, leaf g owns ids[g * cap, (g + 1) * cap); the count goes on past
         // cap (entries beyond it are dropped) so the caller can detect the overflow.
-        template<bool FILL, bool STRIDE>
+        // LIST: fill only the n_list leaves of list (overflowed ones), each at its offset[g].
+        template<bool FILL, bool STRIDE, bool LIST = false>
         __global__ void __launch_bounds__(256)
             k_leaf_neigh_w(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
                            u32 *__restrict cnt, u32 *__restrict offset, u32 *__restrict ids,
-                           u32 cap, u32 *__restrict err, u32 lpw) {
+                           u32 cap, u32 *__restrict err, u32 lpw,
+                           const u32 *__restrict list = nullptr, u32 n_list = 0) {
             // lpw leaves per wave (64, 32 or 16): fewer leaves give more, shorter walks
             const u32 lane = (u32) __lane_id();
-            u32 g      = (blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * lpw + lane;
-            bool valid = lane < lpw && g < L;
-            u64 live   = __ballot(valid);
+            const u32 j = (blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * lpw + lane;
+            bool valid  = lane < lpw && j < (LIST ? n_list : L);
+            u32 g       = LIST ? (valid ? list[j] : 0u) : j;
+            u64 live    = __ballot(valid);
             if (!live) return;
             const f64 *ra = nodes + (u64) (I + (valid ? g : 0u)) * NODE_F64;
             f64 rint_a    = ra[6] * Rkern;
@@ -163,6 +168,27 @@ namespace shamx {
             if (!ok && __lane_id() == __builtin_ctzll(live)) atomicOr(err, 1u);
             if ((!FILL || STRIDE) && valid) cnt[g] = c - c0;
             if (STRIDE && valid) offset[g] = c0;
+        }
+
+        // rows (leaf lists or particle rows) longer than their fixed capacity: flag and count
+        __global__ void k_over_mark(u32 n, const u32 *__restrict cnt, u32 cap,
+                                    u32 *__restrict flag, u32 *__restrict ocnt) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            const bool o = cnt[i] > cap;
+            flag[i]      = o ? 1u : 0u;
+            ocnt[i]      = o ? cnt[i] : 0u;
+        }
+        // the flagged rows in order into list (as map[i] when map is given), their offsets in
+        // the overflow region after the fixed-capacity rows
+        __global__ void k_over_list(u32 n, const u32 *__restrict flag, const u32 *__restrict fpos,
+                                    const u32 *__restrict opos, u32 base,
+                                    const u32 *__restrict map, u32 *__restrict list,
+                                    u32 *__restrict offset) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n || !flag[i]) return;
+            list[fpos[i]] = map ? map[i] : i;
+            offset[i]     = base + opos[i];
         }
 
         // owner leaf, lanes over sorted positions (spatially coherent waves)
@@ -305,7 +331,10 @@ namespace shamx {
         // index, position, cut radius).  The wave then tests P candidates x M members per
         // instruction, lane = candidate * M + member, so the kept lanes of one member are in
         // candidate order and its row is the compaction of a ballot restricted to its lanes.
-        template<bool FILL, bool SORTED, bool STRIDE>
+        // LIST: one wave per particle of list (item_leaf holds particle ids, n_items their
+        // count): its whole row at offset[its sorted position], the path of rows longer than
+        // their fixed capacity.
+        template<bool FILL, bool SORTED, bool STRIDE, bool LIST = false>
         __global__ void __launch_bounds__(GRP_WAVES * WAVE)
             k_group_neigh(u32 n, u32 L, u32 I, const f64 *__restrict nodes,
                           const u32 *__restrict sort_index, const double4 *__restrict ps,
@@ -314,7 +343,7 @@ namespace shamx {
                           const u32 *__restrict item_leaf, const u32 *__restrict bstart,
                           const u32 *__restrict n_items, f64 htol, f64 Rker2,
                           u32 *__restrict cnt, u32 *__restrict offset, u32 *__restrict ids,
-                          u32 cap, u32 leaf_cap) {
+                          u32 cap, u32 leaf_cap, const u32 *__restrict inv = nullptr) {
             __shared__ u32 rstart[GRP_WAVES][WAVE], rexcl[GRP_WAVES][WAVE];
             __shared__ double4 cpos[GRP_WAVES][WAVE]; // candidate x, y, z, cut radius^2
             __shared__ u32 cid[GRP_WAVES][WAVE];      // candidate id written to the rows
@@ -323,10 +352,18 @@ namespace shamx {
             const int lane = __lane_id(), wid = threadIdx.x / WAVE;
             const u32 item = blockIdx.x * GRP_WAVES + wid;
             if (item >= *n_items) return;
-            const u32 l = item_leaf[item];
-            u32 m0, m1;
-            node_children(nodes, I + l, m0, m1);
-            const u32 mb = m0 + (item - bstart[l]) * WAVE;
+            u32 l, mb, m1;
+            if constexpr (LIST) { // the batch is the listed particle alone (lane 0)
+                const u32 a0 = item_leaf[item];
+                l            = owner[a0];
+                mb           = inv[a0];
+                m1           = mb + 1;
+            } else {
+                l = item_leaf[item];
+                u32 m0;
+                node_children(nodes, I + l, m0, m1);
+                mb = m0 + (item - bstart[l]) * WAVE;
+            }
             const u64 lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
             // leaf_cap (fixed-capacity leaf rows): a row that overflowed holds only leaf_cap
             // entries; the build is repeated then, but nothing may be read past the row
@@ -517,10 +554,11 @@ namespace shamx {
         }
 
         // packed rows, two passes (count, scan, fill); SORTED: rows by sorted position
+        // (leaf_cap: capacity of fixed-capacity leaf lists, 0 for packed ones)
         template<bool SORTED>
         void stage2_packed(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
                            const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
-                           f64 Rker2, u32 *hc) {
+                           f64 Rker2, u32 *hc, u32 leaf_cap = 0) {
             u32 L = t.L, I = t.I;
             u32 rows      = SORTED ? t.n_obj : n_local;
             u32 *mism_len = nb.counters.data() + 5;
@@ -531,11 +569,12 @@ namespace shamx {
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
                 nb.item_leaf.data(), nb.bstart.data(), n_items, htol, Rker2, nb.cnt.data(),
-                nullptr, nullptr, 0, 0);
+                nullptr, nullptr, 0, leaf_cap);
             k_part_neigh<false, SORTED, false><<<div_up(n_local, 128), 128, 0, s>>>(
                 n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
-                y, z, h, nb.inv.data(), htol, Rker2, nb.cnt.data(), nullptr, nullptr, 0, 0);
+                y, z, h, nb.inv.data(), htol, Rker2, nb.cnt.data(), nullptr, nullptr, 0,
+                leaf_cap);
             exclusive_scan_u32(s, nb.cnt.data(), nb.offset.data(), rows, nb.counters.data() + 1,
                                nb.scratch.data());
             // longest leaf list and row (counters[8], [9])
@@ -555,11 +594,12 @@ namespace shamx {
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
                 nb.item_leaf.data(), nb.bstart.data(), n_items, htol, Rker2, nullptr,
-                nb.offset.data(), nb.ids.data(), 0, 0);
+                nb.offset.data(), nb.ids.data(), 0, leaf_cap);
             k_part_neigh<true, SORTED, false><<<div_up(n_local, 128), 128, 0, s>>>(
                 n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
-                y, z, h, nb.inv.data(), htol, Rker2, nullptr, nb.offset.data(), nb.ids.data(), 0, 0);
+                y, z, h, nb.inv.data(), htol, Rker2, nullptr, nb.offset.data(), nb.ids.data(), 0,
+                leaf_cap);
         }
 
         // leaves per wave in the stage-1 walk (SHAMX_LEAF_LANES: 64, 32 or 16)
@@ -570,6 +610,13 @@ namespace shamx {
                 return (u32) ((x == 16 || x == 32) ? x : 64);
             }();
             return v;
+        }
+
+        // SHAMX_NEIGH_ROWS=packed (read at every build) takes packed particle rows even where
+        // fixed-capacity rows fit, so that path can be tested at small sizes
+        bool packed_rows_forced() {
+            const char *e = std::getenv("SHAMX_NEIGH_ROWS");
+            return e && std::strcmp(e, "packed") == 0;
         }
 
         // row capacity for a measured maximum: 1/8 headroom, rounded up to 16 entries so that
@@ -655,24 +702,62 @@ namespace shamx {
         k_local_scatter<<<div_up(nm, 256), 256, 0, s>>>(nm, nb.lflag.data(), nb.lscan.data(),
                                                         nb.lpos.data());
 
-        // both stages in one pass each, rows of fixed capacity.  The capacities come from the
-        // previous build; on overflow the counts (exact, they run past the capacity) size a
-        // rebuild.  Stage 2 counts are only meaningful once stage 1 fitted.
+        // Both stages in one pass each, rows of fixed capacity, chosen from the previous build
+        // (about four times the mean length, at most the longest row plus headroom).  Rows
+        // longer than their capacity (the counts run on past it) are listed and filled again,
+        // whole, into an overflow region after the fixed-capacity rows, their offsets pointing
+        // there: consumers read (offset, count) and see one layout.  Leaf lists that overflow
+        // must be complete before stage 2 reads them: once a build has seen leaf overflow,
+        // the next ones resolve it between the stages (one more host sync); the first time, the
+        // build is repeated.
         const u32 gblocks = batch_items(s, nb, t);
-        // first capacities: 128 leaves and 256 entries per row, or the largest multiple of 16
-        // that u32 entry indices can address (a row needing more switches to packed rows)
+        // capacity for rows of measured longest `mx` and total `tot` over `rows_` rows, at most
+        // what u32 entry indices address for rows_ rows (a multiple of 16: 64-byte rows)
         auto fit = [](u32 rows_, u32 cap) {
             u64 lim = 0xffffffffull / std::max<u32>(rows_, 1);
             return (u32) std::max<u64>(16, std::min<u64>(cap, lim) & ~15ull);
         };
+        auto choose = [&](u32 mx, u64 tot, u32 used_rows, u32 rows_) {
+            u64 mean4 = (4 * div_up(tot, (u64) std::max<u32>(used_rows, 1)) + 15) & ~15ull;
+            return fit(rows_, (u32) std::min<u64>(grow_cap(mx), std::max<u64>(64, mean4)));
+        };
         if (nb.cap_leaf == 0) nb.cap_leaf = fit(L, 128);
         if (nb.cap_part == 0) nb.cap_part = fit(nm, 256);
+        if (const char *e = std::getenv("SHAMX_NEIGH_CAP")) { // tests: force overflow
+            nb.cap_leaf = nb.cap_part = std::max(16, std::atoi(e)) & ~15;
+        }
         u32 *mism_len = nb.counters.data() + 5;
+        const u32 big = std::max(L, nm);
+        for (auto *a_ : {&nb.oflag, &nb.ocnt, &nb.ofpos, &nb.opos, &nb.olist}) a_->resize(big);
+        // list the rows of cnt[0, n) longer than cap: count at counters[c], total at c + 1
+        auto mark = [&](u32 n, const u32 *cnt, u32 cap, int c) {
+            k_over_mark<<<div_up(n, 256), 256, 0, s>>>(n, cnt, cap, nb.oflag.data(),
+                                                        nb.ocnt.data());
+            exclusive_scan_u32(s, nb.oflag.data(), nb.ofpos.data(), n, nb.counters.data() + c,
+                               nb.scratch.data());
+            exclusive_scan_u32(s, nb.ocnt.data(), nb.opos.data(), n, nb.counters.data() + c + 1,
+                               nb.scratch.data());
+        };
+        // the listed leaves walked again into the overflow region of leaf_ids
+        auto resolve_leaves = [&](u32 cl, u32 n_over, u32 tot) {
+            const u64 base = (u64) L * cl;
+            if (base + tot > 0xffffffffull)
+                throw std::runtime_error("neighbour cache: leaf lists exceed u32 entry indices");
+            nb.leaf_ids.resize_keep(std::max<u64>(base + tot, 1), s);
+            k_over_list<<<div_up(L, 256), 256, 0, s>>>(L, nb.oflag.data(), nb.ofpos.data(),
+                                                        nb.opos.data(), (u32) base, nullptr,
+                                                        nb.olist.data(), nb.leaf_offset.data());
+            // one leaf per wave: the listed leaves are far apart, and a wave walks the union
+            // of its lanes' walks
+            k_leaf_neigh_w<true, false, true><<<div_up(n_over, 256 / WAVE), 256, 0, s>>>(
+                L, I, t.nodes.data(), Rkern, nullptr, nb.leaf_offset.data(), nb.leaf_ids.data(),
+                0, err, 1, nb.olist.data(), n_over);
+        };
         for (int attempt = 0;; attempt++) {
             if (attempt == 4) throw std::runtime_error("neighbour cache: capacities do not settle");
             u32 cl = nb.cap_leaf, cp = nb.cap_part;
-            if ((u64) L * cl > 0xffffffffull || (u64) nm * cp > 0xffffffffull) {
-                // u32 entry indices cannot address fixed-capacity rows: two passes instead
+            if ((u64) L * cl > 0xffffffffull) {
+                // u32 entry indices cannot address fixed-capacity leaf lists: two passes each
                 k_leaf_neigh_w<false, false><<<div_up(L, 256), 256, 0, s>>>(
                     L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nullptr, nullptr, 0, err, 64);
                 exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.leaf_offset.data(), L,
@@ -689,35 +774,65 @@ namespace shamx {
                 stage2_packed<true>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc);
                 nb.leaf_stride = 0;
                 nb.row_stride  = 0;
-                // the measured maxima size the next build, which takes fixed-capacity rows
-                // again if they fit
-                nb.cap_leaf = grow_cap(hc[8]);
-                nb.cap_part = grow_cap(hc[9]);
+                nb.cap_leaf    = choose(hc[8], nb.n_leaf_pairs, L, L);
+                nb.cap_part    = choose(hc[9], nb.n_pairs, n_local, nm);
                 break;
             }
+            // stage 1: fixed-capacity leaf lists
             nb.leaf_ids.resize(std::max<u64>((u64) L * cl, 1));
-            nb.ids.resize(std::max<u64>((u64) nm * cp, 1));
             fill_u32(s, nb.counters.data() + 8, 0, 8);
             const u32 lpw = leaf_lanes();
             k_leaf_neigh_w<true, true><<<div_up(div_up(L, lpw), 256 / WAVE), 256, 0, s>>>(
                 L, I, t.nodes.data(), Rkern, nb.leaf_cnt.data(), nb.leaf_offset.data(),
                 nb.leaf_ids.data(), cl, err, lpw);
             max_u32(s, L, nb.leaf_cnt.data(), nb.counters.data() + 8);
+            exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.psum.data(), L, nb.counters.data() + 10,
+                               nb.scratch.data());
+            mark(L, nb.leaf_cnt.data(), cl, 12);
+            const bool resolve_first = nb.leaf_overflowed;
+            if (resolve_first) {
+                SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
+                                         hipMemcpyDeviceToHost, s));
+                SHAMX_HIP(hipStreamSynchronize(s));
+                if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
+                if (hc[12]) resolve_leaves(cl, hc[12], hc[13]);
+            }
+            const u32 leaf_cap = resolve_first ? 0u : cl; // lists past cl are truncated
+            if ((u64) nm * cp > 0xffffffffull || packed_rows_forced()) {
+                // particle rows packed (count, scan, fill)
+                if (!resolve_first) {
+                    SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
+                                             hipMemcpyDeviceToHost, s));
+                    SHAMX_HIP(hipStreamSynchronize(s));
+                    if (hc[4])
+                        throw std::runtime_error("neighbour cache: tree traversal stack overflow");
+                    if (hc[12]) resolve_leaves(cl, hc[12], hc[13]);
+                }
+                const u32 n_leaf_pairs = hc[10];
+                nb.leaf_overflowed     = hc[12] > 0;
+                const u32 mxl          = hc[8];
+                stage2_packed<true>(s, nb, t, n_local, x, y, z, h, htol, Rker2, hc, 0);
+                nb.n_leaf_pairs = n_leaf_pairs;
+                nb.leaf_stride  = cl;
+                nb.row_stride   = 0;
+                nb.cap_leaf     = choose(mxl, n_leaf_pairs, L, L);
+                nb.cap_part     = choose(hc[9], nb.n_pairs, n_local, nm);
+                break;
+            }
+            // stage 2: fixed-capacity particle rows
+            nb.ids.resize(std::max<u64>((u64) nm * cp, 1));
             fill_u32(s, nb.cnt.data(), 0, nm); // ghost rows stay empty
             k_group_neigh<true, true, true><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
                 nb.item_leaf.data(), nb.bstart.data(), nb.counters.data() + 7, htol, Rker2,
-                nb.cnt.data(), nb.offset.data(), nb.ids.data(), cp, cl);
+                nb.cnt.data(), nb.offset.data(), nb.ids.data(), cp, leaf_cap);
             k_part_neigh<true, true, true><<<div_up(n_local, 128), 128, 0, s>>>(
                 n_local, nb.mism.data(), mism_len, I, t.nodes.data(), t.sort_index.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(), x,
                 y, z, h, nb.inv.data(), htol, Rker2, nb.cnt.data(), nb.offset.data(),
-                nb.ids.data(), cp, cl);
+                nb.ids.data(), cp, leaf_cap);
             max_u32(s, nm, nb.cnt.data(), nb.counters.data() + 9);
-            // totals (the scans' outputs are not used)
-            exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.psum.data(), L, nb.counters.data() + 10,
-                               nb.scratch.data());
             exclusive_scan_u32(s, nb.cnt.data(), nb.psum.data(), nm, nb.counters.data() + 11,
                                nb.scratch.data());
             SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
@@ -726,20 +841,44 @@ namespace shamx {
             if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
             if (hc[4] & 2u)
                 throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
-            u32 max_leaf = hc[8], max_part = hc[9];
-            if (max_leaf > cl) {
-                nb.cap_leaf = grow_cap(max_leaf);
+            if (!resolve_first && hc[12]) {
+                // leaf lists overflowed for the first time: stage 2 read truncated lists
+                nb.leaf_overflowed = true;
                 continue;
             }
-            if (max_part > cp) {
-                nb.cap_part = grow_cap(max_part);
-                continue;
+            nb.leaf_overflowed = hc[12] > 0;
+            nb.n_leaf_pairs    = hc[10];
+            nb.n_pairs         = hc[11];
+            nb.n_mismatch      = hc[5];
+            nb.leaf_stride     = cl;
+            nb.row_stride      = cp;
+            const u32 mxl = hc[8], mxp = hc[9];
+            // particle rows past cp: list them (sorted positions -> particle ids) and fill each
+            // again, one wave per row, into the overflow region of ids
+            fill_u32(s, nb.counters.data() + 14, 0, 2);
+            mark(nm, nb.cnt.data(), cp, 14);
+            SHAMX_HIP(hipMemcpyAsync(hc + 14, nb.counters.data() + 14, 2 * sizeof(u32),
+                                     hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            if (hc[14]) {
+                const u64 base = (u64) nm * cp;
+                if (base + hc[15] > 0xffffffffull)
+                    throw std::runtime_error("neighbour cache: rows exceed u32 entry indices");
+                nb.ids.resize_keep(base + hc[15], s);
+                k_over_list<<<div_up(nm, 256), 256, 0, s>>>(nm, nb.oflag.data(), nb.ofpos.data(),
+                                                             nb.opos.data(), (u32) base,
+                                                             t.sort_index.data(), nb.olist.data(),
+                                                             nb.offset.data());
+                k_group_neigh<true, true, false, true>
+                    <<<div_up(hc[14], (u32) GRP_WAVES), GRP_WAVES * WAVE, 0, s>>>(
+                        n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
+                        nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(),
+                        nb.leaf_ids.data(), nb.olist.data(), nullptr, nb.counters.data() + 14,
+                        htol, Rker2, nullptr, nb.offset.data(), nb.ids.data(), 0, 0,
+                        nb.inv.data());
             }
-            nb.n_leaf_pairs = hc[10];
-            nb.n_pairs      = hc[11];
-            nb.n_mismatch   = hc[5];
-            nb.leaf_stride  = cl;
-            nb.row_stride   = cp;
+            nb.cap_leaf = choose(mxl, nb.n_leaf_pairs, L, L);
+            nb.cap_part = choose(mxp, nb.n_pairs, n_local, nm);
             break;
         }
         SHAMX_HIP(hipGetLastError());

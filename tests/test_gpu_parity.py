@@ -495,6 +495,43 @@ def test_density_force_pass_parity(workload):
     g.close()
 
 
+@pytest.mark.parametrize("mode", ["packed_rows", "overflow"])
+@pytest.mark.parametrize("workload", ["disc_20000", "plummer_6000", "sedov_16"])
+def test_neighbour_layouts_parity(workload, mode, monkeypatch):
+    """The neighbour-cache layouts that the workloads of these tests do not reach by
+    themselves.  packed_rows (SHAMX_NEIGH_ROWS=packed): fixed-capacity leaf lists and packed
+    particle rows (count, scan, fill), the path of rows that u32 entry indices cannot address
+    at fixed capacity.  overflow (SHAMX_NEIGH_CAP=16): capacities of 16, so that nearly every
+    leaf list and row runs past its capacity and is filled again into the overflow regions
+    (the path of the few outlier rows of a disc whose tail particles have 20x the median h).
+    Whole steps (or passes, Plummer) vs the oracle, lists included.  (The disc needs 20,000
+    particles for its first step's h iteration to converge, in the oracle too.)"""
+    if mode == "packed_rows":
+        monkeypatch.setenv("SHAMX_NEIGH_ROWS", "packed")
+    else:
+        monkeypatch.setenv("SHAMX_NEIGH_CAP", "16")
+    name, n = workload.rsplit("_", 1)
+    if name == "disc":
+        cfg, f = setups.disc(int(n), seed=2)
+    elif name == "plummer":
+        cfg, f = setups.plummer(int(n), seed=1, r_max=30.0)
+    else:
+        cfg, f = setups.sedov(int(n))
+    g = GpuSolver(cfg, f)
+    o = orc.Solver(cfg, f)
+    for step in range(2):
+        say(f"[layouts] {workload} {mode} step {step}")
+        if name == "plummer":
+            lg, lo = g.density_force(), o.density_force()
+        else:
+            lg, lo = g.evolve_once(), o.evolve_once()
+        assert lg["n_pairs"] == lo["n_pairs"]
+        _compare_transient(g.transient(), o.transient(), exact_sort=(name == "sedov"))
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du"):
+            assert relerr(g.get(k), o.get(k)) <= RTOL, (k, relerr(g.get(k), o.get(k)))
+    g.close()
+
+
 def test_mixed_precision_error_bound():
     """Mixed step: rel-L2 of a and du/dt vs the FP64 step on the same state <= 1e-5."""
     cfg, f = setups.sedov(24)
