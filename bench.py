@@ -201,6 +201,78 @@ def plummer_pass(n, pos, hh, reps=3):
     return res
 
 
+def disc_energy(g, cfg):
+    """Total energy of gas + sink: sum m (u + v^2 / 2) of the gas, its potential energy in the
+    softened sink field (-G M m / sqrt(r^2 + eps^2), the potential of the force the library
+    applies; zero force inside the accretion radius, which no particle reaches here) and the
+    sink's kinetic energy."""
+    m = cfg["gpart_mass"]
+    x, y, z = g.get("x"), g.get("y"), g.get("z")
+    vx, vy, vz, u = g.get("vx"), g.get("vy"), g.get("vz"), g.get("u")
+    e = m * float(np.sum(u + 0.5 * (vx * vx + vy * vy + vz * vz)))
+    eps2 = cfg["sink_softening"] ** 2
+    for sk in g.get_sinks():
+        px, py, pz = sk["pos"]
+        r2 = (x - px) ** 2 + (y - py) ** 2 + (z - pz) ** 2
+        e -= cfg["constant_G"] * sk["mass"] * m * float(np.sum(1.0 / np.sqrt(r2 + eps2)))
+        e += 0.5 * sk["mass"] * float(np.dot(sk["velocity"], sk["velocity"]))
+    return e
+
+
+def disc_4m(steps=50, warmup=2):
+    """The Keplerian gas disc: 4,194,304 particles, Sigma ~ 1/r on [1, 10], H/R = 0.05 at r = 1,
+    LP07 c_s ~ r^-1/4, central sink of mass 1 (G = 1, softening 0.1), AV 0.1; `steps` timed
+    mixed-precision steps after `warmup` (the first is the dt = 0 step).  Reports the rate, the
+    relative total-energy drift over the timed steps (gas + sink, potential included) and the
+    rel-L2 error of a and du/dt of one mixed step against one FP64 step from the final state."""
+    import torch
+
+    from shamrock_amd import setups
+    from shamrock_amd.solver import GpuSolver
+
+    t_setup = time.perf_counter()
+    cfg, f = setups.disc(4_194_304, seed=2, mixed=True)
+    log(f"[bench] disc 4M setup {time.perf_counter() - t_setup:.1f} s")
+    g = GpuSolver(cfg, f)
+    del f
+    for i in range(warmup):
+        lg = g.evolve_once()
+        log(f"  disc warmup {i}: {lg['t_total'] * 1e3:.1f} ms (subcycles {lg['subcycles']})")
+    e0 = disc_energy(g, cfg)
+    logs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        logs.append(g.evolve_once())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    e1 = disc_energy(g, cfg)
+    n = g.n
+    st = {k: g.get(k) for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u",
+                                "du")}
+    tm, sinks = g.time(), g.sink_records()
+    g.close()
+    out = {}
+    for mixed in (0, 1):
+        s = GpuSolver(dict(cfg, mixed_precision=mixed, sinks=[]), st)
+        s.set_sink_records(sinks)
+        s.set_time(tm["t"], tm["dt"], tm["cfl_mult"])
+        s.evolve_once()
+        out[mixed] = (np.stack([s.get(k) for k in ("ax", "ay", "az")]), s.get("du"))
+        s.close()
+    phases = {k: float(np.mean([lg[k] for lg in logs])) * 1e3
+              for k in ("t_tree", "t_sort", "t_neigh", "t_hiter", "t_omega", "t_eos", "t_force",
+                        "t_corr")}
+    log(f"[bench] disc 4M mixed: {el / steps * 1e3:.2f} ms/step")
+    return {"workload": "keplerian_disc_4194304_M4_LP07_sink", "n_particles": n, "steps": steps,
+            "dtype": "mixed (f32 pair sums, f64 positions)",
+            "value": n * steps / el, "unit": "particle-updates/s",
+            "ms_per_step": el / steps * 1e3, "phase_ms": phases,
+            "energy_drift_rel": abs(e1 - e0) / abs(e0),
+            "mixed_vs_fp64_relL2": {"acc": relL2(out[1][0], out[0][0]),
+                                     "du": relL2(out[1][1], out[0][1])}}
+
+
 def differentiable_kh64(steps=10, checkpoint_every=5):
     """The differentiable Kelvin-Helmholtz workload: 64^3 (262,444 particles), M4, 10 FP64 steps,
     a checkpoint every 5; loss = mean squared difference of the final density from a target
@@ -304,6 +376,8 @@ def main():
                     help="also run one 512 MiB device copy (counter calibration)")
     ap.add_argument("--no-scale-probe", action="store_true",
                     help="skip the clustered workload (16,777,216 Plummer particles)")
+    ap.add_argument("--no-disc", action="store_true",
+                    help="skip the Keplerian disc workload (4,194,304 particles, 50 mixed steps)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiable Kelvin-Helmholtz workload (64^3)")
     ap.add_argument("--clustered-only", action="store_true",
@@ -392,6 +466,8 @@ def main():
     g64.close()
     if not args.no_adjoint:
         out["differentiable_kh64"] = differentiable_kh64()
+    if not args.no_disc:
+        out["disc_4M"] = disc_4m()
     if not args.no_scale_probe:
         probe = scale_probe()
         out["plummer_16M_pass"] = probe.pop("pass_plummer_16M")

@@ -151,7 +151,7 @@ def knn_h(pos, kernel: str) -> np.ndarray:
     R = KERNEL_RKERN[kernel]
     k = int(round(4.0 / 3.0 * math.pi * (KERNEL_HFACT[kernel] * R) ** 3))
     k = max(1, min(k, pos.shape[0] - 1))
-    d, _ = cKDTree(pos).query(pos, k=k + 1)
+    d, _ = cKDTree(pos).query(pos, k=k + 1, workers=-1)
     return d[:, -1] / R
 
 
