@@ -34,6 +34,14 @@ namespace shamx {
                    (fmax(alo[2], blo[2]) <= fmin(ahi[2], bhi[2]));
         }
 
+        // orders the LDS accesses of one wave's lanes around this point (LDS executes a wave's
+        // operations in issue order; the fences keep the compiler from moving them across)
+        __device__ inline void wave_lds_sync() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+
         // Depth-first traversal shared by the 64 lanes of a wave.  The stack lives in lanes
         // (entry i in lane i: node id, the mask of lanes whose walk reaches it, its two child
         // ids).  A node is tested before it is pushed, by the lanes of its parent's mask, so
@@ -170,6 +178,140 @@ namespace shamx {
             if (STRIDE && valid) offset[g] = c0;
         }
 
+        // One listed leaf per wave, its walk spread over the lanes: the tree is expanded breadth
+        // first into an ordered frontier of at least 64 subtrees (or of all its accepted
+        // leaves), then each lane walks a contiguous run of frontier entries depth first, left
+        // child first, with its own stack.  The frontier stays in left-to-right order and every
+        // lane walk is left-first, so the lanes' outputs in lane order are the leaf's list in
+        // the order of the reference's walk.  Each lane counts first; the wave's total must
+        // equal cnt[g] (the count of the fixed-capacity pass) before anything is written at
+        // offset[g] (err bit 4 otherwise).
+        constexpr int WIDE_FR = 256, WIDE_ST = 48;
+        __global__ void __launch_bounds__(WAVE)
+            k_leaf_walk_wide(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
+                             const u32 *__restrict list, u32 n_list, const u32 *__restrict cnt,
+                             const u32 *__restrict offset, u32 *__restrict ids,
+                             u32 *__restrict err) {
+            __shared__ u32 fr[2][WIDE_FR];
+            __shared__ u32 stk[WIDE_ST][WAVE]; // [depth][lane]
+            const u32 lane = (u32) __lane_id();
+            const u32 j    = blockIdx.x;
+            if (j >= n_list) return;
+            const u32 g     = list[j];
+            const f64 *ra   = nodes + (u64) (I + g) * NODE_F64;
+            const f64 rint_a = ra[6] * Rkern;
+            const f64 alo[3] = {ra[0], ra[1], ra[2]}, ahi[3] = {ra[3], ra[4], ra[5]};
+            const f64 elo[3] = {alo[0] - rint_a, alo[1] - rint_a, alo[2] - rint_a};
+            const f64 ehi[3] = {ahi[0] + rint_a, ahi[1] + rint_a, ahi[2] + rint_a};
+            auto crit = [&](u32 node) {
+                const f64 *rec = nodes + (u64) node * NODE_F64;
+                f64 r      = rec[6] * Rkern;
+                f64 nlo[3] = {rec[0], rec[1], rec[2]}, nhi[3] = {rec[3], rec[4], rec[5]};
+                f64 xlo[3] = {nlo[0] - r, nlo[1] - r, nlo[2] - r};
+                f64 xhi[3] = {nhi[0] + r, nhi[1] + r, nhi[2] + r};
+                return cella_neigh_b(alo, ahi, xlo, xhi) || cella_neigh_b(elo, ehi, nlo, nhi);
+            };
+            const u32 ncell = I + L;
+            constexpr u32 ACC = 0x80000000u; // frontier entry: a leaf already accepted
+            // breadth-first expansion (wave-uniform control)
+            u32 F = 1, cur = 0;
+            if (lane == 0) fr[0][0] = 0; // the root
+            wave_lds_sync();
+            bool bad = false;
+            for (int round = 0;; round++) {
+                if (round == 128) { // deeper than any tree of u64 codes: a broken tree
+                    bad = true;
+                    break;
+                }
+                u32 total = 0;
+                bool internal = false, full = false;
+                for (u32 b = 0; b < F; b += WAVE) {
+                    const u32 e = b + lane;
+                    u32 no = 0, o0 = 0, o1 = 0;
+                    if (e < F) {
+                        const u32 node = fr[cur][e];
+                        if (node & ACC) {
+                            no = 1;
+                            o0 = node;
+                        } else if (node < ncell && crit(node)) {
+                            if (node >= I) {
+                                no = 1;
+                                o0 = node | ACC;
+                            } else {
+                                node_children(nodes, node, o0, o1);
+                                no = (o0 < ncell && o1 < ncell) ? 2u : 0u;
+                                bad |= no == 0;
+                            }
+                        }
+                    }
+                    u32 inc = no;
+                    for (int o = 1; o < WAVE; o <<= 1) {
+                        u32 t = __shfl_up(inc, o, WAVE);
+                        if ((int) lane >= o) inc += t;
+                    }
+                    const u32 ct = __shfl(inc, WAVE - 1, WAVE);
+                    if (total + ct > WIDE_FR) {
+                        full = true;
+                        break;
+                    }
+                    const u32 pos = total + inc - no;
+                    if (no >= 1) fr[cur ^ 1][pos] = o0;
+                    if (no == 2) fr[cur ^ 1][pos + 1] = o1;
+                    internal |= __ballot(no == 2) != 0;
+                    total += ct;
+                }
+                wave_lds_sync();
+                if (full) break; // keep the current frontier
+                cur ^= 1;
+                F = total;
+                if (!internal || F >= WAVE) break;
+            }
+            // lane k: entries [k F / 64, (k + 1) F / 64) in order, each walked depth first
+            const u32 b0 = (lane * F) / WAVE, b1 = ((lane + 1) * F) / WAVE;
+            auto walk = [&](auto emit) {
+                for (u32 e = b0; e < b1; e++) {
+                    const u32 node = fr[cur][e];
+                    if (node & ACC) {
+                        emit(node & ~ACC);
+                        continue;
+                    }
+                    int sp               = 0;
+                    stk[sp++][lane]      = node;
+                    while (sp > 0) {
+                        const u32 nd = stk[--sp][lane];
+                        if (nd >= ncell || !crit(nd)) continue;
+                        if (nd >= I) {
+                            emit(nd);
+                            continue;
+                        }
+                        u32 cl, cr;
+                        node_children(nodes, nd, cl, cr);
+                        if (sp + 2 > WIDE_ST) {
+                            bad = true;
+                            break;
+                        }
+                        stk[sp++][lane] = cr; // the left child is popped first
+                        stk[sp++][lane] = cl;
+                    }
+                }
+            };
+            u32 c = 0;
+            walk([&](u32) { c++; });
+            u32 inc = c;
+            for (int o = 1; o < WAVE; o <<= 1) {
+                u32 t = __shfl_up(inc, o, WAVE);
+                if ((int) lane >= o) inc += t;
+            }
+            const u32 tot = __shfl(inc, WAVE - 1, WAVE);
+            const bool wave_bad = __ballot(bad) != 0;
+            if (wave_bad || tot != cnt[g]) {
+                if (lane == 0) atomicOr(err, wave_bad ? 1u : 4u);
+                return;
+            }
+            u32 pos = offset[g] + inc - c;
+            walk([&](u32 nd) { ids[pos++] = nd; });
+        }
+
         // rows (leaf lists or particle rows) longer than their fixed capacity: flag and count
         __global__ void k_over_mark(u32 n, const u32 *__restrict cnt, u32 cap,
                                     u32 *__restrict flag, u32 *__restrict ocnt) {
@@ -303,13 +445,6 @@ namespace shamx {
             if (s < nm && flag[s]) lpos[pos[s]] = s;
         }
 
-        // orders the LDS accesses of one wave's lanes around this point (LDS executes a wave's
-        // operations in issue order; the fences keep the compiler from moving them across)
-        __device__ inline void wave_lds_sync() {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
 
         __device__ inline f64 wave_bcast(f64 v, int lane) {
             i64 b  = __double_as_longlong(v);
@@ -553,6 +688,15 @@ namespace shamx {
             return div_up(bound, (u32) GRP_WAVES);
         }
 
+        // error flags of the walks (counters[4]) into exceptions
+        void check_walk_flags(u32 f) {
+            if (f & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
+            if (f & 2u)
+                throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
+            if (f & 4u)
+                throw std::runtime_error("neighbour cache: a leaf walk disagrees with its count");
+        }
+
         // packed rows, two passes (count, scan, fill); SORTED: rows by sorted position
         // (leaf_cap: capacity of fixed-capacity leaf lists, 0 for packed ones)
         template<bool SORTED>
@@ -584,9 +728,7 @@ namespace shamx {
             SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
                                      hipMemcpyDeviceToHost, s));
             SHAMX_HIP(hipStreamSynchronize(s));
-            if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
-            if (hc[4] & 2u)
-                throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
+            check_walk_flags(hc[4]);
             nb.n_pairs    = hc[1];
             nb.n_mismatch = hc[5];
             nb.ids.resize(std::max<u64>(nb.n_pairs, 1));
@@ -747,11 +889,12 @@ namespace shamx {
             k_over_list<<<div_up(L, 256), 256, 0, s>>>(L, nb.oflag.data(), nb.ofpos.data(),
                                                         nb.opos.data(), (u32) base, nullptr,
                                                         nb.olist.data(), nb.leaf_offset.data());
-            // one leaf per wave: the listed leaves are far apart, and a wave walks the union
-            // of its lanes' walks
-            k_leaf_neigh_w<true, false, true><<<div_up(n_over, 256 / WAVE), 256, 0, s>>>(
-                L, I, t.nodes.data(), Rkern, nullptr, nb.leaf_offset.data(), nb.leaf_ids.data(),
-                0, err, 1, nb.olist.data(), n_over);
+            // one leaf per wave, its walk spread over the lanes (the listed leaves are far
+            // apart, and a wave walks the union of its lanes' walks)
+            k_leaf_walk_wide<<<n_over, WAVE, 0, s>>>(L, I, t.nodes.data(), Rkern,
+                                                      nb.olist.data(), n_over,
+                                                      nb.leaf_cnt.data(), nb.leaf_offset.data(),
+                                                      nb.leaf_ids.data(), err);
         };
         for (int attempt = 0;; attempt++) {
             if (attempt == 4) throw std::runtime_error("neighbour cache: capacities do not settle");
@@ -838,9 +981,7 @@ namespace shamx {
             SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
                                      hipMemcpyDeviceToHost, s));
             SHAMX_HIP(hipStreamSynchronize(s));
-            if (hc[4] & 1u) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
-            if (hc[4] & 2u)
-                throw std::runtime_error("neighbour cache: a particle is outside every leaf box");
+            check_walk_flags(hc[4]);
             if (!resolve_first && hc[12]) {
                 // leaf lists overflowed for the first time: stage 2 read truncated lists
                 nb.leaf_overflowed = true;
