@@ -59,21 +59,21 @@ namespace shamx {
             k_force_vjp(u32 n, Rows1 r, const FRecView *__restrict rec,
                         const f64 *__restrict omega_m, const f64 *__restrict sax,
                         const f64 *__restrict say, const f64 *__restrict saz,
-                        const f64 *__restrict sdu, PairParams pp, ForceGrad o) {
+                        const f64 *__restrict sdu, PairParams pp, f64 lp_q, ForceGrad o) {
             u32 j = blockIdx.x * ABLK + threadIdx.x;
             if (j >= n) return;
             u32 row = r.sel[j], a = r.outi[row];
             const FRecView A = rec[row];
-            const PairSide SA{{A.vx, A.vy, A.vz}, A.u, A.h, omega_m[a]};
+            const PairSide SA{{A.vx, A.vy, A.vz}, A.u, A.h, omega_m[a], A.cs};
             const f64 seed_a[3] = {sax[row], say[row], saz[row]};
             const f64 sdu_a     = sdu[row];
-            f64 gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gu = 0, gh = 0, gom = 0;
+            f64 gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gu = 0, gh = 0, gom = 0, gcs = 0;
             u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
             for (u32 k = k0; k < k1; k++) {
                 u32 b = r.ids[k];
                 if (b == row) continue;
                 const FRecView B = rec[b];
-                const PairSide SB{{B.vx, B.vy, B.vz}, B.u, B.h, omega_m[r.outi[b]]};
+                const PairSide SB{{B.vx, B.vy, B.vz}, B.u, B.h, omega_m[r.outi[b]], B.cs};
                 f64 rab[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
                 f64 acc[3], du, gr[3];
                 PairGrad g1, g2;
@@ -86,6 +86,7 @@ namespace shamx {
                 gu += g1.u;
                 gh += g1.h;
                 gom += g1.omega;
+                gcs += g1.cs;
                 // mirrored term: b's outputs (pair b, a) seeded by b's seeds; a is side B
                 const f64 seed_b[3] = {sax[b], say[b], saz[b]};
                 f64 rba[3]          = {-rab[0], -rab[1], -rab[2]};
@@ -97,6 +98,15 @@ namespace shamx {
                 gu += g2.u;
                 gh += g2.h;
                 gom += g2.omega;
+                gcs += g2.cs;
+            }
+            if (pp.eos_kind == 2) {
+                // LP07: c_s = c_s0 (|x| / r0)^(-q), so d c_s / dx = -q c_s x / |x|^2
+                const f64 R2 = A.x * A.x + A.y * A.y + A.z * A.z;
+                const f64 c  = R2 > 0 ? gcs * (-lp_q * A.cs / R2) : 0.;
+                gx[0] += c * A.x;
+                gx[1] += c * A.y;
+                gx[2] += c * A.z;
             }
             o.gx[a]  = gx[0];
             o.gy[a]  = gx[1];
@@ -304,7 +314,8 @@ namespace shamx {
         dispatch_adjoint_kernel(A.kernel, [&](auto K) {
             using KT = decltype(K);
             k_force_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(
-                N, rf, (const FRecView *) A.frec, A.omega, sf[0], sf[1], sf[2], sf[3], pp, g);
+                N, rf, (const FRecView *) A.frec, A.omega, sf[0], sf[1], sf[2], sf[3], pp,
+                A.lp_q, g);
             // Omega bar of the merged particles, then the Omega adjoint
             const f64 *oin[1] = {g.gom};
             f64 *oout[1]      = {s_omb};

@@ -87,23 +87,35 @@ namespace shamx {
 
     struct PairParams {
         f64 mass, hfact, gamma, alpha_u, alpha_AV, beta_AV;
-        int eos_kind; // 0 adiabatic, 1 isothermal (sound speed cs_iso)
+        int eos_kind; // 0 adiabatic, 1 isothermal (sound speed cs_iso), 2 LP07 (side's cs)
         f64 cs_iso;
     };
 
-    // per-particle state entering a force pair
+    // per-particle state entering a force pair (cs: the particle's LP07 sound speed, which
+    // depends on its position; unused by the other equations of state)
     struct PairSide {
-        f64 v[3], u, h, omega;
+        f64 v[3], u, h, omega, cs;
     };
     // gradient of the seeded pair outputs w.r.t. one side's inputs
     struct PairGrad {
-        f64 v[3], u, h, omega;
+        f64 v[3], u, h, omega, cs;
     };
 
-    __host__ __device__ inline void eos_of(const PairParams &p, f64 rho, f64 u, f64 &P, f64 &cs,
-                                           f64 &dP_drho, f64 &dP_du, f64 &dcs_drho,
-                                           f64 &dcs_du) {
-        if (p.eos_kind == 0) {
+    // P, c_s and their partial derivatives; for LP07 (ComputeEos.cpp:726-798) P = c_s^2 rho
+    // with c_s the side's own (dP_dcs = 2 c_s rho, c_s passes through)
+    __host__ __device__ inline void eos_of(const PairParams &p, f64 rho, f64 u, f64 cs_loc,
+                                           f64 &P, f64 &cs, f64 &dP_drho, f64 &dP_du,
+                                           f64 &dcs_drho, f64 &dcs_du, f64 &dP_dcs) {
+        dP_dcs = 0;
+        if (p.eos_kind == 2) {
+            P        = cs_loc * cs_loc * rho;
+            cs       = cs_loc;
+            dP_drho  = cs_loc * cs_loc;
+            dP_du    = 0;
+            dcs_drho = 0;
+            dcs_du   = 0;
+            dP_dcs   = 2 * cs_loc * rho;
+        } else if (p.eos_kind == 0) {
             P  = (p.gamma - 1) * rho * u;
             cs = std::sqrt(p.gamma * P / rho);
             dP_drho = (p.gamma - 1) * u;
@@ -136,9 +148,9 @@ namespace shamx {
         f64 e[3] = {r[0] * inv, r[1] * inv, r[2] * inv};
         f64 hfa = p.hfact / A.h, hfb = p.hfact / B.h;
         f64 rho_a = m * hfa * hfa * hfa, rho_b = m * hfb * hfb * hfb;
-        f64 Pa, csa, dPa_r, dPa_u, dca_r, dca_u, Pb, csb, dPb_r, dPb_u, dcb_r, dcb_u;
-        eos_of(p, rho_a, A.u, Pa, csa, dPa_r, dPa_u, dca_r, dca_u);
-        eos_of(p, rho_b, B.u, Pb, csb, dPb_r, dPb_u, dcb_r, dcb_u);
+        f64 Pa, csa, dPa_r, dPa_u, dca_r, dca_u, dPa_c, Pb, csb, dPb_r, dPb_u, dcb_r, dcb_u, dPb_c;
+        eos_of(p, rho_a, A.u, A.cs, Pa, csa, dPa_r, dPa_u, dca_r, dca_u, dPa_c);
+        eos_of(p, rho_b, B.u, B.cs, Pb, csb, dPb_r, dPb_u, dcb_r, dcb_u, dPb_c);
         KernDeriv<K> ka(d, A.h), kb(d, B.h);
         f64 Fa = ka.Wr, Fb = kb.Wr;
         f64 vab[3] = {A.v[0] - B.v[0], A.v[1] - B.v[1], A.v[2] - B.v[2]};
@@ -163,8 +175,8 @@ namespace shamx {
         du = m * Sa * Fa * w + cu * vsu * dU * Y;
         if (!ab) return;
         // ---- reverse
-        gA = PairGrad{{0, 0, 0}, 0, 0, 0};
-        gB = PairGrad{{0, 0, 0}, 0, 0, 0};
+        gA = PairGrad{{0, 0, 0}, 0, 0, 0, 0};
+        gB = PairGrad{{0, 0, 0}, 0, 0, 0, 0};
         gr[0] = gr[1] = gr[2] = 0;
         if (d < 1e-9) return;
         f64 ae    = ab[0] * e[0] + ab[1] * e[1] + ab[2] * e[2];
@@ -239,6 +251,10 @@ namespace shamx {
         gA.u += Pa_b * dPa_u + csa_b * dca_u;
         rho_b_b += Pb_b * dPb_r + csb_b * dcb_r;
         gB.u += Pb_b * dPb_u + csb_b * dcb_u;
+        if (p.eos_kind == 2) { // LP07: c_s is the side's own input
+            gA.cs += Pa_b * dPa_c + csa_b;
+            gB.cs += Pb_b * dPb_c + csb_b;
+        }
         // rho = m (hfact / h)^3
         gA.h += rho_a_b * (-3 * rho_a / A.h);
         gB.h += rho_b_b * (-3 * rho_b / B.h);

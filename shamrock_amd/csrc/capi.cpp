@@ -619,8 +619,9 @@ struct shamx_solver {
     DevArray<f64> adj_work;
     AdjointArgs adjoint_args() {
         need(N > 0 && tree.L > 0, "adjoint: run evolve_once first");
-        need(cfg.eos_kind == 0 || cfg.eos_kind == 1,
-             "adjoint: available for the adiabatic and isothermal equations of state");
+        need(cfg.eos_kind == 0 || cfg.eos_kind == 1 || (cfg.eos_kind == 2 && !cfg.periodic),
+             "adjoint: available for the adiabatic and isothermal equations of state, and for "
+             "LP07 with free boundaries (periodic images sit at other radii)");
         need(!cfg.mixed_precision, "adjoint: the step must run in FP64");
         need(cfg.av_kind == 0, "adjoint: available for the constant artificial viscosity");
         need(sinks.empty(), "adjoint: not available with sinks");
@@ -638,6 +639,7 @@ struct shamx_solver {
                            cfg.beta_AV,
                            cfg.eos_kind,
                            cfg.eos_cs,
+                           cfg.eos_q,
                            tree.sort_index.data(),
                            G ? ghost_src.data() : nullptr,
                            nb.lpos.data(),

@@ -228,6 +228,7 @@ namespace shamx {
         f64 dt, mass, hfact, gamma, alpha_u, alpha_AV, beta_AV;
         int eos_kind;
         f64 cs_iso;
+        f64 lp_q; // LP07 exponent: c_s^2 = c_s0^2 (r / r0)^(-2 q), r the distance to the origin
         const u32 *sort_index, *ghost_src, *lpos, *cnt, *offset, *ids, *fcnt, *fids;
         const void *frec;    // FP64 force records of the step's force evaluation
         const double4 *ps;   // (x, y, z, final h), sorted

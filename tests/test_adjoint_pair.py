@@ -78,16 +78,17 @@ def test_kernel_partials(harness, k):
 
 def _pair(lib, k, par, x, seeds, grad=True):
     loss = np.zeros(1)
-    g = np.zeros(15)
+    g = np.zeros(17)
     assert lib.pair_eval(KIDS[k], _p(par), _p(x), _p(seeds), _p(loss),
                          _p(g) if grad else None) == 0
     return loss[0], g
 
 
 @pytest.mark.parametrize("k", ["M4", "M6"])
-@pytest.mark.parametrize("eos", [0, 1])
+@pytest.mark.parametrize("eos", [0, 1, 2])
 def test_force_pair_reverse_sweep_matches_finite_differences(harness, k, eos):
-    """d(seeds . (a_ab, du_ab)) / d(r, v_a, v_b, u_a, u_b, h_a, h_b, Omega_a, Omega_b)."""
+    """d(seeds . (a_ab, du_ab)) / d(r, v_a, v_b, u_a, u_b, h_a, h_b, Omega_a, Omega_b) and, for
+    LP07 (eos 2), d / d(c_s,a, c_s,b) of the sides' own sound speeds."""
     rng = np.random.default_rng(3 + eos)
     par = np.array([1e-3, HFACT[k], 5 / 3, 1.0, 1.0, 2.0, eos, 0.7])
     checked = 0
@@ -99,11 +100,12 @@ def test_force_pair_reverse_sweep_matches_finite_differences(harness, k, eos):
         va, vb = rng.normal(0, 0.3, 3), rng.normal(0, 0.3, 3)
         ua, ub = rng.uniform(0.5, 2.0, 2)
         om_a, om_b = rng.uniform(0.8, 1.2, 2)
-        x = np.concatenate([r, va, vb, [ua, ub, h_a, h_b, om_a, om_b]])
+        cs_a, cs_b = rng.uniform(0.3, 1.5, 2)
+        x = np.concatenate([r, va, vb, [ua, ub, h_a, h_b, om_a, om_b, cs_a, cs_b]])
         seeds = rng.normal(size=4)
         _, g = _pair(harness, k, par, x, seeds)
-        fd = np.zeros(15)
-        for i in range(15):
+        fd = np.zeros(17)
+        for i in range(17):
             e = 1e-6 * max(abs(x[i]), 1e-2)
             xp, xm = x.copy(), x.copy()
             xp[i] += e

@@ -153,6 +153,59 @@ def test_adjoint_kh64_sixty_four_components():
     assert err.max() < 1e-3 and np.median(err) < 1e-5
 
 
+def test_adjoint_lp07_matches_central_differences():
+    """Locally isothermal LP07 gas (the disc workload's EOS, c_s depending on the distance to
+    the origin), free boundaries, no sink: the gradient of sum_a c_a rho_a after 4 steps w.r.t.
+    the initial velocities vs central differences on 48 random components.  The velocities move
+    the particles, the positions set c_s and P, so the position derivative of the sound speed
+    is exercised."""
+    say("[adjoint lp07]")
+    cfg, f = setups.disc(20000, seed=2)
+    cfg = dict(cfg, sinks=[], epsilon_h=1e-12, cfl_cour=0.1, cfl_force=0.1)
+    rng = np.random.default_rng(5)
+    v = np.stack([f["vx"], f["vy"], f["vz"]], 1)
+    v0 = torch.tensor(v + 1e-3 * rng.standard_normal(v.shape), device=DEV)
+    u0 = torch.tensor(f["u"], device=DEV)
+    steps = 4
+    dts = rollout_dts(cfg, f, v0, u0, steps)
+    c = torch.tensor(rng.normal(size=v0.shape[0]), device=DEV)
+    vv = v0.clone().requires_grad_(True)
+    _loss(cfg, f, vv, u0, c, steps, dts).backward()
+    g = vv.grad.cpu().numpy()
+    n = v0.shape[0]
+    comps = [(int(i), int(d)) for i, d in zip(rng.integers(0, n, 48), rng.integers(0, 3, 48))]
+    fd = []
+    with torch.no_grad():
+        for i, d in comps:
+            e = 1e-6
+            vp, vm = v0.clone(), v0.clone()
+            vp[i, d] += e
+            vm[i, d] -= e
+            fd.append((_loss(cfg, f, vp, u0, c, steps, dts) -
+                       _loss(cfg, f, vm, u0, c, steps, dts)).item() / (2 * e))
+    fd = np.array(fd)
+    ad = np.array([g[i, d] for i, d in comps])
+    scale = np.abs(fd).max()
+    err = np.abs(ad - fd) / scale
+    say(f"[adjoint lp07] max |fd| {scale:.3e} max rel err {err.max():.2e} "
+        f"median {np.median(err):.2e}")
+    assert scale > 0
+    assert err.max() < 1e-4
+
+
+def test_adjoint_rejects_lp07_with_periodic_images():
+    from shamrock_amd.solver import GpuSolver
+
+    cfg, f = setups.random_box(300, seed=4, jitter=0.3)
+    cfg = dict(cfg, eos_kind=2, eos_cs0=1.0, eos_q=0.5, eos_r0=1.0)
+    g = GpuSolver(cfg, f)
+    g.evolve_once()
+    seeds = [torch.zeros(g.n, dtype=torch.float64, device=DEV) for _ in range(11)]
+    with pytest.raises(Exception, match="LP07 with free boundaries"):
+        g.adjoint_step(seeds)
+    g.close()
+
+
 def test_adjoint_independent_of_checkpoint_interval():
     say("[adjoint ckpt]")
     cfg, f, v0, u0 = _case("random_box")
