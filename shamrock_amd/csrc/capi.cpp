@@ -154,8 +154,11 @@ struct shamx_solver {
     }
 
     void alloc(u32 n) {
-        N   = n;
-        cap = n + n / 3 + 1024;
+        N = n;
+        // runs of equal Morton codes ordered spatially on clustered sets (the tie order only;
+        // the reference's bitonic network has its own)
+        tree.allow_fine_ties = true;
+        cap                  = n + n / 3 + 1024;
         for (auto *a : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &omega, &P, &cs}) a->resize(cap);
         for (auto *a : {&ax, &ay, &az, &du, &ax_old, &ay_old, &az_old, &du_old, &eps, &h_old,
                         &vsig, &dtc, &alpha_av, &soundspeed, &alpha_new, &divv, &curlx, &curly,
@@ -232,7 +235,7 @@ struct shamx_solver {
             tree_stage_codes(stream, tree, Nm, x.data(), y.data(), z.data());
             timer.end(stream, PH_TREE, b);
             b = timer.begin(stream);
-            tree_stage_sort(stream, tree);
+            tree_stage_sort(stream, tree, x.data(), y.data(), z.data());
             timer.end(stream, PH_SORT, b);
             b = timer.begin(stream);
             tree_stage_finish(stream, tree, x.data(), y.data(), z.data(), h.data(),
@@ -743,7 +746,7 @@ int shamx_tree_reduce_karras(shamx_ctx *ctx, void *stream, uint32_t n, const dou
         need(ctx != nullptr, "shamx_tree_reduce_karras: null context");
         hipStream_t s = (hipStream_t) stream;
         tree_stage_codes(s, ctx->tree, n, x, y, z);
-        tree_stage_sort(s, ctx->tree);
+        tree_stage_sort(s, ctx->tree, x, y, z);
         tree_stage_karras(s, ctx->tree, reduction_level);
     });
 }

@@ -29,10 +29,11 @@ namespace shamx {
     void morton_codes(hipStream_t s, u32 n, const f64 *x, const f64 *y, const f64 *z,
                       const f64 *bbox6, u32 *codes);
     // stable LSD radix sort of (key, index) pairs; sorts keys[0..n) and returns the permutation
-    // in vals (vals are generated as iota in the first pass).  key_bits <= 32.
+    // in vals (vals are generated as iota in the first pass unless vals_given: then the
+    // values travel with their keys).  key_bits <= 32.
     size_t radix_sort_scratch_bytes(u32 n);
     void radix_sort_pairs(hipStream_t s, u32 n, u32 key_bits, u32 *keys, u32 *vals,
-                          u32 *keys_alt, u32 *vals_alt, void *scratch);
+                          u32 *keys_alt, u32 *vals_alt, void *scratch, bool vals_given = false);
 
     // Tree state of a compressed-leaf BVH built on the merged particle set
     struct TreeBuffers {
@@ -54,6 +55,10 @@ namespace shamx {
         DevArray<f64> nodes;           // (I + L) * NODE_F64 records (tree_layout.hpp)
         DevArray<u32> counters;        // small device scalars
         DevArray<unsigned char> scratch;
+        // order runs of equal codes by the position inside the cell (set from the previous
+        // build's distinct-code count when allow_fine_ties; the phase entries keep the plain
+        // stable order)
+        bool fine_ties = false, allow_fine_ties = false;
     };
     // full CompressedLeafBVH::rebuild_from_positions + rint field (BuildTrees.cpp:27-65,
     // CompressedLeafBVH.cpp:31-95, Solver.cpp:1322-1356).  Synchronises once to read L.
@@ -63,7 +68,8 @@ namespace shamx {
     // then reduction + Karras + boxes (synchronises once to read the leaf count)
     void tree_stage_codes(hipStream_t s, TreeBuffers &t, u32 n, const f64 *x, const f64 *y,
                           const f64 *z);
-    void tree_stage_sort(hipStream_t s, TreeBuffers &t);
+    void tree_stage_sort(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
+                         const f64 *z);
     void tree_stage_finish(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
                            const f64 *z, const f64 *h, u32 reduction_level, f64 htol);
     // the two halves of tree_stage_finish: leaf reduction + Karras tree (synchronises once for

@@ -109,6 +109,32 @@ namespace shamx {
             codes[i] = expand_bits_10(gx) * 4 + expand_bits_10(gy) * 2 + expand_bits_10(gz);
         }
 
+        // position inside its Morton cell, as a 30-bit Morton code of a 1024^3 sub-grid of the
+        // cell (the cell from the same transform as to_grid): orders particles that share a
+        // code spatially
+        __device__ inline u32 to_subgrid(f64 c, f64 lo, f64 hi) {
+            c        = fmin(fmax(c, lo), hi);
+            f64 fact = (hi - lo) / 1024.;
+            f64 t    = (c - lo) / fact;
+            u32 g    = (u32) min(max((u16) t, (u16) 0), (u16) 1023);
+            f64 fr   = t - (f64) g; // in [0, 1] (1 only where the coordinate was clamped)
+            return (u32) fmin(fmax(fr * 1024., 0.), 1023.);
+        }
+        __global__ void k_fine_key(u32 n, const f64 *__restrict x, const f64 *__restrict y,
+                                   const f64 *__restrict z, const f64 *__restrict bb,
+                                   u32 *__restrict key) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            key[i] = expand_bits_10(to_subgrid(x[i], bb[0], bb[3])) * 4 +
+                     expand_bits_10(to_subgrid(y[i], bb[1], bb[4])) * 2 +
+                     expand_bits_10(to_subgrid(z[i], bb[2], bb[5]));
+        }
+        __global__ void k_gather_u32(u32 n, const u32 *__restrict src, const u32 *__restrict idx,
+                                     u32 *__restrict dst) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i < n) dst[i] = src[idx[i]];
+        }
+
         // --------------------------------------------------------------------- radix sort
         // Onesweep (one global histogram pass, then one scatter kernel per 8-bit digit with
         // decoupled look-back across tiles).  Ranks inside a tile are computed with wave-wide
@@ -551,7 +577,7 @@ namespace shamx {
     }
 
     void radix_sort_pairs(hipStream_t s, u32 n, u32 key_bits, u32 *keys, u32 *vals,
-                          u32 *keys_alt, u32 *vals_alt, void *scratch) {
+                          u32 *keys_alt, u32 *vals_alt, void *scratch, bool vals_given) {
         if (n == 0) return;
         if (key_bits == 0 || key_bits > 32) throw std::invalid_argument("radix sort key_bits");
         u32 passes   = div_up(key_bits, 8);
@@ -569,7 +595,7 @@ namespace shamx {
             // (32 keys per thread measured slower: 0.90 vs 0.80 ms at 16.7 M keys)
             k_rs_onesweep<RS_ITEMS><<<nt, RS_BLOCK, 0, s>>>(
                 n, 8 * p, kin, vin, kout, vout, hist + p * RS_BINS,
-                status + (u64) p * nt * RS_BINS, ctr + p, p == 0 ? 1 : 0, err);
+                status + (u64) p * nt * RS_BINS, ctr + p, (p == 0 && !vals_given) ? 1 : 0, err);
             std::swap(kin, kout);
             std::swap(vin, vout);
         }
@@ -602,9 +628,26 @@ namespace shamx {
 
     u32 *radix_sort_error_word(void *scratch) { return (u32 *) scratch + 4 * RS_BINS + 4; }
 
-    void tree_stage_sort(hipStream_t s, TreeBuffers &t) {
-        radix_sort_pairs(s, t.n_obj, 30, t.sorted.data(), t.sort_index.data(),
-                         t.keys_alt.data(), t.vals_alt.data(), t.scratch.data());
+    void tree_stage_sort(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
+                         const f64 *z) {
+        const u32 n = t.n_obj;
+        if (t.fine_ties) {
+            // runs of equal codes ordered by the position inside the cell: a stable sort by
+            // the sub-cell code, then a stable sort of that order by the code
+            k_fine_key<<<div_up(n, 256), 256, 0, s>>>(n, x, y, z, t.bbox6.data(),
+                                                       t.split_a.data());
+            radix_sort_pairs(s, n, 30, t.split_a.data(), t.split_b.data(), t.keys_alt.data(),
+                             t.vals_alt.data(), t.scratch.data(), false);
+            SHAMX_HIP(hipMemcpyAsync(t.counters.data() + 3,
+                                     radix_sort_error_word(t.scratch.data()), sizeof(u32),
+                                     hipMemcpyDeviceToDevice, s));
+            k_gather_u32<<<div_up(n, 256), 256, 0, s>>>(n, t.codes.data(), t.split_b.data(),
+                                                         t.sorted.data());
+            SHAMX_HIP(hipMemcpyAsync(t.sort_index.data(), t.split_b.data(), n * sizeof(u32),
+                                     hipMemcpyDeviceToDevice, s));
+        }
+        radix_sort_pairs(s, n, 30, t.sorted.data(), t.sort_index.data(), t.keys_alt.data(),
+                         t.vals_alt.data(), t.scratch.data(), t.fine_ties);
         // keep the sort's error word before the scratch is reused by the scans
         SHAMX_HIP(hipMemcpyAsync(t.counters.data() + 2, radix_sort_error_word(t.scratch.data()),
                                  sizeof(u32), hipMemcpyDeviceToDevice, s));
@@ -621,6 +664,9 @@ namespace shamx {
         u32 *total = t.counters.data() + 6;
         for (u32 it = 1; it <= reduction_level; it++) {
             exclusive_scan_u32(s, sin, t.red_cnt.data(), n, total, t.scratch.data());
+            if (it == 1) // distinct codes (the split flags mark code changes)
+                SHAMX_HIP(hipMemcpyAsync(t.counters.data() + 1, total, sizeof(u32),
+                                         hipMemcpyDeviceToDevice, s));
             k_compact<<<g, 256, 0, s>>>(n, sin, t.red_cnt.data(), t.red_pos.data());
             k_reduc_iter<<<g, 256, 0, s>>>(n, t.sorted.data(), sin, t.red_cnt.data(),
                                           t.red_pos.data(), total, sout);
@@ -629,10 +675,15 @@ namespace shamx {
         // sin holds the final split table; sout is free for the offsets
         exclusive_scan_u32(s, sin, sout, n, t.counters.data(), t.scratch.data());
         k_compact<<<g, 256, 0, s>>>(n, sin, sout, t.reduc_map.data());
-        u32 hc[3];
-        SHAMX_HIP(hipMemcpyAsync(hc, t.counters.data(), 3 * sizeof(u32), hipMemcpyDeviceToHost, s));
+        u32 hc[4];
+        SHAMX_HIP(hipMemcpyAsync(hc, t.counters.data(), 4 * sizeof(u32), hipMemcpyDeviceToHost, s));
         SHAMX_HIP(hipStreamSynchronize(s));
-        if (hc[2]) throw std::runtime_error("radix sort: look-back or scatter inconsistency");
+        if (hc[2] | hc[3]) throw std::runtime_error("radix sort: look-back or scatter inconsistency");
+        // the next builds order runs of equal codes spatially when codes repeat on average
+        // more than 1.5 times (clustered sets whose cells hold many particles); unique codes
+        // leave nothing to order
+        if (reduction_level > 0)
+            t.fine_ties = t.allow_fine_ties && (u64) hc[1] * 3 < (u64) n * 2;
         u32 L = hc[0];
         if (L == 0) throw std::runtime_error("0 leaf tree cannot exists");
         t.L = L;
@@ -698,7 +749,7 @@ namespace shamx {
     void tree_build(hipStream_t s, TreeBuffers &t, u32 n, const f64 *x, const f64 *y,
                     const f64 *z, const f64 *h, u32 reduction_level, f64 htol) {
         tree_stage_codes(s, t, n, x, y, z);
-        tree_stage_sort(s, t);
+        tree_stage_sort(s, t, x, y, z);
         tree_stage_finish(s, t, x, y, z, h, reduction_level, htol);
     }
 
