@@ -382,6 +382,8 @@ def main():
                     help="skip the differentiable Kelvin-Helmholtz workload (64^3)")
     ap.add_argument("--clustered-only", action="store_true",
                     help="run only the clustered workload and print its JSON (profiling)")
+    ap.add_argument("--disc-only", action="store_true",
+                    help="run only the Keplerian disc workload and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU design "
@@ -394,6 +396,9 @@ def main():
     _lib.lib()
     if args.clustered_only:
         print(json.dumps(scale_probe()))
+        return
+    if args.disc_only:
+        print(json.dumps(disc_4m()))
         return
     cfg, fields = setups.sedov(args.n_side)
     n = fields["x"].size
