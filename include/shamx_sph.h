@@ -230,14 +230,20 @@ int shamx_solver_density_force(shamx_solver *s, shamx_step_log *log);
 int shamx_solver_stream(shamx_solver *s, void **stream);
 
 /* ---- adjoint (no reference counterpart: the reference has no adjoint) ----------------------
- * Vector-Jacobian product of the last evolve_once (FP64, adiabatic or isothermal EOS, M4 / M6 /
- * M8 / C2 kernels, one corrector pass, dt held fixed): seeds11 holds 11 device arrays of N
- * doubles, the adjoints of (x, y, z, vx, vy, vz, u, ax, ay, az, du) at the end of the step,
- * replaced in place by those at its start.  The smoothing length is differentiated through its
- * density equation (implicit function theorem). */
+ * Vector-Jacobian product of the last evolve_once (FP64, constant AV, adiabatic or isothermal
+ * EOS, or LP07 with free boundaries, M4 / M6 / M8 / C2 kernels, one corrector pass, dt held
+ * fixed): seeds11 holds 11 device arrays of N doubles, the adjoints of (x, y, z, vx, vy, vz,
+ * u, ax, ay, az, du) at the end of the step, replaced in place by those at its start.  The
+ * smoothing length is differentiated through its density equation (implicit function
+ * theorem).  With sinks (free boundaries), the sink-gas and sink-sink forces and the sinks'
+ * kicks and drift are reversed too; the sinks' own seeds (position, velocity, acceleration
+ * from the gas) are held by the solver from step to step of a backward pass, which
+ * shamx_solver_adjoint_density starts (it sets them to zero): the gradients of the particle
+ * seeds then include the paths through the sinks. */
 int shamx_solver_adjoint_step(shamx_solver *s, double *const *seeds11);
 /* adds to xbar, ybar, zbar (device, N) the position gradient of sum_a rho_bar_a rho_a at the
- * current state, rho_a = m (hfact / h_a)^3 with h_a from the density equation */
+ * current state, rho_a = m (hfact / h_a)^3 with h_a from the density equation; starts a
+ * backward pass (the sinks' seeds are set to zero) */
 int shamx_solver_adjoint_density(shamx_solver *s, const double *rho_bar, double *xbar,
                                  double *ybar, double *zbar);
 /* transient products of the last step (copies to HOST arrays sized by shamx_solver_sizes):

@@ -3,7 +3,9 @@ adjoint (shamx_solver_adjoint_step / _density, shamrock_amd/csrc/adjoint.hip).
 
 `rollout_density(cfg, fields, v0, u0, steps)` runs `steps` evolve_once from the state `fields`
 with initial velocities v0 (N x 3) and internal energies u0 (N) and returns the per-particle
-density rho_h(h) of the final state; gradients flow to v0 and u0.  The time steps are those the
+density rho_h(h) of the final state; gradients flow to v0 and u0.  `rollout_state` returns the
+final density and internal energy (for losses on both, e.g. an entropy sum).  Sinks (free
+boundaries) are part of the differentiated state: their seeds are carried by the solver.  The time steps are those the
 solver chooses in the forward pass (or `dts` when given) and are held fixed when
 differentiating.  Memory: one state every `checkpoint_every` steps plus one segment; the
 backward pass replays each segment from its checkpoint (the step is deterministic, so replays
@@ -63,14 +65,20 @@ class _Rollout:
         self.dts_used = used
         self.solver = s
         rho = _density(self.cfg, s.get_device("h"))
+        u = s.get_device("u")
         torch.cuda.synchronize()
-        return rho
+        return rho, u
 
-    def backward(self, rho_bar):
+    def backward(self, rho_bar, u_bar=None):
         torch = _torch()
         s, N = self.solver, self.solver.n
         seeds = [torch.zeros(N, dtype=torch.float64, device="cuda") for _ in range(11)]
+        if rho_bar is None:
+            rho_bar = torch.zeros(N, dtype=torch.float64, device="cuda")
+        # starts the backward pass (the sinks' seeds are set to zero)
         s.adjoint_density(rho_bar.contiguous(), seeds[0], seeds[1], seeds[2])
+        if u_bar is not None:
+            seeds[6] += u_bar
         for c0 in sorted(self.ckpt, reverse=True):
             seg = list(range(c0, min(c0 + self.k, self.steps)))
             s.restore(self.ckpt[c0])
@@ -90,32 +98,38 @@ class _Rollout:
 def _make_function():
     torch = _torch()
 
-    class RolloutDensity(torch.autograd.Function):
+    class RolloutState(torch.autograd.Function):
         @staticmethod
         def forward(ctx, v0, u0, cfg, fields, steps, checkpoint_every, dts):
             r = _Rollout(cfg, fields, steps, checkpoint_every, dts)
-            rho = r.forward(v0, u0)
+            rho, u = r.forward(v0, u0)
             ctx.rollout = r
-            return rho
+            return rho, u
 
         @staticmethod
-        def backward(ctx, rho_bar):
-            vbar, ubar, _ = ctx.rollout.backward(rho_bar)
+        def backward(ctx, rho_bar, u_bar):
+            vbar, ubar, _ = ctx.rollout.backward(rho_bar, u_bar)
             ctx.rollout.solver.close()
             return vbar, ubar, None, None, None, None, None
 
-    return RolloutDensity
+    return RolloutState
 
 
 _FN = None
 
 
-def rollout_density(cfg, fields, v0, u0, steps, checkpoint_every=5, dts=None):
-    """Final-state density after `steps` steps, differentiable w.r.t. v0 (N x 3) and u0 (N)."""
+def rollout_state(cfg, fields, v0, u0, steps, checkpoint_every=5, dts=None):
+    """Final-state density and internal energy after `steps` steps, differentiable w.r.t. v0
+    (N x 3) and u0 (N)."""
     global _FN
     if _FN is None:
         _FN = _make_function()
     return _FN.apply(v0, u0, cfg, fields, steps, checkpoint_every, dts)
+
+
+def rollout_density(cfg, fields, v0, u0, steps, checkpoint_every=5, dts=None):
+    """Final-state density after `steps` steps, differentiable w.r.t. v0 (N x 3) and u0 (N)."""
+    return rollout_state(cfg, fields, v0, u0, steps, checkpoint_every, dts)[0]
 
 
 def rollout_dts(cfg, fields, v0, u0, steps):
@@ -126,4 +140,4 @@ def rollout_dts(cfg, fields, v0, u0, steps):
     return r.dts_used
 
 
-__all__ = ["rollout_density", "rollout_dts", "FIELDS"]
+__all__ = ["rollout_density", "rollout_state", "rollout_dts", "FIELDS"]

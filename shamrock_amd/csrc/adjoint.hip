@@ -19,6 +19,8 @@
 #include "adjoint_pair.hpp"
 #include "launch.hpp"
 
+#include <vector>
+
 namespace shamx {
 
     namespace {
@@ -235,6 +237,52 @@ namespace shamx {
         }
 
         // add the force / Omega / IFT gradients, then the predictor adjoint
+        // Sink-gas gravity (SinkParticlesUpdate.cpp:69-150, ExternalForces.cpp:48-70) reversed
+        // for one sink: a_ext,i = -G M r_i / s_i^3 and the sink's a_gas = sum_i G m r_i / s_i^3,
+        // r_i = x_i - p, s_i = sqrt(|r_i|^2 + eps^2) (|r_i| without softening), both zero inside
+        // the accretion radius.  With J_i = d(r / s^3)/dr = I / s^3 - 3 r r^T / s^5 (symmetric):
+        // xbar_i += J_i (G m abar_gas - G M abar_i) and pbar -= the sum over i of the same (per
+        // block into part[3 * block + k]).
+        struct SinkVjp {
+            f64 px, py, pz, M, racc, G, m, soft, bx, by, bz;
+        };
+        __global__ void __launch_bounds__(256)
+            k_sink_gas_vjp(u32 n, const f64 *__restrict x, const f64 *__restrict y,
+                           const f64 *__restrict z, SinkVjp k, const f64 *__restrict fax,
+                           const f64 *__restrict fay, const f64 *__restrict faz,
+                           f64 *__restrict gx, f64 *__restrict gy, f64 *__restrict gz,
+                           f64 *__restrict part) {
+            __shared__ f64 lds[3][256 / WAVE];
+            const u32 i = blockIdx.x * 256 + threadIdx.x;
+            f64 c[3]    = {0, 0, 0};
+            if (i < n) {
+                const f64 r[3] = {x[i] - k.px, y[i] - k.py, z[i] - k.pz};
+                const f64 d2   = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                if (!(sqrt(d2) < k.racc)) {
+                    const f64 s2 = k.soft == 0 ? d2 : d2 + k.soft * k.soft;
+                    const f64 sv = sqrt(s2), s3 = sv * s2, s5 = s3 * s2;
+                    const f64 w[3] = {k.G * k.m * k.bx - k.G * k.M * fax[i],
+                                      k.G * k.m * k.by - k.G * k.M * fay[i],
+                                      k.G * k.m * k.bz - k.G * k.M * faz[i]};
+                    const f64 rw = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
+                    for (int t = 0; t < 3; t++) c[t] = w[t] / s3 - 3 * r[t] * rw / s5;
+                    gx[i] += c[0];
+                    gy[i] += c[1];
+                    gz[i] += c[2];
+                }
+            }
+            for (int t = 0; t < 3; t++) {
+                f64 v = wave_reduce(c[t], [](f64 p, f64 q) { return p + q; });
+                if (__lane_id() == 0) lds[t][threadIdx.x / WAVE] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x < 3) {
+                f64 v = 0;
+                for (int wv = 0; wv < 256 / WAVE; wv++) v += lds[threadIdx.x][wv];
+                part[3 * (size_t) blockIdx.x + threadIdx.x] = -v;
+            }
+        }
+
         __global__ void k_predictor_vjp(u32 n, f64 dt, StepSeeds s, ForceGrad g) {
             u32 i = blockIdx.x * blockDim.x + threadIdx.x;
             if (i >= n) return;
@@ -279,7 +327,7 @@ namespace shamx {
     }
 
     void adjoint_step(hipStream_t s, const AdjointArgs &A, f64 *const *seeds,
-                      DevArray<f64> &work) {
+                      DevArray<f64> &work, const SinkAdjoint *sk) {
         const u32 N = A.n, nm = A.nm;
         if (N == 0) return;
         // work: 4 force seeds (local) + 4 sorted force seeds + 9 gradients + 2 sorted (Omega
@@ -331,6 +379,29 @@ namespace shamx {
             k_ift_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(N, rc, A.ps, s_lam, A.mass, g.gx,
                                                            g.gy, g.gz);
         });
+        if (sk && sk->count) {
+            // the force seeds of the local particles are also those of their external
+            // accelerations (a = hydro + a_ext); the gas gradients add to the positions'
+            const u32 nb = div_up(N, 256u);
+            DevArray<f64> part(3 * (size_t) nb);
+            std::vector<f64> hp(3 * (size_t) nb);
+            for (u32 k = 0; k < sk->count; k++) {
+                SinkVjp a{sk->pos[3 * k],  sk->pos[3 * k + 1],      sk->pos[3 * k + 2],
+                          sk->mass[k],     sk->racc[k],             sk->G,
+                          A.mass,          sk->soft,                sk->abar_gas[3 * k],
+                          sk->abar_gas[3 * k + 1], sk->abar_gas[3 * k + 2]};
+                k_sink_gas_vjp<<<nb, 256, 0, s>>>(N, sk->x, sk->y, sk->z, a, fa[0], fa[1], fa[2],
+                                                  g.gx, g.gy, g.gz, part.data());
+                SHAMX_HIP(hipMemcpyAsync(hp.data(), part.data(), hp.size() * sizeof(f64),
+                                         hipMemcpyDeviceToHost, s));
+                SHAMX_HIP(hipStreamSynchronize(s));
+                for (int t = 0; t < 3; t++) {
+                    f64 acc = 0;
+                    for (u32 b = 0; b < nb; b++) acc += hp[3 * (size_t) b + t];
+                    sk->pbar[3 * k + t] = acc;
+                }
+            }
+        }
         k_predictor_vjp<<<div_up(N, 256), 256, 0, s>>>(N, dt, ss, g);
         SHAMX_HIP(hipGetLastError());
     }

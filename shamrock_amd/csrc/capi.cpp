@@ -131,6 +131,10 @@ struct shamx_solver {
         f64 pos[3], vel[3], acc_sph[3], acc_ext[3], mass, angmom[3], racc;
     };
     std::vector<Sink> sinks;
+    std::vector<f64> sink_pos_start; // sink positions at the start of the last step
+    // adjoint seeds of the sinks carried between steps of a backward pass: 9 per sink,
+    // (position, velocity, acceleration from the gas)
+    std::vector<f64> adj_sink;
     DevArray<f64> axe, aye, aze;
     DevArray<u64> sink_red;
     DevArray<u32> ghost_src, gwork;
@@ -336,6 +340,9 @@ struct shamx_solver {
         f64 box6[6] = {cfg.box_min[0], cfg.box_min[1], cfg.box_min[2],
                        cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
         const bool has_sinks = !sinks.empty();
+        sink_pos_start.resize(3 * sinks.size());
+        for (size_t k = 0; k < sinks.size(); k++)
+            for (int c = 0; c < 3; c++) sink_pos_start[3 * k + c] = sinks[k].pos[c];
         if (has_sinks) { // sink predictor: SinkParticlesUpdate.cpp:23-46 (Solver.cpp:2017)
             sink_ext_forces();
             sink_kick(dtc_);
@@ -627,7 +634,8 @@ struct shamx_solver {
              "LP07 with free boundaries (periodic images sit at other radii)");
         need(!cfg.mixed_precision, "adjoint: the step must run in FP64");
         need(cfg.av_kind == 0, "adjoint: available for the constant artificial viscosity");
-        need(sinks.empty(), "adjoint: not available with sinks");
+        need(sinks.empty() || !cfg.periodic,
+             "adjoint: sinks only with free boundaries (the sink forces see unwrapped positions)");
         f64 hfact = 0;
         dispatch_kernel(cfg.kernel, [&](auto K) { hfact = decltype(K)::hfactd; });
         return AdjointArgs{N,
@@ -655,6 +663,75 @@ struct shamx_solver {
                            nb.ps.data(),
                            h.data(),
                            omega.data()};
+    }
+
+    // sink-sink gravity reversed (SinkParticlesUpdate.cpp:155-200): a_ext,i = -sum_j G M_j
+    // r_ij / (|r_ij|^3 + 1e-9); pbar += (d a_ext / d p)^T abar at the positions pos
+    void sink_ext_vjp(const f64 *pos, const f64 *abar, f64 *pbar) const {
+        const f64 eps = 1e-9;
+        const size_t S = sinks.size();
+        for (size_t i = 0; i < S; i++)
+            for (size_t j = 0; j < S; j++) {
+                if (i == j) continue;
+                f64 r[3], v[3];
+                for (int c = 0; c < 3; c++) {
+                    r[c] = pos[3 * i + c] - pos[3 * j + c];
+                    v[c] = abar[3 * i + c];
+                }
+                f64 d   = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                f64 den = d * d * d + eps;
+                f64 rv  = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
+                f64 gm  = cfg.constant_G * sinks[j].mass;
+                for (int c = 0; c < 3; c++) {
+                    f64 jv = v[c] / den - 3 * d * r[c] * rv / (den * den);
+                    pbar[3 * i + c] -= gm * jv;
+                    pbar[3 * j + c] += gm * jv;
+                }
+            }
+    }
+
+    // the adjoint of the last step with its sinks: the sink kick of the corrector, the
+    // sink-gas and sink-sink forces at the drifted positions, the sink predictor (kick, drift),
+    // around the gas adjoint; adj_sink carries (pbar, wbar, abar_gas) from step to step
+    void adjoint_step_with_sinks(f64 *const *seeds11) {
+        const size_t S = sinks.size();
+        const f64 dt   = last.dt;
+        if (adj_sink.size() != 9 * S) adj_sink.assign(9 * S, 0.);
+        std::vector<f64> pmid(3 * S), mass(S), racc(S), ab_gas(3 * S), ab_ext(3 * S),
+            pbar_gas(3 * S, 0.), pbar(3 * S), wbar(3 * S);
+        for (size_t k = 0; k < S; k++) {
+            mass[k] = sinks[k].mass;
+            racc[k] = sinks[k].racc;
+            for (int c = 0; c < 3; c++) {
+                pmid[3 * k + c] = sinks[k].pos[c];
+                f64 wb          = adj_sink[9 * k + 3 + c];
+                // corrector kick w_end = w_half + dt/2 (a_gas + a_ext)
+                ab_gas[3 * k + c] = adj_sink[9 * k + 6 + c] + dt / 2 * wb;
+                ab_ext[3 * k + c] = dt / 2 * wb;
+                pbar[3 * k + c]   = adj_sink[9 * k + c];
+                wbar[3 * k + c]   = wb;
+            }
+        }
+        SinkAdjoint sa{(u32) S,         pmid.data(), mass.data(), racc.data(),
+                       cfg.constant_G,  cfg.sink_softening,       ab_gas.data(),
+                       pbar_gas.data(), x.data(),    y.data(),    z.data()};
+        adjoint_step(stream, adjoint_args(), seeds11, adj_work, &sa);
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < 3 * S; i++) pbar[i] += pbar_gas[i];
+        sink_ext_vjp(pmid.data(), ab_ext.data(), pbar.data());
+        // predictor: p_mid = p_old + dt w_half, w_half = w_old + dt/2 (a_gas,old + a_ext,old)
+        std::vector<f64> ab_old(3 * S);
+        for (size_t i = 0; i < 3 * S; i++) {
+            wbar[i] += dt * pbar[i];
+            ab_old[i] = dt / 2 * wbar[i];
+        }
+        sink_ext_vjp(sink_pos_start.data(), ab_old.data(), pbar.data());
+        for (size_t k = 0; k < S; k++)
+            for (int c = 0; c < 3; c++) {
+                adj_sink[9 * k + c]     = pbar[3 * k + c];
+                adj_sink[9 * k + 3 + c] = wbar[3 * k + c];
+                adj_sink[9 * k + 6 + c] = ab_old[3 * k + c];
+            }
     }
 
     ~shamx_solver() {
@@ -1101,7 +1178,8 @@ int shamx_solver_adjoint_step(shamx_solver *s, double *const *seeds11) {
              "adjoint: the last step reran its corrector (lower the CFL)");
         for (int k = 0; k < 11; k++) need(seeds11[k] != nullptr, "adjoint: null seed array");
         SHAMX_HIP(hipDeviceSynchronize()); // seeds may still be being written on another stream
-        adjoint_step(s->stream, s->adjoint_args(), seeds11, s->adj_work);
+        if (s->sinks.empty()) adjoint_step(s->stream, s->adjoint_args(), seeds11, s->adj_work);
+        else s->adjoint_step_with_sinks(seeds11);
         SHAMX_HIP(hipStreamSynchronize(s->stream));
     });
 }
@@ -1112,6 +1190,7 @@ int shamx_solver_adjoint_density(shamx_solver *s, const double *rho_bar, double 
         need(s && rho_bar && xbar && ybar && zbar, "shamx_solver_adjoint_density: null argument");
         SHAMX_HIP(hipDeviceSynchronize()); // inputs may still be being written on another stream
         adjoint_density(s->stream, s->adjoint_args(), rho_bar, xbar, ybar, zbar, s->adj_work);
+        s->adj_sink.assign(9 * s->sinks.size(), 0.); // a backward pass starts here
         SHAMX_HIP(hipStreamSynchronize(s->stream));
     });
 }

@@ -241,10 +241,22 @@ namespace shamx {
         const f64 *h;     // final h, local particles
         const f64 *omega; // final Omega, merged (local particles then their images)
     };
+    // the sinks of a step for its adjoint (free boundaries): the positions the sink-gas forces
+    // used, masses, accretion radii and force constants (host), the seeds of the sinks'
+    // accelerations from the gas (host, 3 per sink, in), the gradient w.r.t. the sinks'
+    // positions from the sink-gas terms (host, 3 per sink, out), the gas positions (device)
+    struct SinkAdjoint {
+        u32 count = 0;
+        const f64 *pos = nullptr, *mass = nullptr, *racc = nullptr;
+        f64 G = 0, soft = 0;
+        const f64 *abar_gas = nullptr;
+        f64 *pbar = nullptr;
+        const f64 *x = nullptr, *y = nullptr, *z = nullptr;
+    };
     // seeds (11 device arrays of n: x y z vx vy vz u ax ay az du) at the end of the step are
-    // replaced by the seeds at its start
+    // replaced by the seeds at its start; with sinks, their sink-gas terms are reversed too
     void adjoint_step(hipStream_t s, const AdjointArgs &A, f64 *const *seeds,
-                      DevArray<f64> &work);
+                      DevArray<f64> &work, const SinkAdjoint *sk = nullptr);
     // adds to (gx, gy, gz) the position gradient of sum_a rho_bar_a rho_h(h_a(x))
     void adjoint_density(hipStream_t s, const AdjointArgs &A, const f64 *rho_bar, f64 *gx,
                          f64 *gy, f64 *gz, DevArray<f64> &work);
