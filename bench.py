@@ -273,6 +273,48 @@ def disc_4m(steps=50, warmup=2):
                                      "du": relL2(out[1][1], out[0][1])}}
 
 
+def disc_adjoint_4m(steps=50, checkpoint_every=5):
+    """The disc workload's derivative at full size: 4,194,304 particles, LP07, central sink,
+    `steps` FP64 steps at CFL 0.1 (one corrector pass per step, which the adjoint needs), a
+    checkpoint every `checkpoint_every`; loss = the final entropy sum sum_a u_a /
+    rho_a^(gamma - 1); wall time of the forward rollout and of the backward pass, and the norm
+    of the gradient w.r.t. the initial radial velocities (the gradient projected on r / |r|
+    in the disc plane)."""
+    import torch
+
+    from shamrock_amd import setups
+    from shamrock_amd.adjoint import rollout_state
+
+    cfg, f = setups.disc(4_194_304, seed=2)
+    cfg = dict(cfg, cfl_cour=0.1, cfl_force=0.1)
+    gm1 = cfg["gamma"] - 1
+    v0 = torch.tensor(np.stack([f["vx"], f["vy"], f["vz"]], 1), device="cuda",
+                      requires_grad=True)
+    u0 = torch.tensor(f["u"], device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rho, u = rollout_state(cfg, f, v0, u0, steps, checkpoint_every=checkpoint_every)
+    loss = (u / rho ** gm1).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = v0.grad.cpu().numpy()
+    r = np.hypot(f["x"], f["y"])
+    g_rad = (g[:, 0] * f["x"] + g[:, 1] * f["y"]) / r
+    n = v0.shape[0]
+    log(f"[bench] disc adjoint 4M, {steps} steps: forward {t1 - t0:.2f} s, backward "
+        f"{t2 - t1:.2f} s")
+    return {"workload": "keplerian_disc_4194304_M4_LP07_sink", "n_particles": n,
+            "steps": steps, "checkpoint_every": checkpoint_every, "dtype": "f64",
+            "loss": "final entropy sum, sum u / rho^(gamma - 1)", "loss_value": loss.item(),
+            "forward_s": t1 - t0, "backward_s": t2 - t1,
+            "grad_radial_velocity_norm": float(np.linalg.norm(g_rad)),
+            "check": "central differences at 20,000 particles: "
+                     "tests/test_gpu_adjoint.py::test_adjoint_disc_with_sink"}
+
+
 def differentiable_kh64(steps=10, checkpoint_every=5):
     """The differentiable Kelvin-Helmholtz workload: 64^3 (262,444 particles), M4, 10 FP64 steps,
     a checkpoint every 5; loss = mean squared difference of the final density from a target
@@ -384,6 +426,8 @@ def main():
                     help="run only the clustered workload and print its JSON (profiling)")
     ap.add_argument("--disc-only", action="store_true",
                     help="run only the Keplerian disc workload and print its JSON (profiling)")
+    ap.add_argument("--disc-adjoint-only", action="store_true",
+                    help="run only the disc's adjoint rollout and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU design "
@@ -399,6 +443,9 @@ def main():
         return
     if args.disc_only:
         print(json.dumps(disc_4m()))
+        return
+    if args.disc_adjoint_only:
+        print(json.dumps(disc_adjoint_4m()))
         return
     cfg, fields = setups.sedov(args.n_side)
     n = fields["x"].size
@@ -473,6 +520,8 @@ def main():
         out["differentiable_kh64"] = differentiable_kh64()
     if not args.no_disc:
         out["disc_4M"] = disc_4m()
+        if not args.no_adjoint:
+            out["disc_adjoint_4M"] = disc_adjoint_4m()
     if not args.no_scale_probe:
         probe = scale_probe()
         out["plummer_16M_pass"] = probe.pop("pass_plummer_16M")
