@@ -439,6 +439,8 @@ def main():
 
     _lib.lib()
     if args.clustered_only:
+        if args.copy_probe:
+            copy_probe()
         print(json.dumps(scale_probe()))
         return
     if args.disc_only:

@@ -550,7 +550,22 @@ namespace shamx {
                 // leaf of the batch; candidates are the concatenated ranges in order
                 u32 k  = kb + lane;
                 u32 r0 = 0, r1 = 0;
-                if (k < k1) node_children(nodes, leaf_ids[k], r0, r1); // leaf: sorted range
+                if (k < k1) {
+                    const u32 nd = leaf_ids[k];
+                    node_children(nodes, nd, r0, r1); // leaf: sorted range
+                    // the whole range is dropped when none of its particles can reach a
+                    // member: the leaf's box is farther from the members' box than both the
+                    // members' largest cut radius and the leaf's (rint = max h * htol).  The
+                    // box distance bounds each particle's from below, so this drops only
+                    // candidates the per-particle test below would drop
+                    const f64 *rec = nodes + (u64) nd * NODE_F64;
+                    const f64 ex   = fmax(fmax(rec[0] - bhx, blx - rec[3]), 0.);
+                    const f64 ey   = fmax(fmax(rec[1] - bhy, bly - rec[4]), 0.);
+                    const f64 ez   = fmax(fmax(rec[2] - bhz, blz - rec[5]), 0.);
+                    const f64 d2   = (ex * ex + ey * ey + ez * ez) * (1.0 - 1e-12);
+                    const f64 rr2  = rec[6] * rec[6] * Rker2;
+                    if (d2 > br2 && d2 > rr2) r1 = r0;
+                }
                 u32 sz  = r1 - r0;
                 u32 inc = sz;
                 for (int o = 1; o < WAVE; o <<= 1) {
@@ -570,7 +585,8 @@ namespace shamx {
                     {
                         const u32 c = c0 + lane;
                         // the range holding candidate c: largest j with rexcl[j] <= c
-                        // (leaves are never empty, so c - rexcl[j] < that range's size)
+                        // (a dropped range is empty and shares its rexcl with the next one, so
+                        // the largest such j is a range that holds c)
                         u32 j = 0;
                         for (u32 st = WAVE / 2; st > 0; st >>= 1) {
                             u32 t = j + st;
