@@ -100,6 +100,7 @@ namespace shamx {
         DevArray<u32> inv;                              // sorted position of each object
         DevArray<u32> mism;                             // owner leaf != own leaf
         DevArray<u32> nbatch, bstart, item_leaf;        // stage-2 wave items (member batches)
+        DevArray<f64> bbox; // member batch boxes (batch-level cull on clustered sets)
         // rows past their fixed capacity: flags, counts, positions, list (overflow region)
         DevArray<u32> oflag, ocnt, ofpos, opos, olist;
         bool leaf_overflowed = false; // the last build had leaf lists past their capacity

@@ -469,7 +469,7 @@ namespace shamx {
         // LIST: one wave per particle of list (item_leaf holds particle ids, n_items their
         // count): its whole row at offset[its sorted position], the path of rows longer than
         // their fixed capacity.
-        template<bool FILL, bool SORTED, bool STRIDE, bool LIST = false>
+        template<bool FILL, bool SORTED, bool STRIDE, bool LIST = false, bool BCULL = false>
         __global__ void __launch_bounds__(GRP_WAVES * WAVE)
             k_group_neigh(u32 n, u32 L, u32 I, const f64 *__restrict nodes,
                           const u32 *__restrict sort_index, const double4 *__restrict ps,
@@ -478,8 +478,12 @@ namespace shamx {
                           const u32 *__restrict item_leaf, const u32 *__restrict bstart,
                           const u32 *__restrict n_items, f64 htol, f64 Rker2,
                           u32 *__restrict cnt, u32 *__restrict offset, u32 *__restrict ids,
-                          u32 cap, u32 leaf_cap, const u32 *__restrict inv = nullptr) {
+                          u32 cap, u32 leaf_cap, const u32 *__restrict inv = nullptr,
+                          const f64 *__restrict bbox = nullptr) {
             __shared__ u32 rstart[GRP_WAVES][WAVE], rexcl[GRP_WAVES][WAVE];
+            // BCULL: the neighbour leaves of one pass (range, first batch, batches before it)
+            __shared__ u32 lr0[GRP_WAVES][BCULL ? WAVE : 1], lr1[GRP_WAVES][BCULL ? WAVE : 1];
+            __shared__ u32 lbs[GRP_WAVES][BCULL ? WAVE : 1], lbx[GRP_WAVES][BCULL ? WAVE : 1];
             __shared__ double4 cpos[GRP_WAVES][WAVE]; // candidate x, y, z, cut radius^2
             __shared__ u32 cid[GRP_WAVES][WAVE];      // candidate id written to the rows
             __shared__ double4 mpos[GRP_WAVES][WAVE]; // member x, y, z, cut radius^2
@@ -545,39 +549,9 @@ namespace shamx {
             const double4 me = mpos[wid][on ? mi : 0];
             const u32 mrow0  = mrow[wid][on ? mi : 0];
             u32 cursor = mrow0, count = 0;
-            for (u32 kb = k0; kb < k1; kb += WAVE) {
-                // neighbour leaves in batches of up to 64: lane i loads the range of the i-th
-                // leaf of the batch; candidates are the concatenated ranges in order
-                u32 k  = kb + lane;
-                u32 r0 = 0, r1 = 0;
-                if (k < k1) {
-                    const u32 nd = leaf_ids[k];
-                    node_children(nodes, nd, r0, r1); // leaf: sorted range
-                    // the whole range is dropped when none of its particles can reach a
-                    // member: the leaf's box is farther from the members' box than both the
-                    // members' largest cut radius and the leaf's (rint = max h * htol).  The
-                    // box distance bounds each particle's from below, so this drops only
-                    // candidates the per-particle test below would drop
-                    const f64 *rec = nodes + (u64) nd * NODE_F64;
-                    const f64 ex   = fmax(fmax(rec[0] - bhx, blx - rec[3]), 0.);
-                    const f64 ey   = fmax(fmax(rec[1] - bhy, bly - rec[4]), 0.);
-                    const f64 ez   = fmax(fmax(rec[2] - bhz, blz - rec[5]), 0.);
-                    const f64 d2   = (ex * ex + ey * ey + ez * ez) * (1.0 - 1e-12);
-                    const f64 rr2  = rec[6] * rec[6] * Rker2;
-                    if (d2 > br2 && d2 > rr2) r1 = r0;
-                }
-                u32 sz  = r1 - r0;
-                u32 inc = sz;
-                for (int o = 1; o < WAVE; o <<= 1) {
-                    u32 t = __shfl_up(inc, o, WAVE);
-                    if (lane >= o) inc += t;
-                }
-                const u32 total = __shfl(inc, WAVE - 1, WAVE);
-                const u32 nr    = min(k1 - kb, (u32) WAVE);
-                wave_lds_sync();
-                rstart[wid][lane] = r0;
-                rexcl[wid][lane]  = inc - sz;
-                wave_lds_sync();
+            // the candidates of the ranges rstart / rexcl [0, nr), `total` in all: staged 64 at
+            // a time, tested against the members
+            auto candidates = [&](const u32 nr, const u32 total) {
                 for (u32 c0 = 0; c0 < total; c0 += WAVE) {
                     // stage 64 candidates: lane j takes candidate c0 + j; the ones that can
                     // reach a member are compacted to the front of cpos / cid
@@ -636,6 +610,95 @@ namespace shamx {
                         count += kc;
                     }
                 }
+            };
+            for (u32 kb = k0; kb < k1; kb += WAVE) {
+                // neighbour leaves in batches of up to 64: lane i loads the range of the i-th
+                // leaf of the batch; candidates are the concatenated ranges in order
+                u32 k  = kb + lane;
+                u32 r0 = 0, r1 = 0, nd = 0;
+                if (k < k1) {
+                    nd = leaf_ids[k];
+                    node_children(nodes, nd, r0, r1); // leaf: sorted range
+                    // the whole range is dropped when none of its particles can reach a
+                    // member: the leaf's box is farther from the members' box than both the
+                    // members' largest cut radius and the leaf's (rint = max h * htol).  The
+                    // box distance bounds each particle's from below, so this drops only
+                    // candidates the per-particle test below would drop
+                    const f64 *rec = nodes + (u64) nd * NODE_F64;
+                    const f64 ex   = fmax(fmax(rec[0] - bhx, blx - rec[3]), 0.);
+                    const f64 ey   = fmax(fmax(rec[1] - bhy, bly - rec[4]), 0.);
+                    const f64 ez   = fmax(fmax(rec[2] - bhz, blz - rec[5]), 0.);
+                    const f64 d2   = (ex * ex + ey * ey + ez * ez) * (1.0 - 1e-12);
+                    const f64 rr2  = rec[6] * rec[6] * Rker2;
+                    if (d2 > br2 && d2 > rr2) r1 = r0;
+                }
+                const u32 nr = min(k1 - kb, (u32) WAVE);
+                if constexpr (!BCULL) {
+                    u32 sz  = r1 - r0;
+                    u32 inc = sz;
+                    for (int o = 1; o < WAVE; o <<= 1) {
+                        u32 t = __shfl_up(inc, o, WAVE);
+                        if (lane >= o) inc += t;
+                    }
+                    const u32 total = __shfl(inc, WAVE - 1, WAVE);
+                    wave_lds_sync();
+                    rstart[wid][lane] = r0;
+                    rexcl[wid][lane]  = inc - sz;
+                    wave_lds_sync();
+                    candidates(nr, total);
+                } else {
+                    // each range split into the member batches of its leaf (64 sorted
+                    // positions from the leaf's start; bbox: their boxes and largest cut
+                    // radius^2), every batch dropped whole by the test above
+                    const bool okl = nd >= I && nd - I < L;
+                    const u32 nbl  = okl ? (r1 - r0 + WAVE - 1) / WAVE : 0u;
+                    const u32 bs0  = nbl ? bstart[nd - I] : 0u;
+                    u32 binc       = nbl;
+                    for (int o = 1; o < WAVE; o <<= 1) {
+                        u32 t = __shfl_up(binc, o, WAVE);
+                        if (lane >= o) binc += t;
+                    }
+                    const u32 totb = __shfl(binc, WAVE - 1, WAVE);
+                    wave_lds_sync();
+                    lr0[wid][lane] = r0;
+                    lr1[wid][lane] = r1;
+                    lbs[wid][lane] = bs0;
+                    lbx[wid][lane] = binc - nbl;
+                    wave_lds_sync();
+                    for (u32 t0 = 0; t0 < totb; t0 += WAVE) {
+                        const u32 tb = t0 + lane;
+                        u32 s0 = 0, s1 = 0;
+                        if (tb < totb) {
+                            // the range holding batch tb: largest j with lbx[j] <= tb
+                            u32 j = 0;
+                            for (u32 st = WAVE / 2; st > 0; st >>= 1) {
+                                u32 q = j + st;
+                                if (q < nr && lbx[wid][q] <= tb) j = q;
+                            }
+                            const u32 bi = tb - lbx[wid][j];
+                            s0           = lr0[wid][j] + bi * WAVE;
+                            s1           = min(s0 + WAVE, lr1[wid][j]);
+                            const f64 *bx = bbox + (u64) (lbs[wid][j] + bi) * 8;
+                            const f64 ex  = fmax(fmax(bx[0] - bhx, blx - bx[3]), 0.);
+                            const f64 ey  = fmax(fmax(bx[1] - bhy, bly - bx[4]), 0.);
+                            const f64 ez  = fmax(fmax(bx[2] - bhz, blz - bx[5]), 0.);
+                            const f64 d2  = (ex * ex + ey * ey + ez * ez) * (1.0 - 1e-12);
+                            if (d2 > br2 && d2 > bx[6]) s1 = s0;
+                        }
+                        const u32 sz = s1 - s0;
+                        u32 inc      = sz;
+                        for (int o = 1; o < WAVE; o <<= 1) {
+                            u32 t = __shfl_up(inc, o, WAVE);
+                            if (lane >= o) inc += t;
+                        }
+                        const u32 total = __shfl(inc, WAVE - 1, WAVE);
+                        wave_lds_sync();
+                        rstart[wid][lane] = s0;
+                        rexcl[wid][lane]  = inc - sz;
+                        wave_lds_sync();
+                        candidates(min(totb - t0, (u32) WAVE), total);
+                    }
+                }
             }
             // lane mi (< M) holds member mi's count; the member lane of rank mi writes it
             wave_lds_sync();
@@ -664,6 +727,39 @@ namespace shamx {
             if (l >= L) return;
             const u32 b0 = bstart[l], nbl = nbatch[l];
             for (u32 j = 0; j < nbl; j++) item_leaf[b0 + j] = l;
+        }
+        // box of every member batch (all its sorted positions, images included: they are
+        // candidates too) and the largest cut radius^2 among them, 8 doubles per batch:
+        // lo x, y, z, hi x, y, z, (h htol)^2 R^2, unused.  One wave per batch.
+        __global__ void __launch_bounds__(GRP_WAVES * WAVE)
+            k_batch_boxes(u32 I, const f64 *__restrict nodes, const u32 *__restrict item_leaf,
+                          const u32 *__restrict bstart, const u32 *__restrict n_items,
+                          const double4 *__restrict ps, f64 htol, f64 Rker2,
+                          f64 *__restrict bbox) {
+            const int lane = __lane_id();
+            const u32 item = blockIdx.x * GRP_WAVES + threadIdx.x / WAVE;
+            if (item >= *n_items) return;
+            const u32 l = item_leaf[item];
+            u32 m0, m1;
+            node_children(nodes, I + l, m0, m1);
+            const u32 sm = m0 + (item - bstart[l]) * WAVE + lane;
+            constexpr f64 BIG = 1.7976931348623157e308;
+            f64 lx = BIG, ly = BIG, lz = BIG, hx = -BIG, hy = -BIG, hz = -BIG, r2 = 0;
+            if (sm < m1) {
+                const double4 q = ps[sm];
+                const f64 rb    = q.w * htol;
+                lx = hx = q.x, ly = hy = q.y, lz = hz = q.z;
+                r2 = rb * rb * Rker2;
+            }
+            auto fmn = [](f64 p, f64 q) { return fmin(p, q); };
+            auto fmx = [](f64 p, f64 q) { return fmax(p, q); };
+            lx = wave_reduce(lx, fmn), ly = wave_reduce(ly, fmn), lz = wave_reduce(lz, fmn);
+            hx = wave_reduce(hx, fmx), hy = wave_reduce(hy, fmx), hz = wave_reduce(hz, fmx);
+            r2 = wave_reduce(r2, fmx);
+            if (lane < 8) {
+                const f64 v[8] = {lx, ly, lz, hx, hy, hz, r2, 0.};
+                bbox[(u64) item * 8 + lane] = v[lane];
+            }
         }
     } // namespace
 
@@ -713,6 +809,30 @@ namespace shamx {
                 throw std::runtime_error("neighbour cache: a leaf walk disagrees with its count");
         }
 
+        // boxes of the member batches for the batch-level cull of stage 2, on sets whose tree
+        // sort took the spatial tie order (runs of equal codes: leaves of hundreds of
+        // particles, where a leaf-level test keeps most candidates); SHAMX_BATCH_CULL=1 forces
+        // it (tests).  nullptr: no batch cull.
+        const f64 *batch_boxes(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t,
+                               u32 gblocks, f64 htol, f64 Rker2) {
+            const char *e = std::getenv("SHAMX_BATCH_CULL");
+            if (!t.fine_ties && !(e && std::atoi(e) == 1)) return nullptr;
+            nb.bbox.resize((size_t) gblocks * GRP_WAVES * 8);
+            k_batch_boxes<<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+                t.I, t.nodes.data(), nb.item_leaf.data(), nb.bstart.data(),
+                nb.counters.data() + 7, nb.ps.data(), htol, Rker2, nb.bbox.data());
+            return nb.bbox.data();
+        }
+        // stage-2 launch, with the batch-level cull when bb holds the batch boxes
+        template<bool FILL, bool SORTED, bool STRIDE, class... A>
+        void launch_group(hipStream_t s, u32 gblocks, const f64 *bb, A... a) {
+            if (bb)
+                k_group_neigh<FILL, SORTED, STRIDE, false, true>
+                    <<<gblocks, GRP_WAVES * WAVE, 0, s>>>(a..., nullptr, bb);
+            else
+                k_group_neigh<FILL, SORTED, STRIDE><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(a...);
+        }
+
         // packed rows, two passes (count, scan, fill); SORTED: rows by sorted position
         // (leaf_cap: capacity of fixed-capacity leaf lists, 0 for packed ones)
         template<bool SORTED>
@@ -725,7 +845,8 @@ namespace shamx {
             u32 gblocks   = batch_items(s, nb, t);
             const u32 *n_items = nb.counters.data() + 7;
             if (SORTED) fill_u32(s, nb.cnt.data(), 0, rows); // ghost rows stay empty
-            k_group_neigh<false, SORTED, false><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+            const f64 *bb = batch_boxes(s, nb, t, gblocks, htol, Rker2);
+            launch_group<false, SORTED, false>(s, gblocks, bb,
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
                 nb.item_leaf.data(), nb.bstart.data(), n_items, htol, Rker2, nb.cnt.data(),
@@ -748,7 +869,7 @@ namespace shamx {
             nb.n_pairs    = hc[1];
             nb.n_mismatch = hc[5];
             nb.ids.resize(std::max<u64>(nb.n_pairs, 1));
-            k_group_neigh<true, SORTED, false><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+            launch_group<true, SORTED, false>(s, gblocks, bb,
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
                 nb.item_leaf.data(), nb.bstart.data(), n_items, htol, Rker2, nullptr,
@@ -981,7 +1102,7 @@ namespace shamx {
             // stage 2: fixed-capacity particle rows
             nb.ids.resize(std::max<u64>((u64) nm * cp, 1));
             fill_u32(s, nb.cnt.data(), 0, nm); // ghost rows stay empty
-            k_group_neigh<true, true, true><<<gblocks, GRP_WAVES * WAVE, 0, s>>>(
+            launch_group<true, true, true>(s, gblocks, batch_boxes(s, nb, t, gblocks, htol, Rker2),
                 n_local, L, I, t.nodes.data(), t.sort_index.data(), nb.ps.data(),
                 nb.owner.data(), nb.leaf_cnt.data(), nb.leaf_offset.data(), nb.leaf_ids.data(),
                 nb.item_leaf.data(), nb.bstart.data(), nb.counters.data() + 7, htol, Rker2,

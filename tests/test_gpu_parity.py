@@ -495,7 +495,7 @@ def test_density_force_pass_parity(workload):
     g.close()
 
 
-@pytest.mark.parametrize("mode", ["packed_rows", "overflow"])
+@pytest.mark.parametrize("mode", ["packed_rows", "overflow", "batch_cull", "batch_cull_packed"])
 @pytest.mark.parametrize("workload", ["disc_20000", "plummer_6000", "sedov_16"])
 def test_neighbour_layouts_parity(workload, mode, monkeypatch):
     """The neighbour-cache layouts that the workloads of these tests do not reach by
@@ -504,11 +504,15 @@ def test_neighbour_layouts_parity(workload, mode, monkeypatch):
     at fixed capacity.  overflow (SHAMX_NEIGH_CAP=16): capacities of 16, so that nearly every
     leaf list and row runs past its capacity and is filled again into the overflow regions
     (the path of the few outlier rows of a disc whose tail particles have 20x the median h).
-    Whole steps (or passes, Plummer) vs the oracle, lists included.  (The disc needs 20,000
+    batch_cull (SHAMX_BATCH_CULL=1, alone and with packed rows): stage 2 drops candidates
+    batch by batch (64 sorted positions of a leaf, on their boxes), the path of clustered
+    sets.  Whole steps (or passes, Plummer) vs the oracle, lists included.  (The disc needs 20,000
     particles for its first step's h iteration to converge, in the oracle too.)"""
-    if mode == "packed_rows":
+    if mode in ("packed_rows", "batch_cull_packed"):
         monkeypatch.setenv("SHAMX_NEIGH_ROWS", "packed")
-    else:
+    if mode.startswith("batch_cull"):
+        monkeypatch.setenv("SHAMX_BATCH_CULL", "1")
+    if mode == "overflow":
         monkeypatch.setenv("SHAMX_NEIGH_CAP", "16")
     name, n = workload.rsplit("_", 1)
     if name == "disc":
