@@ -375,14 +375,15 @@ struct shamx_solver {
             b = timer.begin(stream);
             FieldPtrs fp{{h.data(), u.data(), vx.data(), vy.data(), vz.data(), omega.data()}, 6};
             gather_ghosts(stream, N, G, ghost_src.data(), fp);
-            // EOS of the merged set fused with the sorted force records
+            // EOS of the merged set fused with the sorted force records (a corrector rerun
+            // rewrites only the velocity, u, P and c_s of the FP64 records)
             frec.resize(force_record_bytes(cfg.mixed_precision) * (size_t) (N + G));
             pack_force(stream, cfg.kernel, N + G, tree.sort_index.data(), x.data(), y.data(),
                        z.data(), h.data(), vx.data(), vy.data(), vz.data(), u.data(),
                        omega.data(), P.data(), cs.data(), frec.data(), cfg.gpart_mass,
                        EosArgs{cfg.eos_kind, cfg.gamma, cfg.eos_cs, cfg.eos_cs0, cfg.eos_q,
                                cfg.eos_r0},
-                       cfg.mixed_precision);
+                       cfg.mixed_precision, corr_iter > 0);
             timer.end(stream, PH_EOS, b);
             if (cfg.av_kind == 1) {
                 // Solver.cpp:2370-2470: div v, curl v, d(div v)/dt from the previous

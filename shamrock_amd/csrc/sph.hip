@@ -549,8 +549,9 @@ namespace shamx {
             }
         };
 
-        // EOS on the merged set fused with the record build: record s takes particle perm[s]
-        template<class K, bool MIXED>
+        // EOS on the merged set fused with the record build: record s takes particle perm[s].
+        // REFRESH (FP64 records): rewrite only the fields that a corrector pass changes
+        template<class K, bool MIXED, bool REFRESH = false>
         __global__ void k_pack_force(u32 nm, const u32 *__restrict perm, const f64 *__restrict x,
                                      const f64 *__restrict y, const f64 *__restrict z,
                                      const f64 *__restrict h, const f64 *__restrict vx,
@@ -561,13 +562,19 @@ namespace shamx {
             u32 s = blockIdx.x * blockDim.x + threadIdx.x;
             if (s >= nm) return;
             u32 b  = perm ? perm[s] : s;
-            f64 hb = h[b], ub = u[b], xb = x[b], yb = y[b], zb = z[b];
+            f64 hb = h[b], ub = u[b], xb = 0, yb = 0, zb = 0;
+            if (!REFRESH || e.kind == 2) xb = x[b], yb = y[b], zb = z[b];
             f64 rho = rho_h(mass, hb, (f64) K::hfactd);
             f64 p, c;
             eos_eval(e, rho, ub, xb, yb, zb, p, c);
             P[b]  = p;
             cs[b] = c;
-            if constexpr (MIXED)
+            if constexpr (REFRESH) {
+                // corrector reruns: positions, h and omega are those of the full build, only
+                // v and u (and with u the EOS) changed; the other fields are already in place
+                FRec *r = (FRec *) rec + s;
+                r->vx = vx[b], r->vy = vy[b], r->vz = vz[b], r->u = ub, r->P = p, r->cs = c;
+            } else if constexpr (MIXED)
                 ((FRec32 *) rec)[s] =
                     make_frec32<K>(xb, yb, zb, hb, vx[b], vy[b], vz[b], ub, p, c, omega[b], mass);
             else
@@ -1376,11 +1383,14 @@ namespace shamx {
     void pack_force(hipStream_t s, int kernel, u32 nm, const u32 *perm, const f64 *x,
                     const f64 *y, const f64 *z, const f64 *h, const f64 *vx, const f64 *vy,
                     const f64 *vz, const f64 *u, const f64 *omega, f64 *P, f64 *cs, void *rec,
-                    f64 mass, EosArgs e, int mixed) {
+                    f64 mass, EosArgs e, int mixed, bool refresh) {
         if (nm == 0) return;
         dispatch_kernel(kernel, [&](auto K) {
             using KT = decltype(K);
-            if (mixed)
+            if (refresh && !mixed)
+                k_pack_force<KT, false, true><<<div_up(nm, 256), 256, 0, s>>>(
+                    nm, perm, x, y, z, h, vx, vy, vz, u, omega, P, cs, rec, mass, e);
+            else if (mixed)
                 k_pack_force<KT, true><<<div_up(nm, 256), 256, 0, s>>>(
                     nm, perm, x, y, z, h, vx, vy, vz, u, omega, P, cs, rec, mass, e);
             else
