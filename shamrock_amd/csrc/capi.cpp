@@ -376,7 +376,7 @@ struct shamx_solver {
             FieldPtrs fp{{h.data(), u.data(), vx.data(), vy.data(), vz.data(), omega.data()}, 6};
             gather_ghosts(stream, N, G, ghost_src.data(), fp);
             // EOS of the merged set fused with the sorted force records (a corrector rerun
-            // rewrites only the velocity, u, P and c_s of the FP64 records)
+            // rewrites only the velocity, u, P and c_s of the records)
             frec.resize(force_record_bytes(cfg.mixed_precision) * (size_t) (N + G));
             pack_force(stream, cfg.kernel, N + G, tree.sort_index.data(), x.data(), y.data(),
                        z.data(), h.data(), vx.data(), vy.data(), vz.data(), u.data(),

@@ -164,7 +164,7 @@ namespace shamx {
     size_t force_record_bytes(int mixed);
     // EOS on nm merged particles fused with the force-record build; record s describes
     // particle perm[s] (identity when null).  P and cs are written per particle id.
-    // refresh: the FP64 records of the same particles were built by the last call and only
+    // refresh: the records of the same particles were built by the last call and only
     // v and u changed since (corrector reruns): only those fields, P and c_s are rewritten
     void pack_force(hipStream_t s, int kernel, u32 nm, const u32 *perm, const f64 *x,
                     const f64 *y, const f64 *z, const f64 *h, const f64 *vx, const f64 *vy,

@@ -550,7 +550,7 @@ namespace shamx {
         };
 
         // EOS on the merged set fused with the record build: record s takes particle perm[s].
-        // REFRESH (FP64 records): rewrite only the fields that a corrector pass changes
+        // REFRESH: rewrite only the fields that a corrector pass changes
         template<class K, bool MIXED, bool REFRESH = false>
         __global__ void k_pack_force(u32 nm, const u32 *__restrict perm, const f64 *__restrict x,
                                      const f64 *__restrict y, const f64 *__restrict z,
@@ -569,9 +569,13 @@ namespace shamx {
             eos_eval(e, rho, ub, xb, yb, zb, p, c);
             P[b]  = p;
             cs[b] = c;
-            if constexpr (REFRESH) {
+            if constexpr (REFRESH && MIXED) {
                 // corrector reruns: positions, h and omega are those of the full build, only
                 // v and u (and with u the EOS) changed; the other fields are already in place
+                FRec32 *r = (FRec32 *) rec + s;
+                r->vx = (f32) vx[b], r->vy = (f32) vy[b], r->vz = (f32) vz[b];
+                r->u = (f32) ub, r->P = (f32) p, r->cs = (f32) c;
+            } else if constexpr (REFRESH) {
                 FRec *r = (FRec *) rec + s;
                 r->vx = vx[b], r->vy = vy[b], r->vz = vz[b], r->u = ub, r->P = p, r->cs = c;
             } else if constexpr (MIXED)
@@ -1387,7 +1391,10 @@ namespace shamx {
         if (nm == 0) return;
         dispatch_kernel(kernel, [&](auto K) {
             using KT = decltype(K);
-            if (refresh && !mixed)
+            if (refresh && mixed)
+                k_pack_force<KT, true, true><<<div_up(nm, 256), 256, 0, s>>>(
+                    nm, perm, x, y, z, h, vx, vy, vz, u, omega, P, cs, rec, mass, e);
+            else if (refresh)
                 k_pack_force<KT, false, true><<<div_up(nm, 256), 256, 0, s>>>(
                     nm, perm, x, y, z, h, vx, vy, vz, u, omega, P, cs, rec, mass, e);
             else if (mixed)
