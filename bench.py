@@ -173,7 +173,7 @@ def plummer_pass(n, pos, hh, reps=3):
     from shamrock_amd import setups
     from shamrock_amd.solver import GpuSolver
 
-    res = {}
+    res, acc = {}, {}
     for mixed in (0, 1):
         cfg, f = setups.plummer(n, seed=1, r_max=30.0, pos_h=(pos, hh))
         cfg["mixed_precision"] = mixed
@@ -187,17 +187,27 @@ def plummer_pass(n, pos, hh, reps=3):
                 f"h iterations {lg['h_iterations']}, pairs {lg['n_pairs']}")
             if k:
                 logs.append(lg)
+            else:
+                first = lg
         med = lambda key: float(np.median([lg[key] for lg in logs]))  # noqa: E731
         ph = {p: med("t_" + p) * 1e3 for p in ("sort", "tree", "neigh", "hiter", "omega",
                                                 "eos", "force")}
         tot = med("t_total")
         res["mixed" if mixed else "fp64"] = {
             "ms": tot * 1e3, "particles_per_s": n / tot, "phases_ms": ph,
+            "first_pass_ms": first["t_total"] * 1e3,
             "h_iterations": int(logs[-1]["h_iterations"]), "n_pairs": int(logs[-1]["n_pairs"]),
             "n_leaf": int(logs[-1]["n_leaf"])}
+        acc[mixed] = (np.stack([g.get(k) for k in ("ax", "ay", "az")]), g.get("du"))
         g.close()
         del g
         torch.cuda.empty_cache()
+    # same input and pass sequence in both precisions: the mixed pass's a and du/dt against the
+    # FP64 pass's (h differs slightly too, from the FP32 density sums); no time integration,
+    # so no energy drift
+    res["mixed_vs_fp64"] = {"rel_l2_accel": relL2(acc[1][0], acc[0][0]),
+                            "rel_l2_du": relL2(acc[1][1], acc[0][1]),
+                            "energy_drift": "n/a: a single density/force pass"}
     return res
 
 
@@ -374,35 +384,138 @@ def load_traffic(workload):
 
 
 def cpu_info():
+    """lscpu model, sockets, cores and threads, the CPUs this process may run on and the
+    frequency governor."""
     info = {"model": platform.processor() or "unknown"}
+    keys = {"Model name:": "model", "Socket(s):": "sockets",
+            "Core(s) per socket:": "cores_per_socket", "Thread(s) per core:": "threads_per_core",
+            "CPU(s):": "cpus"}
     try:
         txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
         for line in txt.splitlines():
-            if line.startswith("Model name:"):
-                info["model"] = line.split(":", 1)[1].strip()
+            for k, name in keys.items():
+                if line.startswith(k):
+                    info[name] = line.split(":", 1)[1].strip()
     except Exception:
         pass
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/devices/system/cpu/cpu0/cpufreq/scaling_governor") as f:
+            info["governor"] = f.read().strip()
+    except OSError:
+        info["governor"] = "unknown"
     return info
 
 
-def cpu_baseline(cfg, fields, threads):
-    """Oracle (C++ restatement of the reference step, OpenMP) on the same workload."""
+def cpu_baseline(cfg, fields, threads, steps=3):
+    """Oracle (C++ restatement of the reference step, OpenMP, every CPU this process may use)
+    on the same workload: the dt = 0 step untimed, then the median of `steps` timed steps."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
 
     s = orc.Solver(cfg, fields)
     t0 = time.perf_counter()
-    first = s.evolve_once()  # dt = 0 step (not timed)
+    s.evolve_once()  # dt = 0 step (not timed)
     t1 = time.perf_counter()
-    st = s.evolve_once()
-    t2 = time.perf_counter()
+    times = []
+    for _ in range(steps):
+        a = time.perf_counter()
+        s.evolve_once()
+        times.append(time.perf_counter() - a)
+    med = float(np.median(times))
     n = s.n
-    return {"value": n / (t2 - t1), "unit": "particle-updates/s", "cores": threads,
-            "kind": "port",
+    ci = cpu_info()
+    return {"value": n / med, "unit": "particle-updates/s", "cores": threads,
+            "kind": "port", "cpu": ci,
             "sample": f"oracle evolve_once on the same Sedov 128^3 M6 FP64 workload: the dt=0 "
-                      f"step ({t1 - t0:.1f} s, untimed) then one timed step ({t2 - t1:.1f} s), "
-                      f"OpenMP {threads} threads, {cpu_info()['model']}"}
+                      f"step ({t1 - t0:.1f} s, untimed), then the median of {steps} timed steps "
+                      f"({', '.join(f'{t:.2f}' for t in times)} s), OpenMP {threads} threads "
+                      f"(all CPUs of this process), {ci['model']}"}
+
+
+def mixed_report(name, cfg, fields, steps):
+    """FP64 and mixed runs of `steps` steps from the same initial state: the relative drift of
+    the total energy sum m (u + v^2 / 2) over each run, and the rel-L2 error of a and du/dt of
+    one mixed step against one FP64 step from the FP64 run's final state (the FP64 GPU step is
+    the oracle's to 1e-10, which the parity tests check)."""
+    from shamrock_amd.solver import GpuSolver
+
+    out = {"workload": name, "steps": steps}
+    g64 = None
+    for mixed in (0, 1):
+        g = GpuSolver(dict(cfg, mixed_precision=mixed), fields)
+        e = [g.evolve_once()["energy"] for _ in range(steps)]
+        out["energy_drift_rel_" + ("mixed" if mixed else "fp64")] = abs(e[-1] - e[0]) / abs(e[0])
+        if mixed:
+            g.close()
+        else:
+            g64 = g
+    err_a, err_du = mixed_error(cfg, g64)
+    g64.close()
+    out["rel_l2_accel_vs_fp64"], out["rel_l2_du_vs_fp64"] = err_a, err_du
+    log(f"[bench] mixed vs FP64 {name}: a {err_a:.2e}, du {err_du:.2e}")
+    return out
+
+
+def homogeneous_box(n_target_base=2**25, replays=10):
+    """The reference's own benchmark (examples/benchmarks/sph_homogeneous_benchmark.py), run
+    through this library's Python surface with the script's calls: M4, Cullen & Dehnen
+    viscosity, periodic box from get_ideal_hcp_box around [-0.6, 0.6]^3, HCP lattice sized for
+    N_target_base = 2^25 particles (the script's choice for a GPU of 288 GB and for the H100 of
+    the published result), u = 1 injected with add_kernel_value at the origin (r = 16 dr), CFL
+    0.1, one timestep() to converge h, then `replays` timesteps at dt = 0; the rate is the best
+    of them, as the script reports it."""
+    import shamrock_amd as shamrock
+
+    N_target = n_target_base
+    bmin, bmax = (-0.6, -0.6, -0.6), (0.6, 0.6, 0.6)
+    vol_b = (bmax[0] - bmin[0]) * (bmax[1] - bmin[1]) * (bmax[2] - bmin[2])
+    part_vol = vol_b / N_target
+    part_vol_lattice = 0.74 * part_vol
+    dr = (part_vol_lattice / ((4.0 / 3.0) * 3.1416)) ** (1.0 / 3.0)
+    ctx = shamrock.Context()
+    ctx.pdata_layout_new()
+    model = shamrock.get_Model_SPH(context=ctx, vector_type="f64_3", sph_kernel="M4")
+    cfg = model.gen_default_config()
+    cfg.set_artif_viscosity_VaryingCD10(alpha_min=0.0, alpha_max=1, sigma_decay=0.1, alpha_u=1,
+                                        beta_AV=2)
+    cfg.set_boundary_periodic()
+    cfg.set_eos_adiabatic(5.0 / 3.0)
+    model.set_solver_config(cfg)
+    model.init_scheduler(int(2e7), 1)
+    bmin, bmax = shamrock.math.get_ideal_hcp_box(dr, bmin, bmax)
+    model.resize_simulation_box(bmin, bmax)
+    setup = model.get_setup()
+    setup.apply_setup(setup.make_generator_lattice_hcp(dr, bmin, bmax))
+    vol_b = (bmax[0] - bmin[0]) * (bmax[1] - bmin[1]) * (bmax[2] - bmin[2])
+    pmass = model.total_mass_to_part_mass(1 * vol_b)
+    model.set_value_in_a_box("uint", "f64", 0, bmin, bmax)
+    model.add_kernel_value("uint", "f64", 1, (0, 0, 0), 16 * dr)
+    model.set_particle_mass(pmass)
+    model.set_cfl_cour(0.1)
+    model.set_cfl_force(0.1)
+    t0 = time.perf_counter()
+    model.timestep()
+    log(f"[bench] homogeneous box N={model.get_total_part_count()}: first step "
+        f"{time.perf_counter() - t0:.2f} s")
+    rates = []
+    for _ in range(replays):
+        model.set_next_dt(0.0)
+        model.timestep()
+        rates.append(model.solver_logs_last_rate())
+    n = model.solver_logs_last_obj_count()
+    best = max(rates)
+    log(f"[bench] homogeneous box: best {best:.4e} part/s, step {n / best * 1e3:.1f} ms")
+    model.ctx.model = None
+    del model
+    return {"workload": "sph_homogeneous_benchmark (M4, CD10, HCP, periodic, dt = 0 replays)",
+            "n_particles": n, "value": best, "unit": "particle-updates/s",
+            "step_ms": n / best * 1e3, "rates": rates,
+            "published": {"H100_x1": 25502056, "n_particles": 33848064,
+                          "source": "examples/benchmarks/sph_homogeneous_bench_result.json:"
+                                    "244-287"},
+            "ratio_vs_published_H100": best / 25502056}
 
 
 def main():
@@ -412,7 +525,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-side", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)))
+    ap.add_argument("--no-homogeneous", action="store_true",
+                    help="skip the reference's homogeneous-box benchmark (33.8 M particles)")
+    ap.add_argument("--no-config-errors", action="store_true",
+                    help="skip the mixed-vs-FP64 reports of the sound wave and KH workloads")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 512 MiB device copy (counter calibration)")
@@ -481,7 +598,10 @@ def main():
     t_force = sum(lg["t_force"] for lg in logs64)
     n_fe = sum(lg["n_force_evals"] for lg in logs64)
     pairs = logs64[-1]["n_interacting"]
-    force_tfs = FORCE_FLOPS_PER_PAIR * pairs * n_fe / t_force / 1e12 if t_force > 0 else 0.0
+    # every force evaluation of a step sees the same pairs (positions and h are fixed once the
+    # corrector loop starts): flops = 120 x sum over steps of pairs x evaluations
+    pair_evals = sum(lg["n_interacting"] * lg["n_force_evals"] for lg in logs64)
+    force_tfs = FORCE_FLOPS_PER_PAIR * pair_evals / t_force / 1e12 if t_force > 0 else 0.0
     phases = {k: sum(lg[k] for lg in logs64) / args.steps * 1e3
               for k in ("t_ghost", "t_tree", "t_sort", "t_neigh", "t_hiter", "t_omega", "t_eos",
                         "t_force", "t_corr")}
@@ -491,13 +611,12 @@ def main():
                  "frac": sort_gbs / HBM_PEAK_GBS, "traffic": traffic.get("sort"),
                  "avg_ms": t_sort / max(n_sorts, 1) * 1e3,
                  "algorithmic": f"{SORT_BYTES_PER_KEY} B x {nm} keys"},
-        "force": {"bound": "mfma", "achieved": force_tfs, "peak": FP64_PEAK_TFS,
+        "force": {"bound": "valu", "achieved": force_tfs, "peak": FP64_PEAK_TFS,
                   "unit": "TFLOP/s", "frac": force_tfs / FP64_PEAK_TFS,
                   "traffic": traffic.get("force"),
                   "avg_ms": t_force / max(n_fe, 1) * 1e3,
                   "algorithmic": f"{FORCE_FLOPS_PER_PAIR} FP64 flop x {pairs} interacting pairs",
-                  "note": "FP64 VALU loop; the MI355X FP64 vector peak equals its FP64 matrix "
-                          "peak (78.6 TF/s)"},
+                  "note": "FP64 VALU loop (no MFMA); the MI355X FP64 vector peak is 78.6 TF/s"},
     }
     dominant = "force" if t_force >= t_sort else "sort"
     roof = {k: kernels[dominant][k] for k in ("bound", "achieved", "peak", "unit", "frac",
@@ -528,6 +647,16 @@ def main():
         probe = scale_probe()
         out["plummer_16M_pass"] = probe.pop("pass_plummer_16M")
         kernels.update(probe)
+    if not args.no_config_errors:
+        # mixed precision against FP64 on every workload of the brief: B (Sedov) above, E
+        # (disc) in disc_4M, C (Plummer) in plummer_16M_pass, A and D here
+        cfg_a, f_a = setups.sound_wave(32)
+        cfg_d, f_d = setups.kelvin_helmholtz(64)
+        out["mixed_vs_fp64_sound_wave_32"] = mixed_report("sound_wave_32^3_M4", cfg_a, f_a, 5)
+        out["mixed_vs_fp64_kelvin_helmholtz_64"] = mixed_report(
+            "kelvin_helmholtz_64^3_M4", cfg_d, f_d, 10)
+    if not args.no_homogeneous:
+        out["homogeneous_box_33M"] = homogeneous_box()
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline (oracle, {args.cpu_threads} threads)")
         out["cpu_baseline"] = cpu_baseline(cfg, fields, args.cpu_threads)

@@ -65,6 +65,12 @@ typedef struct shamx_config {
                               6.6743015e-11 with no unit system, 1 in code units with G = 1) */
     double sink_softening; /* Plummer softening of the sink-gas force; 0 = the reference's
                               unsoftened G M d / |d|^3 (SinkParticlesUpdate.cpp:127-141) */
+    int32_t keep_storage_order; /* 0: at the start of each step the solver stores the local
+                              particles in the order of the last tree's Morton sort (the
+                              reference's ParticleReordering, Solver.cpp:2043-2048, done every
+                              step); every field and result keeps the caller's particle order
+                              at the interface.  1: keep the caller's order in storage too
+                              (required by the adjoint entries) */
 } shamx_config;
 
 /* Per-step log (reference TimestepLog Solver.hpp:43-54 + per-phase timers).  Times in s. */
@@ -76,6 +82,7 @@ typedef struct shamx_step_log {
     uint64_t n_pairs;       /* neighbour-cache entries */
     uint64_t n_interacting; /* pairs inside max(h_a, h_b) R (last force evaluation) */
     double energy, eps_v, next_dt, time, dt;
+    double t_reorder;       /* the storage reorder at the start of the step */
 } shamx_step_log;
 
 typedef struct shamx_ctx shamx_ctx;
@@ -101,6 +108,15 @@ int shamx_morton(shamx_ctx *ctx, void *stream, uint32_t n, const double *x, cons
  * stable LSD radix sort; keys sorted in place, perm[i] = original index of sorted key i */
 int shamx_sort(shamx_ctx *ctx, void *stream, uint32_t n, uint32_t key_bits, uint32_t *keys,
                uint32_t *perm);
+/* No counterpart in the reference: shamx_sort for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and stream semantics as shamx_sort, without a context: the alternate key
+ * and value buffers and the sort's scratch live in one static device array of the library
+ * (about 36 MB per GPU, held until the process ends), so n is at most 4194304 keys
+ * (SHAMX_E_INVALID_ARG beyond).  The call allocates nothing.  Every call on one GPU uses that
+ * GPU's array: calls that overlap on one GPU (two streams, two threads) race. */
+int shamx_sort_noalloc(void *stream, uint32_t n, uint32_t key_bits, uint32_t *keys,
+                       uint32_t *perm);
 /* replaces CompressedLeafBVH::rebuild_from_positions (CompressedLeafBVH.cpp:78-95) +
  * compute_presteps_rint (Solver.cpp:1322-1356); tree kept in ctx */
 int shamx_tree_build(shamx_ctx *ctx, void *stream, uint32_t n, const double *x, const double *y,
@@ -252,7 +268,8 @@ int shamx_solver_adjoint_density(shamx_solver *s, const double *rho_bar, double 
  * u32 ids: 0 ghost_src (G), 1 sorted codes (Nm), 2 sort_index (Nm), 3 reduc_map (L+2),
  *          4 owner (n), 5 cnt (n), 6 offset (n), 7 ids (P), 8 leaf_cnt (L), 9 leaf_offset (L),
  *          10 leaf_ids (PL), 11 lchild (I), 12 rchild (I), 13 endrange (I), 14 codes (Nm)
- * u8 ids: 0 lflag (I), 1 rflag (I) */
+ * u8 ids: 0 lflag (I), 1 rflag (I), 2 the force records of the last force evaluation, in
+ *          sorted order (n_merged records of 144 B, FP64, or 64 B, mixed precision) */
 int shamx_solver_sizes(shamx_solver *s, uint64_t *out6); /* {Nm, G, L, I, P, PL} */
 int shamx_solver_get_f64(shamx_solver *s, int what, double *dst);
 int shamx_solver_get_u32(shamx_solver *s, int what, uint32_t *dst);

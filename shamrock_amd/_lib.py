@@ -36,6 +36,7 @@ class ShamxConfig(C.Structure):
         ("mixed_precision", C.c_int32), ("av_kind", C.c_int32),
         ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("sigma_decay", C.c_double),
         ("constant_G", C.c_double), ("sink_softening", C.c_double),
+        ("keep_storage_order", C.c_int32),
     ]
 
 
@@ -52,7 +53,7 @@ class ShamxStepLog(C.Structure):
         ("n_force_evals", C.c_uint32),
         ("n_pairs", C.c_uint64), ("n_interacting", C.c_uint64),
         ("energy", C.c_double), ("eps_v", C.c_double), ("next_dt", C.c_double),
-        ("time", C.c_double), ("dt", C.c_double),
+        ("time", C.c_double), ("dt", C.c_double), ("t_reorder", C.c_double),
     ]
 
     def as_dict(self):
@@ -73,6 +74,7 @@ SIGNATURES = {
     "shamx_bbox": (_i, [_P, _P, _u32, _P, _P, _P, _P]),
     "shamx_morton": (_i, [_P, _P, _u32, _P, _P, _P, _P, _P]),
     "shamx_sort": (_i, [_P, _P, _u32, _u32, _P, _P]),
+    "shamx_sort_noalloc": (_i, [_P, _u32, _u32, _P, _P]),
     "shamx_tree_build": (_i, [_P, _P, _u32, _P, _P, _P, _P, _u32, _d]),
     "shamx_tree_reduce_karras": (_i, [_P, _P, _u32, _P, _P, _P, _u32]),
     "shamx_tree_aabb_hmax": (_i, [_P, _P, _P, _P, _P, _P, _d]),

@@ -9,7 +9,9 @@ boundaries) are part of the differentiated state: their seeds are carried by the
 solver chooses in the forward pass (or `dts` when given) and are held fixed when
 differentiating.  Memory: one state every `checkpoint_every` steps plus one segment; the
 backward pass replays each segment from its checkpoint (the step is deterministic, so replays
-are bit-identical) and runs the adjoint of every step on the replayed intermediate products.
+are bit-identical) and runs the adjoint of every step on the replayed intermediate products;
+the last step of a segment is run once (its products are the solver's when the replay of the
+segment ends), and the last step of the rollout not at all (the forward pass left them).
 
 This extension has no counterpart in the reference.
 """
@@ -17,9 +19,11 @@ from __future__ import annotations
 
 import numpy as np
 
+from ._lib import KERNEL_IDS
+from .setups import KERNEL_HFACT
 from .solver import FIELDS, GpuSolver
 
-_HFACT = {"M4": 1.2, "M6": 1.0, "M8": 1.0, "C2": 1.4}
+_KERNEL_NAMES = {v: k for k, v in KERNEL_IDS.items()}
 
 
 def _torch():
@@ -30,7 +34,7 @@ def _torch():
 
 def _density(cfg, h):
     kernel = cfg["kernel"]
-    hf = _HFACT[kernel] if isinstance(kernel, str) else [1.2, 1.0, 1.0, 1.4][int(kernel)]
+    hf = KERNEL_HFACT[kernel if isinstance(kernel, str) else _KERNEL_NAMES[int(kernel)]]
     return cfg["gpart_mass"] * (hf / h) ** 3
 
 
@@ -46,7 +50,8 @@ def _step(solver, dt):
 
 class _Rollout:
     def __init__(self, cfg, fields, steps, checkpoint_every, dts):
-        self.cfg, self.fields, self.steps = dict(cfg), fields, int(steps)
+        # the adjoint reads the step's products in the caller's storage order
+        self.cfg, self.fields, self.steps = dict(cfg, keep_storage_order=1), fields, int(steps)
         self.k = max(1, int(checkpoint_every))
         self.dts = None if dts is None else [float(d) for d in dts]
 
@@ -79,14 +84,24 @@ class _Rollout:
         s.adjoint_density(rho_bar.contiguous(), seeds[0], seeds[1], seeds[2])
         if u_bar is not None:
             seeds[6] += u_bar
+        first = True
         for c0 in sorted(self.ckpt, reverse=True):
             seg = list(range(c0, min(c0 + self.k, self.steps)))
+            if first:
+                # the forward pass has just run the last step: its products are in place
+                s.adjoint_step(seeds)
+                seg, first = seg[:-1], False
+                if not seg:
+                    continue
             s.restore(self.ckpt[c0])
             starts = []
-            for n in seg:
+            for n in seg[:-1]:
                 starts.append(s.snapshot())
                 _step(s, self.dts_used[n])
-            for n, st in zip(reversed(seg), reversed(starts)):
+            # the last step of the segment runs once, and its adjoint reads its products
+            _step(s, self.dts_used[seg[-1]])
+            s.adjoint_step(seeds)
+            for n, st in zip(reversed(seg[:-1]), reversed(starts)):
                 s.restore(st)
                 _step(s, self.dts_used[n])  # replay: the adjoint reads this step's products
                 s.adjoint_step(seeds)

@@ -7,6 +7,7 @@
 #include "launch.hpp"
 #include "tree_layout.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -63,6 +64,13 @@ namespace {
     }
     u64 *slots() { return slot_state().v.data(); }
 
+    // tests: SHAMX_FULL_PACK=1 rebuilds the force records on corrector reruns instead of
+    // refreshing their v, u, P and c_s in place (the two must give the same bytes)
+    bool full_pack_forced() {
+        const char *e = std::getenv("SHAMX_FULL_PACK");
+        return e && e[0] == '1';
+    }
+
     // phase timing with HIP events, read back once the step has synchronised
     struct PhaseTimer {
         std::vector<hipEvent_t> pool;
@@ -101,7 +109,10 @@ namespace {
             for (auto e : pool) (void) hipEventDestroy(e);
         }
     };
-    enum Phase { PH_GHOST, PH_TREE, PH_SORT, PH_NEIGH, PH_HITER, PH_OMEGA, PH_EOS, PH_FORCE, PH_CORR, PH_N };
+    enum Phase {
+        PH_GHOST, PH_TREE, PH_SORT, PH_NEIGH, PH_HITER, PH_OMEGA, PH_EOS, PH_FORCE, PH_CORR,
+        PH_REORDER, PH_N
+    };
 
 } // namespace
 
@@ -141,6 +152,17 @@ struct shamx_solver {
     DevArray<u8> ghost_oid;
     DevArray<unsigned char> gscratch;
     DevArray<u64> red, part;
+    HLoopBuffers hloop;
+    // storage order: the local particles are kept in the order of the last tree's sort (the
+    // reference's ParticleReordering, Solver.cpp:2043-2048, here at every step), so that the
+    // per-particle gathers of the tree, records and outputs read and write nearly contiguous
+    // memory.  pid[j] = particle id (the caller's order) of storage slot j; field getters and
+    // setters and the inspection getters translate, so every result keeps the caller's order.
+    DevArray<u32> pid, pid_tmp, reo_perm, reo_flag, reo_off;
+    DevArray<unsigned char> reo_scan;
+    DevArray<f64> tmp_f64;
+    bool pid_identity = true; // pid[j] = j (nothing reordered yet)
+    bool tree_valid   = false; // tree.sort_index describes the current storage
     DevArray<unsigned char> frec; // force records, sorted order
     TreeBuffers tree;
     NeighBuffers nb;
@@ -202,6 +224,112 @@ struct shamx_solver {
                    tree.sort_index.data()};
     }
 
+    // permute every per-particle field of the local particles into the sorted order of the
+    // last tree build (one gather per field; the images are rebuilt by the step anyway)
+    void reorder_storage() {
+        if (cfg.keep_storage_order || !tree_valid || tree.n_obj != N + G || N == 0) return;
+        const u32 Nm = N + G;
+        reo_perm.resize(N);
+        reo_flag.resize(Nm);
+        reo_off.resize(Nm);
+        reo_scan.resize(std::max(reo_scan.size(), scan_scratch_bytes(Nm)));
+        local_sorted_order(stream, Nm, N, tree.sort_index.data(), reo_perm.data(),
+                           reo_flag.data(), reo_off.data(), reo_scan.data());
+        const u32 *perm = reo_perm.data();
+        for (auto *f : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &ax, &ay, &az, &du, &alpha_av,
+                        &soundspeed, &divv, &curlx, &curly, &curlz, &dtdivv, &axe, &aye, &aze}) {
+            tmp_f64.resize(f->size()); // same element count: no allocation once sizes settle
+            perm_gather_f64(stream, N, perm, f->data(), tmp_f64.data());
+            f->swap(tmp_f64);
+        }
+        if (pid_identity) {
+            std::vector<u32> iota(N);
+            for (u32 j = 0; j < N; j++) iota[j] = j;
+            pid.resize(N);
+            SHAMX_HIP(hipMemcpyAsync(pid.data(), iota.data(), N * sizeof(u32),
+                                     hipMemcpyHostToDevice, stream));
+            SHAMX_HIP(hipStreamSynchronize(stream));
+        }
+        pid_tmp.resize(N);
+        perm_gather_u32(stream, N, perm, pid.data(), pid_tmp.data());
+        pid.swap(pid_tmp);
+        pid_identity = false;
+        tree_valid   = false;
+    }
+
+    // field in the caller's particle order <-> storage order
+    void field_from_caller(DevArray<f64> &f, const f64 *src, bool on_device) {
+        if (pid_identity) {
+            SHAMX_HIP(hipMemcpyAsync(f.data(), src, (size_t) N * sizeof(f64),
+                                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                     stream));
+            return;
+        }
+        const f64 *dsrc = src;
+        if (!on_device) {
+            tmp_f64.resize(std::max<u32>(N, 1));
+            SHAMX_HIP(hipMemcpyAsync(tmp_f64.data(), src, (size_t) N * sizeof(f64),
+                                     hipMemcpyHostToDevice, stream));
+            dsrc = tmp_f64.data();
+        }
+        perm_gather_f64(stream, N, pid.data(), dsrc, f.data());
+    }
+    void field_to_caller(const DevArray<f64> &f, f64 *dst, bool on_device) {
+        if (pid_identity) {
+            SHAMX_HIP(hipMemcpyAsync(dst, f.data(), (size_t) N * sizeof(f64),
+                                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                     stream));
+            return;
+        }
+        if (on_device) {
+            perm_scatter_f64(stream, N, pid.data(), f.data(), dst);
+            return;
+        }
+        tmp_f64.resize(std::max<u32>(N, 1));
+        perm_scatter_f64(stream, N, pid.data(), f.data(), tmp_f64.data());
+        SHAMX_HIP(hipMemcpyAsync(dst, tmp_f64.data(), (size_t) N * sizeof(f64),
+                                 hipMemcpyDeviceToHost, stream));
+    }
+
+    // particle id of every storage slot (host copy)
+    std::vector<u32> host_pid() {
+        std::vector<u32> p(N);
+        if (pid_identity) {
+            for (u32 j = 0; j < N; j++) p[j] = j;
+        } else if (N) {
+            SHAMX_HIP(hipMemcpyAsync(p.data(), pid.data(), N * sizeof(u32),
+                                     hipMemcpyDeviceToHost, stream));
+            SHAMX_HIP(hipStreamSynchronize(stream));
+        }
+        return p;
+    }
+    // the label, in the caller's order, of every merged index: local slots through pid, the
+    // periodic images in the order the reference generates them (offset by offset, then by
+    // source particle id: BasicSPHGhosts.cpp:260-579), which is the order of generation when
+    // the storage holds the caller's order
+    std::vector<u32> external_labels() {
+        const u32 Nm = N + G;
+        std::vector<u32> p = host_pid(), ext(Nm);
+        for (u32 j = 0; j < N; j++) ext[j] = p[j];
+        if (G) {
+            std::vector<u32> src(G);
+            std::vector<u8> oid(G);
+            SHAMX_HIP(hipMemcpyAsync(src.data(), ghost_src.data(), G * sizeof(u32),
+                                     hipMemcpyDeviceToHost, stream));
+            SHAMX_HIP(hipMemcpyAsync(oid.data(), ghost_oid.data(), G, hipMemcpyDeviceToHost,
+                                     stream));
+            SHAMX_HIP(hipStreamSynchronize(stream));
+            std::vector<u32> order(G);
+            for (u32 g = 0; g < G; g++) order[g] = g;
+            std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+                if (oid[a] != oid[b]) return oid[a] < oid[b];
+                return p[src[a]] < p[src[b]];
+            });
+            for (u32 r = 0; r < G; r++) ext[N + order[r]] = N + r;
+        }
+        return ext;
+    }
+
     // Solver.cpp:984-1046 + BasicSPHGhosts: images of the local particles after them
     void ghosts() {
         if (!cfg.periodic) {
@@ -245,6 +373,7 @@ struct shamx_solver {
             tree_stage_finish(stream, tree, x.data(), y.data(), z.data(), h.data(),
                               (u32) cfg.tree_reduction_level, cfg.htol_up_coarse);
             timer.end(stream, PH_TREE, b);
+            tree_valid = true;
             last.n_sorts++;
             b = timer.begin(stream);
             neigh_build(stream, nb, tree, N, x.data(), y.data(), z.data(), h.data(),
@@ -256,18 +385,14 @@ struct shamx_solver {
             SHAMX_HIP(hipMemcpyAsync(h_old.data(), h.data(), N * sizeof(f64),
                                      hipMemcpyDeviceToDevice, stream));
             f64 max_eps = std::numeric_limits<f64>::max(), min_eps = 0;
-            for (int it = 0; it < cfg.h_iter_per_subcycles; it++) {
-                h_iterate(stream, cfg.kernel, N, csr(), nb.ps.data(), x.data(), y.data(),
-                          z.data(), h_old.data(), h.data(), eps.data(), cfg.gpart_mass,
-                          cfg.htol_up_coarse, cfg.htol_up_fine, red.data() + 1,
-                          cfg.mixed_precision, part);
-                u64 k[2];
-                SHAMX_HIP(hipMemcpyAsync(k, red.data() + 1, 16, hipMemcpyDeviceToHost, stream));
-                SHAMX_HIP(hipStreamSynchronize(stream));
-                max_eps = ordered_to_f64(k[0]);
-                min_eps = ordered_to_f64(k[1]);
-                last.h_iterations++;
-                if (max_eps < cfg.epsilon_h) break;
+            if (cfg.h_iter_per_subcycles > 0) {
+                HLoopResult hl = h_iterate_loop(
+                    stream, cfg.kernel, N, csr(), nb.ps.data(), h_old.data(), h.data(),
+                    eps.data(), cfg.gpart_mass, cfg.htol_up_coarse, cfg.htol_up_fine,
+                    cfg.epsilon_h, (u32) cfg.h_iter_per_subcycles, cfg.mixed_precision, hloop);
+                max_eps = hl.max_eps;
+                min_eps = hl.min_eps;
+                last.h_iterations += hl.iterations;
             }
             timer.end(stream, PH_HITER, b);
             bool converged = (max_eps < cfg.epsilon_h) && !(min_eps < 0);
@@ -285,7 +410,8 @@ struct shamx_solver {
         nb.fcnt.resize(std::max<u32>(N + G, 1));
         nb.fids.resize(std::max<size_t>(nb.ids.size(), 1)); // same row offsets as ids
         compute_omega(stream, cfg.kernel, N, csr(), nb.ps.data(), x.data(), y.data(), z.data(),
-                      h.data(), omega.data(), cfg.gpart_mass, nb.fcnt.data(), nb.fids.data());
+                      h.data(), omega.data(), cfg.gpart_mass, nb.fcnt.data(), nb.fids.data(),
+                      cfg.mixed_precision);
         timer.end(stream, PH_OMEGA, b);
     }
 
@@ -340,6 +466,9 @@ struct shamx_solver {
         f64 box6[6] = {cfg.box_min[0], cfg.box_min[1], cfg.box_min[2],
                        cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
         const bool has_sinks = !sinks.empty();
+        int b = timer.begin(stream);
+        reorder_storage();
+        timer.end(stream, PH_REORDER, b);
         sink_pos_start.resize(3 * sinks.size());
         for (size_t k = 0; k < sinks.size(); k++)
             for (int c = 0; c < 3; c++) sink_pos_start[3 * k + c] = sinks[k].pos[c];
@@ -349,7 +478,7 @@ struct shamx_solver {
             for (auto &s : sinks)
                 for (int k = 0; k < 3; k++) s.pos[k] = s.pos[k] + dtc_ * s.vel[k];
         }
-        int b = timer.begin(stream);
+        b = timer.begin(stream);
         // with sinks, the sink forces see the drifted positions before the periodic wrap
         // (Solver.cpp:2033-2040)
         int wrap_now = cfg.periodic && !has_sinks;
@@ -383,7 +512,7 @@ struct shamx_solver {
                        omega.data(), P.data(), cs.data(), frec.data(), cfg.gpart_mass,
                        EosArgs{cfg.eos_kind, cfg.gamma, cfg.eos_cs, cfg.eos_cs0, cfg.eos_q,
                                cfg.eos_r0},
-                       cfg.mixed_precision, corr_iter > 0);
+                       cfg.mixed_precision, corr_iter > 0 && !full_pack_forced());
             timer.end(stream, PH_EOS, b);
             if (cfg.av_kind == 1) {
                 // Solver.cpp:2370-2470: div v, curl v, d(div v)/dt from the previous
@@ -478,6 +607,7 @@ struct shamx_solver {
         last.t_eos    = ph[PH_EOS];
         last.t_force  = ph[PH_FORCE];
         last.t_corr   = ph[PH_CORR];
+        last.t_reorder = ph[PH_REORDER];
         last.n_local  = N;
         last.n_merged = N + G;
         last.n_leaf   = tree.L;
@@ -507,12 +637,15 @@ struct shamx_solver {
         SHAMX_HIP(hipStreamSynchronize(stream));
         auto t0              = std::chrono::steady_clock::now();
         const bool has_sinks = !sinks.empty();
+        int b                = timer.begin(stream);
+        reorder_storage();
+        timer.end(stream, PH_REORDER, b);
         if (has_sinks) {
             sink_ext_forces();
             sink_gas_forces();
         }
         prestep(1, false);
-        int b = timer.begin(stream);
+        b = timer.begin(stream);
         FieldPtrs fp{{h.data(), u.data(), vx.data(), vy.data(), vz.data(), omega.data()}, 6};
         gather_ghosts(stream, N, G, ghost_src.data(), fp);
         frec.resize(force_record_bytes(cfg.mixed_precision) * (size_t) (N + G));
@@ -554,6 +687,7 @@ struct shamx_solver {
         last.t_omega       = ph[PH_OMEGA];
         last.t_eos         = ph[PH_EOS];
         last.t_force       = ph[PH_FORCE];
+        last.t_reorder     = ph[PH_REORDER];
         last.n_local       = N;
         last.n_merged      = N + G;
         last.n_leaf        = tree.L;
@@ -582,6 +716,8 @@ struct shamx_solver {
     // rebuilt on the host from the sorted layout: 5 counts, 6 offsets, 7 ids
     void reference_layout_csr(int what, u32 *dst) {
         u32 Nm = N + G;
+        std::vector<u32> ext = external_labels(), pslot(N);
+        for (u32 j = 0; j < N; j++) pslot[ext[j]] = j; // particle id -> storage slot
         std::vector<u32> cnt_s(Nm), off_s(Nm), ids_s(nb.ids.size()), sidx(Nm), inv(Nm);
         SHAMX_HIP(hipMemcpyAsync(cnt_s.data(), nb.cnt.data(), Nm * 4, hipMemcpyDeviceToHost, stream));
         SHAMX_HIP(hipMemcpyAsync(off_s.data(), nb.offset.data(), Nm * 4, hipMemcpyDeviceToHost,
@@ -595,11 +731,11 @@ struct shamx_solver {
         for (u32 k = 0; k < Nm; k++) inv[sidx[k]] = k;
         u64 acc = 0;
         for (u32 a = 0; a < N; a++) {
-            u32 row = inv[a], c = cnt_s[row];
+            u32 row = inv[pslot[a]], c = cnt_s[row];
             if (what == 5) dst[a] = c;
             if (what == 6) dst[a] = (u32) acc;
             if (what == 7)
-                for (u32 k = 0; k < c; k++) dst[acc + k] = sidx[ids_s[off_s[row] + k]];
+                for (u32 k = 0; k < c; k++) dst[acc + k] = ext[sidx[ids_s[off_s[row] + k]]];
             acc += c;
         }
     }
@@ -635,6 +771,8 @@ struct shamx_solver {
              "LP07 with free boundaries (periodic images sit at other radii)");
         need(!cfg.mixed_precision, "adjoint: the step must run in FP64");
         need(cfg.av_kind == 0, "adjoint: available for the constant artificial viscosity");
+        need(pid_identity, "adjoint: the solver must keep the caller's storage order "
+                           "(keep_storage_order = 1)");
         need(sinks.empty() || !cfg.periodic,
              "adjoint: sinks only with free boundaries (the sink forces see unwrapped positions)");
         f64 hfact = 0;
@@ -807,6 +945,13 @@ int shamx_sort(shamx_ctx *ctx, void *stream, uint32_t n, uint32_t key_bits, uint
         ctx->scratch.resize(radix_sort_scratch_bytes(n));
         radix_sort_pairs((hipStream_t) stream, n, key_bits, keys, perm, ctx->alt_k.data(),
                          ctx->alt_v.data(), ctx->scratch.data());
+    });
+}
+
+int shamx_sort_noalloc(void *stream, uint32_t n, uint32_t key_bits, uint32_t *keys,
+                       uint32_t *perm) {
+    return guarded([&] {
+        radix_sort_noalloc((hipStream_t) stream, n, key_bits, keys, perm);
     });
 }
 
@@ -1067,9 +1212,7 @@ int shamx_solver_set_field(shamx_solver *s, int field, const double *src, int on
         auto *f = s->field(field);
         // a device source may still be being written on another stream
         if (on_device) SHAMX_HIP(hipDeviceSynchronize());
-        SHAMX_HIP(hipMemcpyAsync(f->data(), src, (size_t) s->N * sizeof(f64),
-                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                                 s->stream));
+        s->field_from_caller(*f, src, on_device != 0);
         SHAMX_HIP(hipStreamSynchronize(s->stream));
     });
 }
@@ -1078,9 +1221,7 @@ int shamx_solver_get_field(shamx_solver *s, int field, double *dst, int on_devic
     return guarded([&] {
         auto *f = s->field(field);
         if (on_device) SHAMX_HIP(hipDeviceSynchronize());
-        SHAMX_HIP(hipMemcpyAsync(dst, f->data(), (size_t) s->N * sizeof(f64),
-                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                                 s->stream));
+        s->field_to_caller(*f, dst, on_device != 0);
         SHAMX_HIP(hipStreamSynchronize(s->stream));
     });
 }
@@ -1217,9 +1358,18 @@ int shamx_solver_get_f64(shamx_solver *s, int what, double *dst) {
         u32 Nm = s->N + s->G, cells = s->tree.I + s->tree.L;
         DevArray<f64> *m[11] = {&s->x, &s->y, &s->z, &s->h, &s->vx, &s->vy, &s->vz,
                                 &s->u, &s->omega, &s->P, &s->cs};
-        if (what >= 0 && what <= 10) d2h(st, dst, m[what]->data(), Nm);
-        else if (what == 11) d2h(st, dst, s->omega.data(), s->N);
-        else if (what == 12) d2h(st, dst, s->vsig.data(), s->N);
+        // per-particle arrays come back in the caller's particle order (images in the
+        // reference's order after them)
+        auto relabel = [&](const f64 *src, u32 n, bool merged) {
+            std::vector<f64> tmp(n);
+            d2h(st, tmp.data(), src, n);
+            SHAMX_HIP(hipStreamSynchronize(st));
+            std::vector<u32> lab = merged ? s->external_labels() : s->host_pid();
+            for (u32 j = 0; j < n; j++) dst[lab[j]] = tmp[j];
+        };
+        if (what >= 0 && what <= 10) relabel(m[what]->data(), Nm, true);
+        else if (what == 11) relabel(s->omega.data(), s->N, false);
+        else if (what == 12) relabel(s->vsig.data(), s->N, false);
         else if (what == 13) d2h(st, dst, s->tree.bbox6.data(), 6);
         else if (what >= 14 && what <= 16) {
             std::vector<f64> nodes((size_t) cells * NODE_F64);
@@ -1235,13 +1385,33 @@ int shamx_solver_get_f64(shamx_solver *s, int what, double *dst) {
 int shamx_solver_get_u32(shamx_solver *s, int what, uint32_t *dst) {
     return guarded([&] {
         hipStream_t st = s->stream;
-        u32 Nm = s->N + s->G, L = s->tree.L, I = s->tree.I;
+        u32 Nm = s->N + s->G, L = s->tree.L, I = s->tree.I, N = s->N, G = s->G;
+        // arrays indexed by, or holding, particle indices are translated to the caller's
+        // particle order (images: the reference's order)
+        auto host = [&](const u32 *src, u32 n) {
+            std::vector<u32> v(n);
+            d2h(st, v.data(), src, n);
+            SHAMX_HIP(hipStreamSynchronize(st));
+            return v;
+        };
         switch (what) {
-        case 0: d2h(st, dst, s->ghost_src.data(), s->G); break;
+        case 0: {
+            std::vector<u32> ext = s->external_labels(), src = host(s->ghost_src.data(), G);
+            for (u32 g = 0; g < G; g++) dst[ext[N + g] - N] = ext[src[g]];
+            break;
+        }
         case 1: d2h(st, dst, s->tree.sorted.data(), Nm); break;
-        case 2: d2h(st, dst, s->tree.sort_index.data(), Nm); break;
+        case 2: {
+            std::vector<u32> ext = s->external_labels(), si = host(s->tree.sort_index.data(), Nm);
+            for (u32 k = 0; k < Nm; k++) dst[k] = ext[si[k]];
+            break;
+        }
         case 3: d2h(st, dst, s->tree.reduc_map.data(), (size_t) L + 2); break;
-        case 4: d2h(st, dst, s->nb.owner.data(), s->N); break;
+        case 4: {
+            std::vector<u32> p = s->host_pid(), o = host(s->nb.owner.data(), N);
+            for (u32 j = 0; j < N; j++) dst[p[j]] = o[j];
+            break;
+        }
         case 5:
         case 6:
         case 7: s->reference_layout_csr(what, dst); break;
@@ -1251,7 +1421,11 @@ int shamx_solver_get_u32(shamx_solver *s, int what, uint32_t *dst) {
         case 11: d2h(st, dst, s->tree.lchild.data(), I); break;
         case 12: d2h(st, dst, s->tree.rchild.data(), I); break;
         case 13: d2h(st, dst, s->tree.endrange.data(), I); break;
-        case 14: d2h(st, dst, s->tree.codes.data(), Nm); break;
+        case 14: {
+            std::vector<u32> ext = s->external_labels(), c = host(s->tree.codes.data(), Nm);
+            for (u32 m = 0; m < Nm; m++) dst[ext[m]] = c[m];
+            break;
+        }
         default: throw std::invalid_argument("shamx_solver_get_u32: unknown id");
         }
         SHAMX_HIP(hipStreamSynchronize(st));
@@ -1263,6 +1437,9 @@ int shamx_solver_get_u8(shamx_solver *s, int what, uint8_t *dst) {
         hipStream_t st = s->stream;
         if (what == 0) d2h(st, dst, s->tree.lflag.data(), s->tree.I);
         else if (what == 1) d2h(st, dst, s->tree.rflag.data(), s->tree.I);
+        else if (what == 2)
+            d2h(st, dst, s->frec.data(),
+                force_record_bytes(s->cfg.mixed_precision) * (size_t) (s->N + s->G));
         else throw std::invalid_argument("shamx_solver_get_u8: unknown id");
         SHAMX_HIP(hipStreamSynchronize(st));
     });

@@ -34,6 +34,11 @@ namespace shamx {
     size_t radix_sort_scratch_bytes(u32 n);
     void radix_sort_pairs(hipStream_t s, u32 n, u32 key_bits, u32 *keys, u32 *vals,
                           u32 *keys_alt, u32 *vals_alt, void *scratch, bool vals_given = false);
+    // the same sort on buffers held in a static device array (no allocation), n at most
+    // NOALLOC_SORT_MAX_KEYS; calls that overlap on one GPU share that array and race
+    constexpr u32 NOALLOC_SORT_MAX_KEYS = 1u << 22;
+    void radix_sort_noalloc(hipStream_t s, u32 n, u32 key_bits, u32 *keys, u32 *perm);
+    size_t radix_sort_noalloc_bytes();
 
     // Tree state of a compressed-leaf BVH built on the merged particle set
     struct TreeBuffers {
@@ -137,11 +142,31 @@ namespace shamx {
                    const f64 *y, const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
                    f64 h_evol_max, f64 h_evol_iter_max, u64 *eps_minmax, int mixed,
                    DevArray<u64> &part);
+    // The Newton loop of one subcycle on the device (LoopSmoothingLengthIter.cpp:28-80): up to
+    // max_iter sweeps over the rows c (sel / outi select and map them), stopping after the
+    // first sweep whose max eps over all rows is below epsilon_h.  Sweeps are queued several
+    // at a time with one host read per batch (a queued sweep after the stop does nothing), and
+    // once few particles still iterate (eps > 1e-6) only their rows are swept.  The h values,
+    // eps values and sweep count equal those of one sweep per host read.
+    struct HLoopBuffers {
+        DevArray<u64> st, part;
+        DevArray<u32> sel_a, sel_b, flag, off;
+        DevArray<unsigned char> scan;
+    };
+    struct HLoopResult {
+        u32 iterations = 0, host_reads = 0, compactions = 0;
+        f64 max_eps = 0, min_eps = 0;
+    };
+    HLoopResult h_iterate_loop(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps,
+                               const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
+                               f64 h_evol_max, f64 h_evol_iter_max, f64 epsilon_h,
+                               u32 max_iter, int mixed, HLoopBuffers &b);
     // ComputeOmega; with records ps (row-indexed, .w = final h) and non-null fids, also writes
     // the force-interacting subsequence of every row to fids (same offsets) and fcnt
+    // (mixed: the dW/dh sum in FP32 with Kahan compensation, with the filter only)
     void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
                        const f64 *y, const f64 *z, const f64 *h, f64 *omega, f64 mass,
-                       u32 *fcnt, u32 *fids);
+                       u32 *fcnt, u32 *fids, int mixed = 0);
     // ps[s].w = h[sort_index[s]] for s < nm
     void ps_refresh_h(hipStream_t s, u32 nm, const u32 *sort_index, const f64 *h, double4 *ps);
     struct EosArgs {
@@ -228,6 +253,14 @@ namespace shamx {
         int n;
     };
     void gather_ghosts(hipStream_t s, u32 n, u32 G, const u32 *ghost_src, FieldPtrs fields);
+    // perm[j] = the j-th local index (< n) of the sorted order sort_index[0..nm) (flag, off:
+    // nm words of work space, scan_scratch: scan_scratch_bytes(nm))
+    void local_sorted_order(hipStream_t s, u32 nm, u32 n, const u32 *sort_index, u32 *perm,
+                            u32 *flag, u32 *off, void *scan_scratch);
+    // dst[j] = src[perm[j]] (gather) and dst[perm[j]] = src[j] (scatter), j < n
+    void perm_gather_f64(hipStream_t s, u32 n, const u32 *perm, const f64 *src, f64 *dst);
+    void perm_scatter_f64(hipStream_t s, u32 n, const u32 *perm, const f64 *src, f64 *dst);
+    void perm_gather_u32(hipStream_t s, u32 n, const u32 *perm, const u32 *src, u32 *dst);
 
     // ------------------------------------------------------------------ adjoint.hip
     // inputs of the adjoint of one step, all from the solver after that step (sorted layout)

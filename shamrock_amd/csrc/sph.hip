@@ -212,10 +212,13 @@ namespace shamx {
         __global__ void __launch_bounds__(BLK)
             k_h_iterate(u32 n, Rows r, Src src, const f64 *__restrict h_old,
                         f64 *__restrict h_new, f64 *__restrict eps, f64 mass, f64 h_evol_max,
-                        f64 h_evol_iter_max, u64 *__restrict part) {
+                        f64 h_evol_iter_max, u64 *__restrict part, const u64 *__restrict stop) {
             using KK = Kern<K, f64>;
             using KF = Kern<K, f32>;
-            __shared__ u64 lds[2][BLK / WAVE];
+            // a sweep queued after the loop has converged does nothing (same value in every
+            // thread: the whole grid leaves)
+            if (stop && *stop) return;
+            __shared__ u64 lds[3][BLK / WAVE];
             const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
             const u32 lg   = threadIdx.x % G;
             const bool act = j < n;
@@ -303,14 +306,80 @@ namespace shamx {
                 }
                 eps[a] = e;
             }
+            u64 kmax = ORDERED_NEG_INF, kmin = ORDERED_POS_INF, kact = 0;
+            if (act && lg == 0) {
+                kmax = kmin = f64_to_ordered(e);
+                kact         = e > 1e-6 ? 1u : 0u; // still iterating in the next sweep
+            }
+            kmax = block_reduce(kmax, umax64, lds[0]);
+            kmin = block_reduce(kmin, umin64, lds[1]);
+            kact = block_reduce(kact, [](u64 p, u64 q) { return p + q; }, lds[2]);
+            if (threadIdx.x == 0) {
+                part[blockIdx.x]                 = kmax;
+                part[gridDim.x + blockIdx.x]     = kmin;
+                part[2 * gridDim.x + blockIdx.x] = kact;
+            }
+        }
+
+        // ---- the Newton loop on the device (LoopSmoothingLengthIter.cpp:28-80)
+        // Loop state, u64 words: 0 stop, 1 sweeps done, 2/3 max/min eps key of the particles
+        // dropped from the sweep list, 4/5/6 max/min eps key and active count of the last
+        // sweep (over its list), 8/9 max/min eps key over all particles, 10 active count.
+        enum : int {
+            HL_STOP = 0, HL_ITERS, HL_FROZEN_MAX, HL_FROZEN_MIN, HL_SWEEP_MAX, HL_SWEEP_MIN,
+            HL_SWEEP_ACT, HL_PAD, HL_MAX, HL_MIN, HL_ACT, HL_WORDS = 16
+        };
+        __global__ void k_h_loop_init(u64 *__restrict st) {
+            if (threadIdx.x < HL_WORDS) {
+                u64 v = 0;
+                if (threadIdx.x == HL_FROZEN_MAX) v = ORDERED_NEG_INF;
+                if (threadIdx.x == HL_FROZEN_MIN) v = ORDERED_POS_INF;
+                st[threadIdx.x] = v;
+            }
+        }
+        // after each sweep: count it and stop the loop once max eps < epsilon_h, the
+        // reference's break condition
+        __global__ void k_h_loop_ctl(u64 *__restrict st, f64 epsilon_h) {
+            if (threadIdx.x != 0 || st[HL_STOP]) return;
+            st[HL_ITERS] += 1;
+            u64 mx = umax64(st[HL_FROZEN_MAX], st[HL_SWEEP_MAX]);
+            u64 mn = umin64(st[HL_FROZEN_MIN], st[HL_SWEEP_MIN]);
+            st[HL_MAX] = mx;
+            st[HL_MIN] = mn;
+            st[HL_ACT] = st[HL_SWEEP_ACT];
+            if (ordered_to_f64(mx) < epsilon_h) st[HL_STOP] = 1;
+        }
+        // rows of the sweep list whose particle still iterates (eps > 1e-6: the sweep's own
+        // test); the eps of the rows dropped is final for the subcycle and goes into the
+        // frozen max / min
+        __global__ void __launch_bounds__(BLK)
+            k_h_active_flags(u32 n, const u32 *__restrict sel, const u32 *__restrict outi,
+                             const f64 *__restrict eps, u32 *__restrict flag,
+                             u64 *__restrict st) {
+            __shared__ u64 lds[2][BLK / WAVE];
+            u32 j = blockIdx.x * BLK + threadIdx.x;
             u64 kmax = ORDERED_NEG_INF, kmin = ORDERED_POS_INF;
-            if (act && lg == 0) kmax = kmin = f64_to_ordered(e);
+            if (j < n) {
+                u32 row = sel ? sel[j] : j;
+                f64 e   = eps[outi ? outi[row] : row];
+                bool keep = e > 1e-6;
+                flag[j]   = keep ? 1u : 0u;
+                if (!keep) kmax = kmin = f64_to_ordered(e);
+            }
             kmax = block_reduce(kmax, umax64, lds[0]);
             kmin = block_reduce(kmin, umin64, lds[1]);
             if (threadIdx.x == 0) {
-                part[blockIdx.x]             = kmax;
-                part[gridDim.x + blockIdx.x] = kmin;
+                if (kmax != ORDERED_NEG_INF) atomicMax((unsigned long long *) &st[HL_FROZEN_MAX],
+                                                       (unsigned long long) kmax);
+                if (kmin != ORDERED_POS_INF) atomicMin((unsigned long long *) &st[HL_FROZEN_MIN],
+                                                       (unsigned long long) kmin);
             }
+        }
+        __global__ void k_h_compact_rows(u32 n, const u32 *__restrict sel,
+                                         const u32 *__restrict flag, const u32 *__restrict off,
+                                         u32 *__restrict out) {
+            u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+            if (j < n && flag[j]) out[off[j]] = sel ? sel[j] : j;
         }
 
         // ------------------------------------------------------------- omega
@@ -319,11 +388,14 @@ namespace shamx {
         // force loop, the subsequence of each row that passes the force criterion
         // rab2 <= max(h_a, h_b)^2 R^2 (UpdateDerivs.cpp:160-167), in row order, at the row's
         // offset in fids, with its length in fcnt (ps.w must hold the final h then).
-        template<class K, int G, bool FILTER, class Src>
+        // MIXED: the dW/dh sum in FP32 with Kahan compensation per lane on FP64 r_ab (lanes
+        // combined in FP64), like the mixed density sum; the force-row criterion stays FP64.
+        template<class K, int G, bool FILTER, bool MIXED, class Src>
         __global__ void __launch_bounds__(BLK)
             k_omega(u32 n, Rows r, Src src, const f64 *__restrict h, f64 *__restrict omega,
                     f64 mass, u32 *__restrict fcnt, u32 *__restrict fids) {
             using KK = Kern<K, f64>;
+            using KI = KernInv<K>;
             // the kept ids of a row go through a ring in LDS and leave in runs of FL entries
             // (64 bytes), so every line of the force rows is written whole and at once
             constexpr u32 RING = 32, FL = 16, PER = FL / G;
@@ -335,6 +407,7 @@ namespace shamx {
             const f64 Rkern = K::Rkern, Rker2 = K::Rkern * K::Rkern;
             u32 row = 0, a = 0, fc = 0, flushed = 0;
             f64 h_a = 0, part_omega_sum = 0;
+            f32 ms = 0, mc = 0; // MIXED: per-lane Kahan sum of dW/dh
             if (act) {
                 row        = r.sel ? r.sel[j] : j;
                 a          = r.outi ? r.outi[row] : row;
@@ -344,21 +417,30 @@ namespace shamx {
                 f64 ca     = h_a * h_a * Rker2;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
                 const KernAt<K> ka(h_a);
+                f32 ih = 0, ih4 = 0;
+                if constexpr (MIXED) {
+                    ih  = (f32) (1. / h_a);
+                    ih4 = ih * ih * ih * ih;
+                }
                 // one entry of the row: the Omega term and the force-row keep flag
                 auto entry = [&](bool v, u32 id, double4 q, f64 &th, bool &keep) {
                     if (!v) return;
                     f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
                     f64 rab2 = dx * dx + dy * dy + dz * dz;
                     if (!(rab2 > dint)) {
-                        f64 rab = sqrt(rab2);
-                        if (ka.fast) th = mass * ka.template dhW<true>(rab);
-                        else th = mass * ka.template dhW<false>(rab);
+                        if constexpr (MIXED) {
+                            kahan_add(ms, mc, KI::dhW(sqrt_approx((f32) rab2), ih, ih4));
+                        } else {
+                            f64 rab = sqrt(rab2);
+                            if (ka.fast) th = mass * ka.template dhW<true>(rab);
+                            else th = mass * ka.template dhW<false>(rab);
+                        }
                     }
                     if constexpr (FILTER) keep = !(rab2 > ca && rab2 > q.w * q.w * Rker2);
                 };
                 // fold the G terms of one pass, then append the kept ids in order
                 auto emit = [&](f64 th, bool keep, u32 id) {
-                    part_omega_sum = ordered_fold<G>(part_omega_sum, th);
+                    if constexpr (!MIXED) part_omega_sum = ordered_fold<G>(part_omega_sum, th);
                     if constexpr (FILTER) {
                         u64 bal   = __ballot(keep);
                         u32 gbits = (u32) (bal >> (__lane_id() & ~(G - 1))) & ((1u << G) - 1u);
@@ -415,6 +497,7 @@ namespace shamx {
                     for (u32 p = flushed + lg; p < fc; p += G) fids[k0 + p] = ring[grp][p & (RING - 1)];
                 }
             }
+            if constexpr (MIXED) part_omega_sum = mass * group_sum<G>((f64) ms - (f64) mc);
             if (act && lg == 0) {
                 f64 rho_ha = rho_h(mass, h_a, KK::hfactd);
                 omega[a]   = 1 + (h_a / (3 * rho_ha)) * part_omega_sum;
@@ -1310,40 +1393,113 @@ namespace shamx {
         size_t part_words(int nslot, u32 nb) { return (size_t) nslot * (nb + div_up(nb, FIN_CHUNK)); }
     } // namespace
 
+    namespace {
+        // one Newton sweep over n rows; partials of max eps, min eps and the active count in
+        // part (3 slots of div_up(n G, BLK) blocks, then the finalize room)
+        void h_sweep(hipStream_t s, int kernel, u32 n, Rows r, const double4 *ps, const f64 *x,
+                     const f64 *y, const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps,
+                     f64 mass, f64 h_evol_max, f64 h_evol_iter_max, int mixed,
+                     DevArray<u64> &part, const u64 *stop) {
+            u32 nb = div_up((u64) n * pair_lanes(), BLK);
+            part.resize(std::max<size_t>(part.size(), part_words(3, std::max<u32>(nb, 1))));
+            if (n == 0) return;
+            dispatch_kernel(kernel, [&](auto K) {
+                using KT = decltype(K);
+                dispatch_lanes([&](auto GC) {
+                    constexpr int G = decltype(GC)::value;
+                    auto go         = [&](auto src) {
+                        if (mixed)
+                            k_h_iterate<KT, G, true><<<nb, BLK, 0, s>>>(
+                                n, r, src, h_old, h_new, eps, mass, h_evol_max, h_evol_iter_max,
+                                part.data(), stop);
+                        else
+                            k_h_iterate<KT, G, false><<<nb, BLK, 0, s>>>(
+                                n, r, src, h_old, h_new, eps, mass, h_evol_max, h_evol_iter_max,
+                                part.data(), stop);
+                    };
+                    if (ps) go(PosRec{ps});
+                    else go(PosSoA{x, y, z});
+                });
+            });
+            SHAMX_HIP(hipGetLastError());
+        }
+    } // namespace
+
     void h_iterate(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
                    const f64 *y, const f64 *z, const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
                    f64 h_evol_max, f64 h_evol_iter_max, u64 *eps_minmax, int mixed,
                    DevArray<u64> &part) {
         if (n == 0) return;
-        u32 nb = div_up((u64) n * pair_lanes(), BLK);
-        part.resize(part_words(2, nb));
-        Rows r = rows_of(c);
-        dispatch_kernel(kernel, [&](auto K) {
-            using KT = decltype(K);
-            dispatch_lanes([&](auto GC) {
-                constexpr int G = decltype(GC)::value;
-                auto go         = [&](auto src) {
-                    if (mixed)
-                        k_h_iterate<KT, G, true><<<nb, BLK, 0, s>>>(
-                            n, r, src, h_old, h_new, eps, mass, h_evol_max, h_evol_iter_max,
-                            part.data());
-                    else
-                        k_h_iterate<KT, G, false><<<nb, BLK, 0, s>>>(
-                            n, r, src, h_old, h_new, eps, mass, h_evol_max, h_evol_iter_max,
-                            part.data());
-                };
-                if (ps) go(PosRec{ps});
-                else go(PosSoA{x, y, z});
-            });
-        });
-        SHAMX_HIP(hipGetLastError());
-        finalize(s, nb, part.data(), RedSpec{2, {RED_MAX_KEY, RED_MIN_KEY}, {0, 1}}, eps_minmax,
-                 part.data() + 2 * (size_t) nb);
+        h_sweep(s, kernel, n, rows_of(c), ps, x, y, z, h_old, h_new, eps, mass, h_evol_max,
+                h_evol_iter_max, mixed, part, nullptr);
+        finalize(s, div_up((u64) n * pair_lanes(), BLK), part.data(),
+                 RedSpec{2, {RED_MAX_KEY, RED_MIN_KEY}, {0, 1}}, eps_minmax,
+                 part.data() + 3 * (size_t) div_up((u64) n * pair_lanes(), BLK));
+    }
+
+    HLoopResult h_iterate_loop(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps,
+                               const f64 *h_old, f64 *h_new, f64 *eps, f64 mass,
+                               f64 h_evol_max, f64 h_evol_iter_max, f64 epsilon_h,
+                               u32 max_iter, int mixed, HLoopBuffers &b) {
+        HLoopResult res{};
+        if (n == 0 || max_iter == 0) return res;
+        b.st.resize(HL_WORDS);
+        k_h_loop_init<<<1, 64, 0, s>>>(b.st.data());
+        const u32 *sel = c.sel;
+        u32 n_act      = n;
+        u32 done = 0, batch = 1;
+        u64 host[HL_WORDS];
+        while (done < max_iter) {
+            u32 k = std::min(batch, max_iter - done);
+            for (u32 i = 0; i < k; i++) {
+                Rows r{sel, c.outi, c.cnt, c.offset, c.ids};
+                h_sweep(s, kernel, n_act, r, ps, nullptr, nullptr, nullptr, h_old, h_new, eps,
+                        mass, h_evol_max, h_evol_iter_max, mixed, b.part, b.st.data());
+                u32 nb = div_up((u64) n_act * pair_lanes(), BLK);
+                finalize(s, nb, b.part.data(),
+                         RedSpec{3, {RED_MAX_KEY, RED_MIN_KEY, RED_SUM_U64},
+                                 {HL_SWEEP_MAX, HL_SWEEP_MIN, HL_SWEEP_ACT}},
+                         b.st.data(), b.part.data() + 3 * (size_t) nb);
+                k_h_loop_ctl<<<1, 64, 0, s>>>(b.st.data(), epsilon_h);
+            }
+            SHAMX_HIP(hipGetLastError());
+            SHAMX_HIP(hipMemcpyAsync(host, b.st.data(), sizeof(host), hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            done += k;
+            res.host_reads++;
+            if (host[HL_STOP]) break;
+            // the particles still iterating are few: sweep only their rows from now on
+            u32 keep = (u32) host[HL_ACT];
+            if (done < max_iter && (u64) keep * 4 < n_act) {
+                b.flag.resize(std::max<u32>(n_act, 1));
+                b.off.resize(std::max<u32>(n_act, 1));
+                DevArray<u32> &dst = (sel == b.sel_a.data()) ? b.sel_b : b.sel_a;
+                dst.resize(std::max<u32>(keep, 1));
+                b.scan.resize(std::max(b.scan.size(), scan_scratch_bytes(n_act)));
+                k_h_active_flags<<<div_up(n_act, BLK), BLK, 0, s>>>(n_act, sel, c.outi, eps,
+                                                                    b.flag.data(), b.st.data());
+                exclusive_scan_u32(s, b.flag.data(), b.off.data(), n_act, nullptr, b.scan.data());
+                k_h_compact_rows<<<div_up(n_act, 256), 256, 0, s>>>(n_act, sel, b.flag.data(),
+                                                                   b.off.data(), dst.data());
+                SHAMX_HIP(hipGetLastError());
+                sel   = dst.data();
+                n_act = keep;
+                // (with no row left the remaining sweeps launch nothing; their reduction over
+                // no partials leaves max / min to the frozen extrema, and they still count,
+                // as the reference's sweeps over converged particles do)
+                res.compactions++;
+            }
+            batch = std::min<u32>(2 * batch, 8);
+        }
+        res.iterations = (u32) host[HL_ITERS];
+        res.max_eps    = ordered_to_f64(host[HL_MAX]);
+        res.min_eps    = ordered_to_f64(host[HL_MIN]);
+        return res;
     }
 
     void compute_omega(hipStream_t s, int kernel, u32 n, Csr c, const double4 *ps, const f64 *x,
                        const f64 *y, const f64 *z, const f64 *h, f64 *omega, f64 mass,
-                       u32 *fcnt, u32 *fids) {
+                       u32 *fcnt, u32 *fids, int mixed) {
         if (n == 0) return;
         if (fids && !ps) throw std::invalid_argument("compute_omega: the filter needs records");
         u32 nb = div_up((u64) n * pair_lanes(), BLK);
@@ -1352,15 +1508,19 @@ namespace shamx {
             using KT = decltype(K);
             dispatch_lanes([&](auto GC) {
                 constexpr int G = decltype(GC)::value;
-                if (ps && fids)
-                    k_omega<KT, G, true><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega, mass,
-                                                            fcnt, fids);
+                if (ps && fids && mixed)
+                    k_omega<KT, G, true, true><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega,
+                                                                  mass, fcnt, fids);
+                else if (ps && fids)
+                    k_omega<KT, G, true, false><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega,
+                                                                   mass, fcnt, fids);
                 else if (ps)
-                    k_omega<KT, G, false><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega, mass,
-                                                             nullptr, nullptr);
+                    k_omega<KT, G, false, false><<<nb, BLK, 0, s>>>(n, r, PosRec{ps}, h, omega,
+                                                                    mass, nullptr, nullptr);
                 else
-                    k_omega<KT, G, false><<<nb, BLK, 0, s>>>(n, r, PosSoA{x, y, z}, h, omega,
-                                                             mass, nullptr, nullptr);
+                    k_omega<KT, G, false, false><<<nb, BLK, 0, s>>>(n, r, PosSoA{x, y, z}, h,
+                                                                    omega, mass, nullptr,
+                                                                    nullptr);
             });
         });
         SHAMX_HIP(hipGetLastError());
@@ -1596,6 +1756,58 @@ namespace shamx {
                                            ghost_off_id.data());
         SHAMX_HIP(hipGetLastError());
         return G;
+    }
+
+    // ---- storage order: local particles kept in the order of the last tree's sort
+    namespace {
+        // flag[k] = 1 when sorted position k holds a local particle (index < n)
+        __global__ void k_local_flags(u32 nm, u32 n, const u32 *__restrict sort_index,
+                                      u32 *__restrict flag) {
+            u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+            if (k < nm) flag[k] = sort_index[k] < n ? 1u : 0u;
+        }
+        __global__ void k_local_compact(u32 nm, const u32 *__restrict sort_index,
+                                        const u32 *__restrict flag, const u32 *__restrict off,
+                                        u32 *__restrict perm) {
+            u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+            if (k < nm && flag[k]) perm[off[k]] = sort_index[k];
+        }
+        template<class T>
+        __global__ void k_perm_gather(u32 n, const u32 *__restrict perm, const T *__restrict src,
+                                      T *__restrict dst) {
+            u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+            if (j < n) dst[j] = src[perm[j]];
+        }
+        template<class T>
+        __global__ void k_perm_scatter(u32 n, const u32 *__restrict perm, const T *__restrict src,
+                                       T *__restrict dst) {
+            u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+            if (j < n) dst[perm[j]] = src[j];
+        }
+    } // namespace
+
+    void local_sorted_order(hipStream_t s, u32 nm, u32 n, const u32 *sort_index, u32 *perm,
+                            u32 *flag, u32 *off, void *scan_scratch) {
+        if (nm == 0) return;
+        k_local_flags<<<div_up(nm, 256), 256, 0, s>>>(nm, n, sort_index, flag);
+        exclusive_scan_u32(s, flag, off, nm, nullptr, scan_scratch);
+        k_local_compact<<<div_up(nm, 256), 256, 0, s>>>(nm, sort_index, flag, off, perm);
+        SHAMX_HIP(hipGetLastError());
+    }
+    void perm_gather_f64(hipStream_t s, u32 n, const u32 *perm, const f64 *src, f64 *dst) {
+        if (n == 0) return;
+        k_perm_gather<f64><<<div_up(n, 256), 256, 0, s>>>(n, perm, src, dst);
+        SHAMX_HIP(hipGetLastError());
+    }
+    void perm_scatter_f64(hipStream_t s, u32 n, const u32 *perm, const f64 *src, f64 *dst) {
+        if (n == 0) return;
+        k_perm_scatter<f64><<<div_up(n, 256), 256, 0, s>>>(n, perm, src, dst);
+        SHAMX_HIP(hipGetLastError());
+    }
+    void perm_gather_u32(hipStream_t s, u32 n, const u32 *perm, const u32 *src, u32 *dst) {
+        if (n == 0) return;
+        k_perm_gather<u32><<<div_up(n, 256), 256, 0, s>>>(n, perm, src, dst);
+        SHAMX_HIP(hipGetLastError());
     }
 
     void gather_ghosts(hipStream_t s, u32 n, u32 G, const u32 *ghost_src, FieldPtrs fields) {

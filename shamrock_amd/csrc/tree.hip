@@ -19,6 +19,8 @@
 #include "launch.hpp"
 #include "tree_layout.hpp"
 
+#include <mutex>
+
 namespace shamx {
 
     namespace {
@@ -571,10 +573,16 @@ namespace shamx {
         SHAMX_HIP(hipGetLastError());
     }
 
-    size_t radix_sort_scratch_bytes(u32 n) {
-        u32 nt = div_up(std::max<u32>(n, 1), RS_TILE);
-        return (size_t) (4 * RS_BINS + 8 + (size_t) 4 * nt * RS_BINS) * sizeof(u32);
-    }
+    namespace {
+        // scratch of radix_sort_pairs for n keys: 4 digit histograms, 8 counters / error
+        // words, then 4 passes of per-tile digit status words
+        constexpr size_t rs_scratch_bytes(u32 n) {
+            size_t nt = ((size_t) (n > 0 ? n : 1) + RS_TILE - 1) / RS_TILE;
+            return (4 * RS_BINS + 8 + 4 * nt * RS_BINS) * sizeof(u32);
+        }
+    } // namespace
+
+    size_t radix_sort_scratch_bytes(u32 n) { return rs_scratch_bytes(n); }
 
     void radix_sort_pairs(hipStream_t s, u32 n, u32 key_bits, u32 *keys, u32 *vals,
                           u32 *keys_alt, u32 *vals_alt, void *scratch, bool vals_given) {
@@ -604,6 +612,53 @@ namespace shamx {
             SHAMX_HIP(hipMemcpyAsync(vals, vin, n * sizeof(u32), hipMemcpyDeviceToDevice, s));
         }
         SHAMX_HIP(hipGetLastError());
+    }
+
+    // ---- the sort without allocation: its buffers live in one file-scope device array
+    // (one copy per GPU, held until the process ends) whose address is looked up once per
+    // device.  Layout: the alternate keys, the alternate values (NOALLOC_SORT_MAX_KEYS each),
+    // then the scratch of radix_sort_pairs for that many keys.
+    namespace {
+        constexpr size_t NOALLOC_ALT_BYTES = (size_t) NOALLOC_SORT_MAX_KEYS * sizeof(u32);
+        constexpr size_t NOALLOC_SCRATCH_BYTES =
+            2 * NOALLOC_ALT_BYTES + ((rs_scratch_bytes(NOALLOC_SORT_MAX_KEYS) + 255) / 256) * 256;
+        static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+        constexpr int NOALLOC_MAX_DEVICES = 64;
+        std::mutex g_noalloc_mutex;
+        void *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+        // device address of `scratch` on the current device (the host-side name is only a
+        // stand-in for it)
+        unsigned char *noalloc_scratch_ptr() {
+            int dev = -1;
+            SHAMX_HIP(hipGetDevice(&dev));
+            if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+                throw std::runtime_error("shamx_sort_noalloc: device id out of range");
+            std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+            if (!g_noalloc_ptr[dev]) {
+                void *p = nullptr;
+                SHAMX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+                if (!p) throw std::runtime_error("shamx_sort_noalloc: no address for scratch");
+                g_noalloc_ptr[dev] = p;
+            }
+            return (unsigned char *) g_noalloc_ptr[dev];
+        }
+    } // namespace
+
+    size_t radix_sort_noalloc_bytes() { return NOALLOC_SCRATCH_BYTES; }
+
+    void radix_sort_noalloc(hipStream_t s, u32 n, u32 key_bits, u32 *keys, u32 *perm) {
+        if (n > NOALLOC_SORT_MAX_KEYS)
+            throw std::invalid_argument("shamx_sort_noalloc: n exceeds the maximum of 4194304 "
+                                        "keys of the library's static sort buffer");
+        if (key_bits == 0 || key_bits > 32)
+            throw std::invalid_argument("shamx_sort_noalloc: key_bits must be in [1, 32]");
+        if (n == 0) return;
+        unsigned char *base = noalloc_scratch_ptr();
+        u32 *alt_k = (u32 *) base;
+        u32 *alt_v = (u32 *) (base + NOALLOC_ALT_BYTES);
+        radix_sort_pairs(s, n, key_bits, keys, perm, alt_k, alt_v, base + 2 * NOALLOC_ALT_BYTES);
     }
 
     void tree_stage_codes(hipStream_t s, TreeBuffers &t, u32 n, const f64 *x, const f64 *y,

@@ -16,7 +16,10 @@ from typing import Callable
 
 import numpy as np
 
-from .setups import KERNEL_HFACT, KERNEL_RKERN, base_config, kernel_W, lattice_cubic
+from .analysis import AnalysisSodTube
+from .phys import SodTube
+from .setups import (KERNEL_HFACT, KERNEL_RKERN, base_config, fcc_cube, hcp_box_dim,
+                     hcp_lattice, kernel_W, lattice_cubic)
 from .solver import FIELDS, G_SI, GpuSolver, sink_dict
 
 _VEC_FIELDS = {"xyz": ("x", "y", "z"), "vxyz": ("vx", "vy", "vz"), "axyz": ("ax", "ay", "az"),
@@ -173,19 +176,69 @@ class SolverConfig:
         print(self.d)
 
 
+class SetupNode:
+    """A node of the setup graph (ISPHSetupNode, modules/setup/ISPHSetupNode.hpp): a lattice
+    generator or a combiner of two nodes.  `positions()` returns the particles the node emits
+    (positions and smoothing lengths) in emission order."""
+
+    def __init__(self, kind, dr=0.0, box_min=None, box_max=None, parents=()):
+        self.kind, self.dr = kind, float(dr)
+        self.box_min = None if box_min is None else tuple(map(float, box_min))
+        self.box_max = None if box_max is None else tuple(map(float, box_max))
+        self.parents = tuple(parents)
+
+    def positions(self):
+        if self.kind == "combiner_add":
+            parts = [p.positions() for p in self.parents]
+            return (np.concatenate([q for q, _ in parts]), np.concatenate([h for _, h in parts]))
+        gen = hcp_lattice if self.kind == "lattice_hcp" else lattice_cubic
+        pos = gen(self.dr, self.box_min, self.box_max)
+        return pos, np.full(pos.shape[0], self.dr)
+
+    def get_dot(self):
+        """Graphviz text of the node graph (ISPHSetupNode::get_dot)."""
+        lines, ids = [], {}
+
+        def visit(node):
+            if id(node) in ids:
+                return ids[id(node)]
+            ids[id(node)] = k = len(ids)
+            lines.append(f'  n{k} [label="{node.kind}"];')
+            for p in node.parents:
+                lines.append(f"  n{visit(p)} -> n{k};")
+            return k
+
+        visit(self)
+        return "digraph setup {\n" + "\n".join(lines) + "\n}"
+
+
 class _Setup:
-    """SPHSetup subset (SPHSetup.cpp:111, GeneratorLatticeCubic)."""
+    """SPHSetup subset (SPHSetup.cpp, pySPHModel.cpp:449-640): lattice generators, the add
+    combiner and apply_setup.  Generators give h = their spacing and u = 0
+    (GeneratorLatticeHCP.hpp:83-130, GeneratorLatticeCubic.hpp); the combiner emits its first
+    parent's particles, then its second's."""
 
     def __init__(self, model):
         self.model = model
 
     def make_generator_lattice_cubic(self, dr, box_min, box_max):
-        return ("lattice_cubic", float(dr), tuple(box_min), tuple(box_max))
+        return SetupNode("lattice_cubic", dr, box_min, box_max)
+
+    def make_generator_lattice_hcp(self, dr, box_min, box_max, discontinuous=True):
+        """pySPHModel.cpp:452-462.  `discontinuous` selects the reference's emission order
+        (an interleaved walk of the index ranges); the point set is the same either way and
+        only particle ids differ, so one order is used here."""
+        return SetupNode("lattice_hcp", dr, box_min, box_max)
+
+    def make_combiner_add(self, parent1, parent2):
+        """pySPHModel.cpp:624-630"""
+        return SetupNode("combiner_add", parents=(parent1, parent2))
 
     def apply_setup(self, gen, **kw):
-        kind, dr, bmin, bmax = gen
-        pos = lattice_cubic(dr, bmin, bmax)
-        self.model.push_particle(pos, np.full(pos.shape[0], dr), np.zeros(pos.shape[0]))
+        """SPHSetup::apply_setup: insert the particles of the node graph (the keyword
+        arguments of the reference tune its MPI insertion and are accepted and ignored)."""
+        pos, h = gen.positions()
+        self.model.push_particle(pos, h, np.zeros(pos.shape[0]))
 
 
 class SPHModel:
@@ -229,6 +282,53 @@ class SPHModel:
 
     def get_setup(self):
         return _Setup(self)
+
+    def get_box_dim_fcc_3d(self, dr, xcnt, ycnt, zcnt):
+        """pySPHModel.cpp:802-806 (Model::get_box_dim_fcc_3d, generators.hpp:31-51): extent of
+        xcnt x ycnt x zcnt cells of the HCP lattice of spacing dr."""
+        return hcp_box_dim(float(dr), int(xcnt), int(ycnt), int(zcnt))
+
+    def get_ideal_hcp_box(self, dr, box_min, box_max):
+        """pySPHModel.cpp:818-827 (deprecated there in favour of shamrock.math)."""
+        from .math import get_ideal_hcp_box
+
+        return get_ideal_hcp_box(dr, box_min, box_max)
+
+    def add_cube_fcc_3d(self, dr, box_min, box_max):
+        """pySPHModel.cpp:849-853 (Model::add_cube_fcc_3d, Model.cpp:1093-1190): the lattice of
+        add_particles_fcc (generators.hpp:82-130), anchored at box_min, kept inside the
+        half-open box; h = dr, u = 0."""
+        pos = fcc_cube(float(dr), box_min, box_max)
+        self.push_particle(pos, np.full(pos.shape[0], float(dr)), np.zeros(pos.shape[0]))
+
+    def make_analysis_sodtube(self, sod, direction, time_val, x_ref, x_min, x_max):
+        """pySPHModel.cpp:1466-1485: an analysis whose compute_L2_dist() returns the mean
+        squared deviations (rho, (vx, vy, vz), P) from the exact solution `sod`."""
+        return AnalysisSodTube(self, sod, direction, time_val, x_ref, x_min, x_max)
+
+    def get_closest_part_to(self, pos):
+        """Model::get_closest_part_to: position of the particle nearest to pos."""
+        p = self._field_get("xyz")
+        if p.shape[0] == 0:
+            raise RuntimeError("no particles")
+        d = ((p - np.asarray(pos, float)) ** 2).sum(1)
+        return tuple(float(v) for v in p[int(np.argmin(d))])
+
+    def get_solver_tex(self):
+        """A text description of the step this model runs (the reference renders its solver
+        graph; this library's step is the fixed sequence below)."""
+        av = "Cullen \\& Dehnen varying" if self.cfg.d.get("av_kind", 0) else "constant"
+        return ("SPH step (" + self.kernel + " kernel, " + av + " viscosity): leapfrog "
+                "predictor; periodic images; Morton codes, radix sort, leaf reduction, Karras "
+                "tree, cell boxes; two-stage neighbour cache; smoothing-length Newton "
+                "iteration; $\\Omega$; equation of state; forces and $du/dt$; leapfrog "
+                "corrector with rerun on $\\epsilon_v > 10^{-2}$; CFL time step.")
+
+    def get_solver_dot_graph(self):
+        """Graphviz edges of the step's phases, in execution order."""
+        phases = ["predictor", "ghosts", "tree", "neighbour_cache", "h_iteration", "omega",
+                  "eos", "force", "corrector", "cfl"]
+        return "\n".join(f"{a} -> {b};" for a, b in zip(phases, phases[1:]))
 
     # --- particles -----------------------------------------------------------------------
     def _sync_from_gpu(self):

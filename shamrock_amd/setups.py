@@ -230,6 +230,26 @@ def hcp_box_dim(dr: float, xcnt: int, ycnt: int, zcnt: int) -> tuple[float, floa
     return r[0] * dr, r[1] * dr, r[2] * dr
 
 
+def fcc_cube(dr: float, box_min, box_max) -> np.ndarray:
+    """Model::add_cube_fcc_3d (Model.cpp:1093-1190, generators.hpp:82-130): points
+    box_min + dr * (2 i + (j + k) % 2, sqrt(3) (j + (k % 2) / 3), 2 sqrt(6) k / 3) for
+    0 <= i, j, k < ceil(extent / (dr * (2, sqrt(3), 2 sqrt(6) / 3))), kept inside the
+    half-open box."""
+    bmin = np.asarray(box_min, float)
+    bmax = np.asarray(box_max, float)
+    s3, s6 = math.sqrt(3.0), math.sqrt(6.0)
+    cells = ((bmax - bmin) / np.array([2.0, s3, 2 * s6 / 3])) / dr
+    ni, nj, nk = (int(math.ceil(c)) for c in cells)
+    k, j, i = np.meshgrid(np.arange(nk), np.arange(nj), np.arange(ni), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    x = (2 * i + (j + k) % 2).astype(np.float64)
+    y = s3 * (j + (1.0 / 3.0) * (k % 2))
+    z = 2 * s6 * k.astype(np.float64) / 3
+    p = np.stack([x * dr, y * dr, z * dr], axis=1) + bmin
+    keep = np.all((p >= bmin) & (p < bmax), axis=1)
+    return np.ascontiguousarray(p[keep])
+
+
 def hcp_lattice(dr: float, box_min, box_max) -> np.ndarray:
     """HCP lattice points inside the half-open box (crystalLattice.hpp LatticeHCP: generator,
     get_box_index_bounds, index ranges [min, max); GeneratorLatticeHCP.hpp:83-106 keeps the

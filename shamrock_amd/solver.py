@@ -35,6 +35,7 @@ def make_config(cfg: dict) -> _lib.ShamxConfig:
               "h_max_subcycles", "mixed_precision"):
         setattr(c, k, int(cfg[k]))
     c.av_kind = int(cfg.get("av_kind", 0))
+    c.keep_storage_order = int(cfg.get("keep_storage_order", 0))
     for k, d in (("alpha_min", 0.0), ("alpha_max", 1.0), ("sigma_decay", 0.1),
                  ("constant_G", G_SI), ("sink_softening", 0.0)):
         setattr(c, k, float(cfg.get(k, d)))

@@ -1,14 +1,22 @@
 """The Sod-tube analytic solution (shamphys/src/SodTube.cpp:22-133) against the textbook values of
 Sod's problem (gamma = 1.4, left rho = 1, P = 1, right rho = 0.125, P = 0.1; Toro, Riemann
 Solvers and Numerical Methods for Fluid Dynamics, 3rd ed., table 4.3, test 1 shifted to rest):
-p* = 0.30313, u* = 0.92745, rho*_L = 0.42632, rho*_R = 0.26557, shock speed 1.75216."""
+p* = 0.30313, u* = 0.92745, rho*_L = 0.42632, rho*_R = 0.26557, shock speed 1.75216.  The L2
+analysis (AnalysisSodTube.cpp:27-120) is checked on particles placed on the exact solution
+and on hand-made deviations."""
 import numpy as np
+import pytest
 
+import shamrock_amd as shamrock
 from shamrock_amd import analysis
 
 
+def _sod():
+    return shamrock.phys.SodTube(gamma=1.4, rho_1=1, P_1=1, rho_5=0.125, P_5=0.1)
+
+
 def test_sod_star_state():
-    sod = analysis.SodTube(1.4, 1, 1, 0.125, 0.1)
+    sod = _sod()
     t = 0.2
     # contact at u* t, shock at 1.75216 t
     rho3, v3, P3 = sod.get_value(t, 0.5 * 0.92745 * t)
@@ -20,5 +28,70 @@ def test_sod_star_state():
     assert sod.get_value(t, 1.8 * t) == (0.125, 0.0, 0.1)
     rho, v, P = sod.get_value(t, -1.1 * t)
     assert 0 < v < 0.92745 and 0.30313 < P < 1 and 0.42632 < rho < 1
-    # the post-shock pressure is the reference's float solve
-    assert sod._P4 == float(np.float32(sod._P4))
+    # inside the fan the flow is isentropic: P / rho^gamma is the left state's
+    assert P / rho**1.4 == pytest.approx(1.0, rel=1e-12)
+    # the post-shock pressure is the reference's single-precision Newton result
+    assert sod.p_star == float(np.float32(sod.p_star))
+    with pytest.raises(ValueError):
+        shamrock.phys.SodTube(gamma=1.4, rho_1=1, P_1=0.1, rho_5=0.125, P_5=1)
+
+
+def test_sod_l2_on_exact_and_perturbed_particles():
+    sod = _sod()
+    t, gamma, m, hfact = 0.2, 1.4, 1e-3, 1.2
+    x = np.linspace(-0.6, 0.6, 1201)
+    rho, v, P = sod.values(t, x)
+    h = hfact * (m / rho) ** (1 / 3)
+    u = P / ((gamma - 1) * rho)
+    xyz = np.stack([x, np.zeros_like(x), np.zeros_like(x)], 1)
+    vxyz = np.stack([v, np.zeros_like(x), np.zeros_like(x)], 1)
+    r, vv, p = analysis.sod_l2_dist(sod, xyz, vxyz, h, u, m, hfact, gamma, t)
+    assert r < 1e-28 and p < 1e-28 and max(vv) < 1e-28
+    # a velocity error of 0.01 in y on every selected particle, and 0.1 in u on one
+    vxyz[:, 1] = 0.01
+    sel = (x > -0.5) & (x < 0.5)
+    k = np.flatnonzero(sel)[7]
+    u2 = u.copy()
+    u2[k] += 0.1
+    r, vv, p = analysis.sod_l2_dist(sod, xyz, vxyz, h, u2, m, hfact, gamma, t)
+    assert vv[1] == pytest.approx(1e-4, rel=1e-12) and vv[2] == 0
+    dP = (gamma - 1) * m * (hfact / h[k]) ** 3 * 0.1
+    assert p == pytest.approx(dP * dP / sel.sum(), rel=1e-9)
+    with pytest.raises(RuntimeError):
+        analysis.sod_l2_dist(sod, xyz, vxyz, h, u, m, hfact, gamma, t, x_min=5, x_max=6)
+
+
+def test_lattice_helpers_closed_forms():
+    """get_box_dim_fcc_3d, add_cube_fcc_3d's lattice and get_ideal_hcp_box against the HCP
+    geometry: cell extents (2, sqrt(3), 2 sqrt(6)/3) dr, nearest-neighbour distance 2 dr, and a
+    periodic box whose images of the lattice coincide with the lattice."""
+    ctx = shamrock.Context()
+    ctx.pdata_layout_new()
+    model = shamrock.get_Model_SPH(context=ctx, vector_type="f64_3", sph_kernel="M6")
+    xs, ys, zs = model.get_box_dim_fcc_3d(0.5, 4, 6, 8)
+    assert (xs, ys, zs) == pytest.approx((4.0, 3 * np.sqrt(3), 8 * np.sqrt(6) / 3), rel=1e-15)
+    bmin, bmax = shamrock.math.get_ideal_hcp_box(0.1, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    from shamrock_amd import setups
+    pos = setups.hcp_lattice(0.1, bmin, bmax)
+    L = np.asarray(bmax) - np.asarray(bmin)
+    # the lattice continued past each face of the box, shifted back by the box length, lands
+    # on lattice points inside it (the box is periodic for the lattice)
+    key = {tuple(np.round(p, 9)) for p in pos}
+    big = setups.hcp_lattice(0.1, np.asarray(bmin) - 0.35, np.asarray(bmax) + 0.35)
+    for k in range(3):
+        other = [a for a in range(3) if a != k]
+        out = (big[:, k] >= bmax[k]) & (big[:, k] < bmax[k] + 0.3)
+        for a in other:
+            out &= (big[:, a] >= bmin[a]) & (big[:, a] < bmax[a])
+        sh = big[out].copy()
+        sh[:, k] -= L[k]
+        assert out.sum() > 50
+        assert all(tuple(np.round(q, 9)) in key for q in sh)
+    model.resize_simulation_box(bmin, bmax)
+    model.add_cube_fcc_3d(0.1, bmin, bmax)
+    data = model.collect_data()
+    assert data["xyz"].shape[0] > 0
+    assert np.all(data["hpart"] == 0.1) and np.all(data["uint"] == 0)
+    p = data["xyz"][:500]
+    d = np.sqrt(((p[:, None] - p[None]) ** 2).sum(-1)) + np.eye(len(p)) * 9
+    assert d.min() == pytest.approx(0.2, rel=1e-12)

@@ -44,13 +44,36 @@ def test_binding_loads_and_reports_errors_without_gpu():
     assert b"kernel" in L.shamx_last_error()
 
 
-def test_struct_layouts_match_header():
+def test_struct_layouts_match_header(tmp_path):
+    """Size and every field offset of shamx_config and shamx_step_log as the C compiler lays
+    them out from include/shamx_sph.h, against the ctypes mirrors."""
+    import ctypes
+    import os
+    import subprocess
+
     from shamrock_amd import _lib
 
-    # shamx_config: 6 int32 + 6 f64 box + 15 f64 + 2 int32 = 24 + 48 + 120 + 8
-    assert _lib.ShamxConfig.__dict__ and ctypes_size(_lib.ShamxConfig) == 240
-    # shamx_step_log: 11 f64 + 8 u32 + 2 u64 + 5 f64
-    assert ctypes_size(_lib.ShamxStepLog) == 11 * 8 + 8 * 4 + 2 * 8 + 5 * 8
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "shamx_sph.h"',
+             "int main(void) {"]
+    for cname, ct in (("shamx_config", _lib.ShamxConfig), ("shamx_step_log", _lib.ShamxStepLog)):
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in ct._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    c = {}
+    for line in out.splitlines():
+        name, field, v = line.split()
+        c[(name, field)] = int(v)
+    for cname, ct in (("shamx_config", _lib.ShamxConfig), ("shamx_step_log", _lib.ShamxStepLog)):
+        assert c[(cname, "size")] == ctypes.sizeof(ct), cname
+        for f, _ in ct._fields_:
+            assert c[(cname, f)] == getattr(ct, f).offset, (cname, f)
 
 
 def ctypes_size(t):

@@ -28,15 +28,33 @@ def gpu_ready():
 
 
 def _model(cfg, f):
+    """The disc of setups.disc built through the public surface only: config setters, box,
+    push_particle, a velocity field set from the positions (Keplerian about the sink), particle
+    mass, the sink."""
     ctx = shamrock.Context()
     ctx.pdata_layout_new()
     m = shamrock.get_Model_SPH(context=ctx, vector_type="f64_3", sph_kernel=cfg["kernel"])
     c = m.gen_default_config()
-    c.d.update({k: v for k, v in cfg.items() if k != "sinks"})
+    c.set_artif_viscosity_VaryingCD10(alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1,
+                                      alpha_u=1.0, beta_AV=2.0)
+    c.set_boundary_free()
+    c.set_eos_locally_isothermalLP07(cfg["eos_cs0"], cfg["eos_q"], cfg["eos_r0"])
+    # code units with G = 1: a mass unit of 1 / G_SI kg
+    c.set_units(shamrock.UnitSystem(unit_mass=1.0 / 6.6743015e-11))
+    c.set_sink_softening(cfg["sink_softening"])
     m.set_solver_config(c)
+    m.resize_simulation_box(cfg["box_min"], cfg["box_max"])
     pos = np.stack([f["x"], f["y"], f["z"]], 1)
     m.push_particle(pos, f["h"], f["u"])
-    m._field_set("vxyz", np.stack([f["vx"], f["vy"], f["vz"]], 1))
+    G = c.get_constant_G()
+
+    def kepler(p):
+        r = np.hypot(p[0], p[1])
+        vk = np.sqrt(G * 1.0 / r)
+        return (-vk * p[1] / r, vk * p[0] / r, 0.0)
+
+    m.set_field_value_lambda_f64_3("vxyz", kepler)
+    m.set_particle_mass(cfg["gpart_mass"])
     for s in cfg.get("sinks", ()):
         m.add_sink(s["mass"], s["pos"], s["velocity"], s["accretion_radius"])
     return ctx, m
@@ -44,8 +62,8 @@ def _model(cfg, f):
 
 def test_restart_continues_the_run(tmp_path):
     cfg, f = setups.disc(20000, seed=3)
-    cfg = dict(cfg, av_kind=1, alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1)
     ctx, m = _model(cfg, f)
+    assert abs(m.get_current_config().get_constant_G() - 1.0) < 1e-15
     for i in range(3):
         say(f"[dump] step {i}")
         m.evolve_once()

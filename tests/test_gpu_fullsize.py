@@ -1,0 +1,173 @@
+"""The workloads at their stated sizes.
+
+Sound wave 32^3 (5 steps) and Sedov 128^3 (2,097,152 particles, the state behind the headline
+number; 3 FP64 steps, the third of which reruns its corrector) against the oracle: integer
+products bit for bit, FP64 fields within 1e-10.  The 16.7 M Plummer pass and the 4.2 M disc are
+too large for the oracle in a test; they are checked by size-independent properties: total
+momentum (pairwise antisymmetric forces, kick-drift-kick leapfrog) and, for Plummer, neighbour
+counts against a brute-force search on 1,000 random particles.  The mixed-precision disc (sink,
+LP07) is held to the mixed-precision error bound against its FP64 step.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+import oracle_lib as orc  # noqa: E402
+from shamrock_amd import setups  # noqa: E402
+from shamrock_amd.solver import GpuSolver  # noqa: E402
+
+RTOL = 1e-10
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "h", "u", "du")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_ready():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def relerr(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = np.maximum(np.abs(b), np.max(np.abs(b)) * 1e-3 + 1e-300)
+    return float(np.max(np.abs(a - b) / scale)) if a.size else 0.0
+
+
+def _steps_vs_oracle(cfg, f, steps, full_lists_first):
+    g = GpuSolver(cfg, f)
+    o = orc.Solver(cfg, f)
+    passes = []
+    for step in range(steps):
+        say(f"[fullsize] N={g.n} step {step}")
+        lg = g.evolve_once()
+        lo = o.evolve_once()
+        assert lg["subcycles"] == lo["subcycles"] and lg["h_iterations"] == lo["h_iterations"]
+        assert lg["corrector_iterations"] == lo["corrector_iterations"]
+        assert lg["n_pairs"] == lo["n_pairs"]
+        passes.append(lg["corrector_iterations"])
+        sg, so = g.sizes(), o.sizes()
+        assert sg == so
+        Nm, G, L, I = sg["Nm"], sg["G"], sg["L"], sg["I"]
+        for what, n, name in ((0, G, "ghost_src"), (1, Nm, "sorted_codes"),
+                              (2, Nm, "sort_index"), (3, L + 2, "reduc_map"),
+                              (5, g.n, "part_cnt"), (11, I, "lchild"), (12, I, "rchild")):
+            np.testing.assert_array_equal(g.get_u32(what, n), o.get_u32(what, n), err_msg=name)
+        if step == 0 and full_lists_first:
+            P = sg["P"]
+            np.testing.assert_array_equal(g.get_u32(7, P), o.get_u32(7, P), err_msg="part_ids")
+        for k in FIELDS:
+            assert relerr(g.get(k), o.get(k)) <= RTOL, (k, relerr(g.get(k), o.get(k)))
+        tg, to = g.time(), o.time()
+        assert abs(tg["dt"] - to["dt"]) <= RTOL * abs(to["dt"])
+        assert tg["cfl_mult"] == to["cfl_mult"]
+    g.close()
+    return passes
+
+
+def test_sound_wave_32_five_steps():
+    """The CPU-runnable workload as stated: 32^3 cubic lattice, M4, 5 leapfrog steps at CFL 0.3."""
+    cfg, f = setups.sound_wave(32)
+    _steps_vs_oracle(cfg, f, 5, full_lists_first=True)
+
+
+def test_sedov_128_three_steps():
+    """The headline workload's state: Sedov 128^3, M6, periodic (2,460,375 particles with their
+    images), the dt = 0 step and two more, neighbour lists compared whole on the first."""
+    cfg, f = setups.sedov(128)
+    passes = _steps_vs_oracle(cfg, f, 3, full_lists_first=True)
+    assert max(passes) >= 2, passes
+
+
+def _momentum(g, cfg):
+    m = cfg["gpart_mass"]
+    v = np.stack([g.get(k) for k in ("vx", "vy", "vz")], 1)
+    p = m * v.sum(0)
+    scale = m * np.abs(v).sum(0)
+    for sk in g.get_sinks():
+        p = p + sk["mass"] * np.asarray(sk["velocity"])
+        scale = scale + sk["mass"] * np.abs(np.asarray(sk["velocity"]))
+    return p, scale
+
+
+def test_plummer_16M_pass_momentum_and_counts():
+    """The clustered workload's density/force pass at 16,777,216 particles: sum m a = 0 to
+    rounding, and the neighbour-cache count of 1,000 random particles equal to a brute-force
+    count of {b : |r_ab|^2 <= R^2 max(h_a, h_b)^2 htol^2} over all particles (h of the cache
+    build, htol = 1.1, M4 R = 2)."""
+    n = 16_777_216
+    cfg, f = setups.plummer(n, seed=1, r_max=30.0)
+    h0 = f["h"].copy()
+    g = GpuSolver(cfg, f)
+    say("[fullsize] plummer 16M pass")
+    g.density_force()
+    m = cfg["gpart_mass"]
+    a = np.stack([g.get(k) for k in ("ax", "ay", "az")], 1)
+    assert np.all(np.abs(m * a.sum(0)) <= 1e-12 * m * np.abs(a).sum(0))
+    cnt = g.get_u32(5, n)
+    g.close()
+    dev = "cuda:0"
+    X = torch.from_numpy(f["x"]).to(dev)
+    Y = torch.from_numpy(f["y"]).to(dev)
+    Z = torch.from_numpy(f["z"]).to(dev)
+    rb = torch.from_numpy(h0 * 1.1).to(dev)
+    R = 2.0
+    sample = np.random.default_rng(7).choice(n, 1000, replace=False)
+    brute = np.zeros(sample.size, np.int64)
+    for i, a_ in enumerate(sample):
+        dx, dy, dz = X - X[a_], Y - Y[a_], Z - Z[a_]
+        d2 = dx * dx + dy * dy + dz * dz
+        ra = rb[a_]
+        inside = ~((d2 > ra * ra * R * R) & (d2 > rb * rb * R * R))
+        brute[i] = int(inside.sum().item())
+    say(f"[fullsize] plummer counts: median {int(np.median(brute))}, max {brute.max()}")
+    np.testing.assert_array_equal(cnt[sample].astype(np.int64), brute)
+
+
+def test_disc_4M_momentum_four_steps():
+    """The Keplerian disc at 4,194,304 particles with its sink (mixed precision, LP07): total
+    momentum of gas and sink is kept over 4 steps (the dt = 0 step and 3 more) to rounding."""
+    cfg, f = setups.disc(4_194_304, seed=2, mixed=True)
+    g = GpuSolver(cfg, f)
+    p0, s0 = _momentum(g, cfg)
+    for step in range(4):
+        say(f"[fullsize] disc 4M step {step}")
+        g.evolve_once()
+    p1, s1 = _momentum(g, cfg)
+    g.close()
+    assert np.all(np.abs(p1 - p0) <= 1e-12 * np.maximum(s0, s1)), (p0, p1, s0)
+
+
+def test_disc_mixed_precision_error_bound():
+    """Mixed precision with the sink force and LP07: one step from the same state in FP64 and
+    mixed; rel-L2 of a and du/dt <= 1e-5, the same number of corrector passes."""
+    cfg, f = setups.disc(20000, seed=2)
+    g = GpuSolver(cfg, f)
+    for step in range(2):
+        say(f"[fullsize] disc 20000 FP64 step {step}")
+        g.evolve_once()
+    st = {k: g.get(k) for k in FIELDS}
+    tm, sinks = g.time(), g.sink_records()
+    g.close()
+    res = {}
+    for mixed in (0, 1):
+        s = GpuSolver(dict(cfg, mixed_precision=mixed, sinks=[]), st)
+        s.set_sink_records(sinks)
+        s.set_time(tm["t"], tm["dt"], tm["cfl_mult"])
+        lg = s.evolve_once()
+        res[mixed] = (lg["corrector_iterations"],
+                      np.stack([s.get(k) for k in ("ax", "ay", "az")]), s.get("du"))
+        s.close()
+    assert res[0][0] == res[1][0]
+    ea = np.linalg.norm(res[1][1] - res[0][1]) / np.linalg.norm(res[0][1])
+    eu = np.linalg.norm(res[1][2] - res[0][2]) / np.linalg.norm(res[0][2])
+    say(f"[fullsize] disc mixed rel-L2 a={ea:.3e} du={eu:.3e}")
+    assert ea <= 1e-5 and eu <= 1e-5

@@ -223,6 +223,48 @@ def test_cfl_closed_form():
     assert dmin[0] == want.min()
 
 
+def _eos_gpu(eos_kind, pos, h, u, mass, gamma=5 / 3, cs=1.0, cs0=1.0, q=0.5, r0=1.0):
+    n = pos.shape[0]
+    ins = [dev(h), dev(u), dev(pos[:, 0]), dev(pos[:, 1]), dev(pos[:, 2])]
+    P, c = dev(np.zeros(n)), dev(np.zeros(n))
+    check(lib().shamx_eos(stream(), 0, n, eos_kind, gamma, cs, cs0, q, r0,
+                          *[t.data_ptr() for t in ins], P.data_ptr(), c.data_ptr(), mass))
+    torch.cuda.synchronize()
+    return P.cpu().numpy(), c.cpu().numpy()
+
+
+def test_eos_closed_forms_on_gpu():
+    """shamx_eos against the closed forms of the three equations of state (eos.hpp:31-98,
+    ComputeEos.cpp:152-247, 656-665, 726-798): adiabatic P = (gamma - 1) rho u,
+    c_s = sqrt(gamma P / rho); isothermal P = c_s^2 rho with the configured c_s; LP07
+    c_s^2 = c_s0^2 (|r|^2 / r0^2)^(-q), P = c_s^2 rho, with c_s = c_s0 at |r| = r0 and
+    c_s = c_s0 / 2 at |r| = 16 r0 for q = 1/4 (rho = m (hfact / h)^3, M4 hfact 1.2)."""
+    say("[eos]")
+    rng = np.random.default_rng(3)
+    n = 10_001
+    pos = rng.normal(0, 5, (n, 3))
+    pos[0], pos[1], pos[2] = (1, 0, 0), (0, -1, 0), (16, 0, 0)
+    h = rng.uniform(0.05, 0.5, n)
+    u = rng.uniform(0.1, 2.0, n)
+    m = 1e-3
+    rho = m * (1.2 / h) ** 3
+    P, c = _eos_gpu(0, pos, h, u, m, gamma=1.4)
+    np.testing.assert_allclose(P, 0.4 * rho * u, rtol=1e-15)
+    np.testing.assert_allclose(c, np.sqrt(1.4 * P / rho), rtol=1e-15)
+    P, c = _eos_gpu(1, pos, h, u, m, cs=0.37)
+    assert np.all(c == 0.37)
+    np.testing.assert_allclose(P, 0.37 * 0.37 * rho, rtol=1e-15)
+    cs0, q = 0.8, 0.25
+    P, c = _eos_gpu(2, pos, h, u, m, cs0=cs0, q=q, r0=1.0)
+    assert c[0] == pytest.approx(cs0, rel=1e-15) and c[1] == pytest.approx(cs0, rel=1e-15)
+    assert c[2] == pytest.approx(cs0 / 2, rel=1e-15)
+    cs2 = cs0**2 * ((pos**2).sum(1)) ** (-q)
+    np.testing.assert_allclose(c, np.sqrt(cs2), rtol=1e-14)
+    np.testing.assert_allclose(P, cs2 * rho, rtol=1e-14)
+    _, c2 = _eos_gpu(2, pos, h, u, m, cs0=cs0, q=q, r0=2.0)
+    assert c2[2] ** 2 == pytest.approx((cs0 / 2) ** 2 * 4**q, rel=1e-14)
+
+
 def _neigh_gpu(ctx, pos, h, n_local, htol, Rkern):
     g = _tree_gpu(ctx, pos, h, 3, htol)
     check(lib().shamx_neigh_build(ctx, stream(), n_local, g["x"].data_ptr(), g["y"].data_ptr(),
@@ -553,11 +595,55 @@ def test_mixed_precision_error_bound():
     b = GpuSolver(cfg_m, st)
     for s in (a, b):
         s.set_time(*[g64.time()[k] for k in ("t", "dt", "cfl_mult")])
-    a.evolve_once()
-    b.evolve_once()
+    la = a.evolve_once()
+    lb = b.evolve_once()
+    # the same number of force evaluations on both sides (this step reruns its corrector: the
+    # oracle takes 3 passes on the third Sedov 24^3 step), so the accelerations compared come
+    # from the same sequence of corrector passes
+    say(f"[mixed] corrector passes fp64={la['corrector_iterations']} "
+        f"mixed={lb['corrector_iterations']}")
+    assert la["corrector_iterations"] == lb["corrector_iterations"] >= 2
     A = np.stack([a.get(k) for k in ("ax", "ay", "az")])
     B = np.stack([b.get(k) for k in ("ax", "ay", "az")])
     err_a = np.linalg.norm(A - B) / np.linalg.norm(A)
     err_u = np.linalg.norm(a.get("du") - b.get("du")) / np.linalg.norm(a.get("du"))
     say(f"[mixed] rel-L2 a={err_a:.3e} du={err_u:.3e}")
     assert err_a <= 1e-5 and err_u <= 1e-5
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed", "cd10"])
+def test_record_refresh_equals_full_pack(mode, monkeypatch):
+    """Corrector reruns refresh v, u, P and c_s of the force records in place instead of
+    rebuilding them.  SHAMX_FULL_PACK=1 forces the full rebuild: on Sedov 20^3, whose second
+    and third steps rerun their corrector (2 and 3 passes in the oracle), the records of the
+    last force evaluation and every field after each step must be the same bytes either way
+    (FP64, mixed precision, and the varying viscosity, whose alpha the records carry too)."""
+    cfg, f = setups.sedov(20)
+    if mode == "mixed":
+        cfg = dict(cfg, mixed_precision=1)
+    if mode == "cd10":
+        cfg = dict(cfg, av_kind=1, alpha_min=0.0, alpha_max=1.0, sigma_decay=0.1)
+    runs = {}
+    for full in (False, True):
+        if full:
+            monkeypatch.setenv("SHAMX_FULL_PACK", "1")
+        else:
+            monkeypatch.delenv("SHAMX_FULL_PACK", raising=False)
+        g = GpuSolver(cfg, f)
+        out = []
+        for step in range(3):
+            say(f"[refresh] {mode} full_pack={full} step {step}")
+            lg = g.evolve_once()
+            nm = g.sizes()["Nm"]
+            rb = (64 if mode == "mixed" else 144) * nm
+            out.append((lg["corrector_iterations"], g.get_u8(2, rb),
+                        {k: g.get(k) for k in ("vx", "vy", "vz", "ax", "ay", "az", "u", "du",
+                                               "h", "alpha_AV")}))
+        g.close()
+        runs[full] = out
+    assert max(c for c, _, _ in runs[False]) >= 2, "no step reran its corrector"
+    for (ca, ra, fa), (cb, rb_, fb) in zip(runs[False], runs[True]):
+        assert ca == cb
+        np.testing.assert_array_equal(ra, rb_)
+        for k in fa:
+            np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)

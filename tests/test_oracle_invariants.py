@@ -174,3 +174,57 @@ def test_first_step_dt_zero_identity_and_eos():
     dt = np.minimum(0.003 * h / t["vsig"], 0.003 * np.sqrt(h / a)).min()
     assert tm["dt"] == pytest.approx(dt, rel=1e-15)
     assert st["next_dt"] == tm["dt"]
+
+
+def _eos_first_step(eos_kind, **kw):
+    """A free-boundary cubic lattice spanning x in [-2, 17], |y|, |z| <= 2 (spacing 1), so
+    lattice points sit at |r| = r0 = 1 and |r| = 16 r0; one dt = 0 step of the oracle leaves
+    the positions unchanged and evaluates the EOS at them."""
+    g = np.arange(-2, 18, dtype=float)
+    s = np.arange(-2, 3, dtype=float)
+    zz, yy, xx = np.meshgrid(s, s, g, indexing="ij")
+    pos = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1)
+    cfg = setups.base_config("M4", (-3, -3, -3), (18, 3, 3), periodic=False,
+                             gpart_mass=1.0, eos_kind=eos_kind, **kw)
+    s_ = orc.Solver(cfg, setups.fields_from(pos, 1.0, 1.0))
+    s_.evolve_once()
+    t = s_.transient()
+    rho = cfg["gpart_mass"] * (1.2 / t["mh"]) ** 3
+    return pos, t, rho
+
+
+def test_eos_isothermal_closed_form():
+    """Isothermal EOS (eos.hpp:31-35, ComputeEos.cpp:656-665): c_s is the configured constant
+    whatever u is, and P = c_s^2 rho."""
+    cs = 0.37
+    pos, t, rho = _eos_first_step(1, eos_cs=cs)
+    assert np.all(t["cs"] == cs)
+    np.testing.assert_allclose(t["P"], cs * cs * rho, rtol=1e-15)
+    # u plays no part: a P computed from u (adiabatic form) would be (gamma - 1) rho u
+    assert not np.allclose(t["P"], (5 / 3 - 1) * rho * t["mu"], rtol=1e-3)
+
+
+def test_eos_lp07_closed_form():
+    """Locally isothermal LP07 (eos.hpp:86-98, ComputeEos.cpp:726-798):
+    c_s^2 = c_s0^2 (|r|^2 / r0^2)^(-q) with the spherical |r|, P = c_s^2 rho.  At |r| = r0,
+    c_s = c_s0; at |r| = 16 r0 with q = 1/4, c_s = c_s0 / 2 (a slip to (|r| / r0)^(-q) gives
+    c_s0 / 2^(1/2), a sign slip 2 c_s0); P then follows from the same c_s^2."""
+    cs0, q, r0 = 0.8, 0.25, 1.0
+    pos, t, rho = _eos_first_step(2, eos_cs0=cs0, eos_q=q, eos_r0=r0)
+    mpos = np.stack([t["mx"], t["my"], t["mz"]], axis=1)
+    np.testing.assert_array_equal(mpos, pos)
+    r2 = (pos**2).sum(1)
+    at_r0 = r2 == 1.0
+    at_16 = (pos[:, 0] == 16.0) & (pos[:, 1] == 0) & (pos[:, 2] == 0)
+    assert at_r0.sum() == 6 and at_16.sum() == 1
+    np.testing.assert_allclose(t["cs"][at_r0], cs0, rtol=1e-15)
+    np.testing.assert_allclose(t["cs"][at_16], cs0 / 2, rtol=1e-15)
+    np.testing.assert_allclose(t["P"][at_16], (cs0 / 2) ** 2 * rho[at_16], rtol=1e-15)
+    # every other particle against the closed form (the origin, |r| = 0, is singular)
+    ok = r2 > 0
+    cs2 = cs0**2 * (r2[ok] / r0**2) ** (-q)
+    np.testing.assert_allclose(t["cs"][ok], np.sqrt(cs2), rtol=1e-14)
+    np.testing.assert_allclose(t["P"][ok], cs2 * rho[ok], rtol=1e-14)
+    # r0 enters squared: doubling r0 multiplies c_s^2 by 4^q
+    _, t2, _ = _eos_first_step(2, eos_cs0=cs0, eos_q=q, eos_r0=2 * r0)
+    np.testing.assert_allclose(t2["cs"][at_16] ** 2, (cs0 / 2) ** 2 * 4**q, rtol=1e-14)

@@ -2,15 +2,16 @@
 steps with the Cullen & Dehnen viscosity, HCP lattices at resolution 128 (82944 particles), run to
 t = 0.245 with the last step clipped, L2 errors vs the exact Riemann solution against the
 reference's expected values (relative 1e-11, :114-121).  About 7 minutes on 8 cores, so it runs
-only with SHAMX_SLOW=1; tests/golden/sod_oracle_run.py prints the same comparison.
+only with SHAMX_SLOW=1.
 """
 import json
 import os
 
+import numpy as np
 import pytest
 
 import oracle_lib as orc
-from shamrock_amd import analysis, setups
+from shamrock_amd import analysis, phys, setups
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -32,9 +33,11 @@ def test_oracle_sod_tube_ci_l2():
         s.evolve_once()
         steps += 1
         assert steps < 5000
-    out = {k: s.get(k) for k in ("x", "y", "z", "vx", "vy", "vz", "h", "u")}
-    sod = analysis.SodTube(1.4, 1, 1, 0.125, 0.1)
-    rho, v, P = analysis.sod_l2(sod, out, cfg["gpart_mass"], 1.0, 1.4, target)
+    xyz = np.stack([s.get(k) for k in ("x", "y", "z")], 1)
+    vxyz = np.stack([s.get(k) for k in ("vx", "vy", "vz")], 1)
+    sod = phys.SodTube(gamma=1.4, rho_1=1, P_1=1, rho_5=0.125, P_5=0.1)
+    rho, v, P = analysis.sod_l2_dist(sod, xyz, vxyz, s.get("h"), s.get("u"), cfg["gpart_mass"],
+                                     1.0, 1.4, target, (1, 0, 0), 0.0, -0.5, 0.5)
     got = dict(rho=rho, vx=v[0], vy=v[1], vz=v[2], P=P)
     for k, val in got.items():
         e = fx["expect_" + k]
