@@ -119,15 +119,17 @@ namespace shamx {
             int hi = __builtin_amdgcn_mov_dpp((int) (b >> 32), ctrl, 0xf, 0xf, false);
             return __longlong_as_double(((i64) hi << 32) | (u32) lo);
         }
-        // acc + t[lane 0] + t[lane 1] + ... + t[lane G-1], left to right, in every lane of the
-        // group: the G terms of one iteration enter the sum in list order, one IEEE add each,
-        // exactly as in a sequential loop over the row.  Must be reached by the whole group.
+        // acc + t[lane 0] + t[lane 1] + ... + t[lane G-1], left to right, in the first lane of
+        // the group: the G terms of one iteration enter the sum in list order, one IEEE add
+        // each, exactly as in a sequential loop over the row.  The first lane adds its own term
+        // and then the broadcasts of the others (the other lanes carry sums that are not used:
+        // only the group's first lane may read the result).  Must be reached by the whole group.
         template<int G, int L = 0>
         __device__ inline f64 ordered_fold(f64 acc, f64 t) {
             if constexpr (G == 1) {
                 return acc + t;
             } else {
-                acc = acc + group_lane<G, L>(t);
+                acc = acc + (L == 0 ? t : group_lane<G, L>(t));
                 if constexpr (L + 1 < G) return ordered_fold<G, L + 1>(acc, t);
                 else return acc;
             }
@@ -139,8 +141,8 @@ namespace shamx {
                 acc = acc + t1;
                 return acc + t2;
             } else {
-                acc = acc + group_lane<G, L>(t1);
-                acc = acc + group_lane<G, L>(t2);
+                acc = acc + (L == 0 ? t1 : group_lane<G, L>(t1));
+                acc = acc + (L == 0 ? t2 : group_lane<G, L>(t2));
                 if constexpr (L + 1 < G) return ordered_fold2<G, L + 1>(acc, t1, t2);
                 else return acc;
             }
@@ -735,81 +737,90 @@ namespace shamx {
                 // particle's own divisions go through its reciprocals (one loop per case)
                 auto sweep = [&](auto fa_tag) {
                 constexpr bool FA = decltype(fa_tag)::value;
-                // the next neighbour's record is loaded while this one is used (measured:
-                // 3.21 -> 2.98 ms per evaluation at Sedov 128^3)
-                FRec Bn{};
-                if (k0 + lg < k1) Bn = src.get(r.ids[k0 + lg]);
-                for (u32 base = k0; base < k1; base += G) {
-                    u32 k  = base + lg;
-                    f64 tx = 0, ty = 0, tz = 0, tu1 = 0, tu2 = 0;
-                    const FRec Bc = Bn;
-                    if (k + G < k1) Bn = src.get(r.ids[k + G]);
-                    if (k < k1) {
-                        const FRec B = Bc;
-                        f64 dx = xa - B.x, dy = ya - B.y, dz = za - B.z;
-                        f64 rab2 = dx * dx + dy * dy + dz * dz;
-                        if (!(rab2 > ca && rab2 > rec_hR2<K>(B))) {
-                            npair++;
-                            f64 rab   = sqrt(rab2);
-                            // dW = norm f'(r / h) / h^4 (sphkernels.hpp:2329-2343) and
-                            // Fab_b / (rho_b omega_b) of the conductivity term, the divisions
-                            // by way of the particles' reciprocals
-                            f64 Fab_a, Fab_b, rho_b, fb_rom;
-                            if constexpr (FA)
-                                Fab_a = div_rcp(KK::norm * K::df(div_rcp(rab, h_a, A.rh)), h4_a,
-                                                A.rh4);
-                            else
-                                Fab_a = KK::dW(rab, h_a);
-                            rho_b = B.rho;
-                            if (B.rh != 0) {
-                                f64 h4_b = B.h * B.h * B.h * B.h;
-                                Fab_b  = div_rcp(KK::norm * K::df(div_rcp(rab, B.h, B.rh)), h4_b,
-                                                 B.rh4);
-                                fb_rom = div_rcp(Fab_b, B.romega, B.rromega);
-                            } else {
-                                Fab_b  = KK::dW(rab, B.h);
-                                fb_rom = Fab_b / B.romega;
-                            }
-                            f64 vabx = vxa - B.vx, vaby = vya - B.vy, vabz = vza - B.vz;
-                            f64 inv  = inv_sat_positive(rab);
-                            f64 rux = dx * inv, ruy = dy * inv, ruz = dz * inv;
-                            f64 v_ab_r_ab = vabx * rux + vaby * ruy + vabz * ruz;
-                            f64 abs_v     = fabs(v_ab_r_ab);
-                            f64 vsig_a = alpha_a * cs_a + fa.beta_AV * abs_v;
-                            f64 vsig_b = (fa.varying ? B.alpha : fa.alpha_AV) * B.cs +
-                                         fa.beta_AV * abs_v;
-                            f64 rho_avg = (rho_a + rho_b) * 0.5;
-                            f64 vsig_u  = sqrt(fabs(P_a - B.P) / rho_avg);
-                            f64 qa_ab = fmax(-0.5 * rho_a * vsig_a * v_ab_r_ab, 0.);
-                            f64 qb_ab = fmax(-0.5 * rho_b * vsig_b * v_ab_r_ab, 0.);
-                            f64 AV_P_a = P_a + qa_ab, AV_P_b = B.P + qb_ab;
-                            f64 nax = rux * Fab_a, nay = ruy * Fab_a, naz = ruz * Fab_a;
-                            f64 nbx = rux * Fab_b, nby = ruy * Fab_b, nbz = ruz * Fab_b;
-                            f64 ia = AV_P_a * ia_fac;
-                            f64 ib = AV_P_b * B.inv_sub;
-                            f64 mm = -pmass;
-                            tx     = mm * (ia * nax + ib * nbx);
-                            ty     = mm * (ia * nay + ib * nby);
-                            tz     = mm * (ia * naz + ib * nbz);
-                            tu1    = AV_P_a * (omega_a_rho_a_inv * rho_a_inv) * pmass *
-                                  (vabx * nax + vaby * nay + vabz * naz);
-                            tu2 = pmass * fa.alpha_u * vsig_u * (u_a - B.u) * 0.5 *
-                                  (Fab_a * omega_a_rho_a_inv + fb_rom);
-                            // CFL signal speed: r_ab / |r_ab|, alpha 1, beta 2.  inv is
-                            // RN(1 / rab) from 1e-9 on and 0 below, where r_ab / |r_ab| is
-                            // taken as 0: div_rcp gives both
-                            f64 r2x = div_rcp(dx, rab, inv), r2y = div_rcp(dy, rab, inv),
-                                r2z = div_rcp(dz, rab, inv);
-                            f64 vdot = vabx * r2x + vaby * r2y + vabz * r2z;
-                            vsig_max = fmax(vsig_max, 1.0 * cs_a + 2.0 * fabs(vdot));
-                        }
+                // the pair terms of entry k with neighbour record B; a skipped entry leaves
+                // +0.0 terms, which leave a sum that starts at +0.0 unchanged (it can never
+                // become -0.0 under round-to-nearest)
+                auto pair = [&](const FRec &B, u32 k, f64 &tx, f64 &ty, f64 &tz, f64 &tu1,
+                                f64 &tu2) {
+                    if (k >= k1) return;
+                    f64 dx = xa - B.x, dy = ya - B.y, dz = za - B.z;
+                    f64 rab2 = dx * dx + dy * dy + dz * dz;
+                    if (rab2 > ca && rab2 > rec_hR2<K>(B)) return;
+                    npair++;
+                    f64 rab = sqrt(rab2);
+                    // dW = norm f'(r / h) / h^4 (sphkernels.hpp:2329-2343) and
+                    // Fab_b / (rho_b omega_b) of the conductivity term, the divisions
+                    // by way of the particles' reciprocals
+                    f64 Fab_a, Fab_b, rho_b, fb_rom;
+                    if constexpr (FA)
+                        Fab_a = div_rcp(KK::norm * K::df(div_rcp(rab, h_a, A.rh)), h4_a, A.rh4);
+                    else
+                        Fab_a = KK::dW(rab, h_a);
+                    rho_b = B.rho;
+                    if (B.rh != 0) {
+                        f64 h4_b = B.h * B.h * B.h * B.h;
+                        Fab_b  = div_rcp(KK::norm * K::df(div_rcp(rab, B.h, B.rh)), h4_b, B.rh4);
+                        fb_rom = div_rcp(Fab_b, B.romega, B.rromega);
+                    } else {
+                        Fab_b  = KK::dW(rab, B.h);
+                        fb_rom = Fab_b / B.romega;
                     }
-                    // a skipped entry adds +0.0, which leaves a sum that starts at +0.0
-                    // unchanged (it can never become -0.0 under round-to-nearest)
+                    f64 vabx = vxa - B.vx, vaby = vya - B.vy, vabz = vza - B.vz;
+                    f64 inv  = inv_sat_positive(rab);
+                    f64 rux = dx * inv, ruy = dy * inv, ruz = dz * inv;
+                    f64 v_ab_r_ab = vabx * rux + vaby * ruy + vabz * ruz;
+                    f64 abs_v     = fabs(v_ab_r_ab);
+                    f64 vsig_a = alpha_a * cs_a + fa.beta_AV * abs_v;
+                    f64 vsig_b = (fa.varying ? B.alpha : fa.alpha_AV) * B.cs + fa.beta_AV * abs_v;
+                    f64 rho_avg = (rho_a + rho_b) * 0.5;
+                    f64 vsig_u  = sqrt(fabs(P_a - B.P) / rho_avg);
+                    f64 qa_ab = fmax(-0.5 * rho_a * vsig_a * v_ab_r_ab, 0.);
+                    f64 qb_ab = fmax(-0.5 * rho_b * vsig_b * v_ab_r_ab, 0.);
+                    f64 AV_P_a = P_a + qa_ab, AV_P_b = B.P + qb_ab;
+                    f64 nax = rux * Fab_a, nay = ruy * Fab_a, naz = ruz * Fab_a;
+                    f64 nbx = rux * Fab_b, nby = ruy * Fab_b, nbz = ruz * Fab_b;
+                    f64 ia = AV_P_a * ia_fac;
+                    f64 ib = AV_P_b * B.inv_sub;
+                    f64 mm = -pmass;
+                    tx     = mm * (ia * nax + ib * nbx);
+                    ty     = mm * (ia * nay + ib * nby);
+                    tz     = mm * (ia * naz + ib * nbz);
+                    tu1    = AV_P_a * (omega_a_rho_a_inv * rho_a_inv) * pmass *
+                          (vabx * nax + vaby * nay + vabz * naz);
+                    tu2 = pmass * fa.alpha_u * vsig_u * (u_a - B.u) * 0.5 *
+                          (Fab_a * omega_a_rho_a_inv + fb_rom);
+                    // CFL signal speed: r_ab / |r_ab|, alpha 1, beta 2.  inv is RN(1 / rab)
+                    // from 1e-9 on and 0 below, where r_ab / |r_ab| is taken as 0: div_rcp
+                    // gives both
+                    f64 r2x = div_rcp(dx, rab, inv), r2y = div_rcp(dy, rab, inv),
+                        r2z = div_rcp(dz, rab, inv);
+                    f64 vdot = vabx * r2x + vaby * r2y + vabz * r2z;
+                    vsig_max = fmax(vsig_max, 1.0 * cs_a + 2.0 * fabs(vdot));
+                };
+                auto step = [&](const FRec &B, u32 k) {
+                    f64 tx = 0, ty = 0, tz = 0, tu1 = 0, tu2 = 0;
+                    pair(B, k, tx, ty, tz, tu1, tu2);
                     fx  = ordered_fold<G>(fx, tx);
                     fy  = ordered_fold<G>(fy, ty);
                     fz  = ordered_fold<G>(fz, tz);
                     tdu = ordered_fold2<G>(tdu, tu1, tu2);
+                };
+                // two records in flight, used in turn: the next neighbour's record is loaded
+                // while the current one is used (3.21 -> 2.98 ms per evaluation at Sedov
+                // 128^3), and unrolling by two keeps the records in place instead of copying
+                // 144 bytes of registers per entry.  Trip counts are uniform in the group.
+                // Loads are unconditional (entries past the row end read its last entry again,
+                // a cached line), so no record has to be kept across a skipped load.
+                if (k0 < k1) {
+                    const u32 last = k1 - 1;
+                    FRec B0 = src.get(r.ids[min(k0 + lg, last)]);
+                    for (u32 base = k0; base < k1; base += 2 * G) {
+                        u32 k          = base + lg;
+                        const FRec B1  = src.get(r.ids[min(k + G, last)]);
+                        step(B0, k);
+                        B0 = src.get(r.ids[min(k + 2 * G, last)]);
+                        if (base + G < k1) step(B1, k + G);
+                    }
                 }
                 };
                 if (A.rh != 0) sweep(std::true_type{});
@@ -861,17 +872,12 @@ namespace shamx {
                 const f32 c_du1 = omega_a_rho_a_inv * rho_a_inv * pmass;
                 const f64 ca = h_a64 * h_a64 * Rker2;
                 u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
-                // the next neighbour's record is loaded while this one is used
-                FRec32 Bn{};
-                if (k0 + lg < k1) Bn = src.get(r.ids[k0 + lg]);
-                for (u32 k = k0 + lg; k < k1; k += G) {
-                    const FRec32 B = Bn;
-                    if (k + G < k1) Bn = src.get(r.ids[k + G]);
+                auto pair = [&](const FRec32 &B) {
                     f64 dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
                     f64 rab2 = dx * dx + dy * dy + dz * dz;
                     // neighbour-side criterion in FP32 (the solver's rows are prefiltered in
                     // FP64 already; this only matters for unfiltered reference-layout rows)
-                    if (rab2 > ca && (f32) rab2 * (B.inv_h * B.inv_h) > (f32) Rker2) continue;
+                    if (rab2 > ca && (f32) rab2 * (B.inv_h * B.inv_h) > (f32) Rker2) return;
                     npair++;
                     f32 rab  = sqrt_approx((f32) rab2);
                     f32 inv  = (rab >= 1e-9f) ? rcp_approx(rab) : 0.f;
@@ -900,6 +906,19 @@ namespace shamx {
                              (Fab_a * omega_a_rho_a_inv + Fab_b * B.inv_romega);
                     kahan_add(su, cu, t1 + t2);
                     vsig_max = fmaxf(vsig_max, cs_a + 2.f * abs_v);
+                };
+                // two records in flight, used in turn (no register copies between entries);
+                // loads past the row end read its last entry again (a cached line)
+                if (k0 < k1) {
+                    const u32 last = k1 - 1;
+                    FRec32 B0 = src.get(r.ids[min(k0 + lg, last)]);
+                    for (u32 base = k0; base < k1; base += 2 * G) {
+                        u32 k           = base + lg;
+                        const FRec32 B1 = src.get(r.ids[min(k + G, last)]);
+                        if (k < k1) pair(B0);
+                        B0 = src.get(r.ids[min(k + 2 * G, last)]);
+                        if (k + G < k1) pair(B1);
+                    }
                 }
             }
             f64 fx   = group_sum<G>((f64) sx - (f64) cx);
