@@ -71,6 +71,9 @@ typedef struct shamx_config {
                               step); every field and result keeps the caller's particle order
                               at the interface.  1: keep the caller's order in storage too
                               (required by the adjoint entries) */
+    int32_t adjoint_fp32;  /* no reference counterpart: 0 = the adjoint entries compute in FP64;
+                              1 = their pair terms in FP32 with Kahan-compensated sums, on FP64
+                              r_ab and FP64 neighbour criteria (the forward step is unchanged) */
 } shamx_config;
 
 /* Per-step log (reference TimestepLog Solver.hpp:43-54 + per-phase timers).  Times in s. */
@@ -112,7 +115,7 @@ int shamx_sort(shamx_ctx *ctx, void *stream, uint32_t n, uint32_t key_bits, uint
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
  * arguments, results and stream semantics as shamx_sort, without a context: the alternate key
  * and value buffers and the sort's scratch live in one static device array of the library
- * (about 36 MB per GPU, held until the process ends), so n is at most 4194304 keys
+ * (about 40 MB per GPU, held until the process ends), so n is at most 4194304 keys
  * (SHAMX_E_INVALID_ARG beyond).  The call allocates nothing.  Every call on one GPU uses that
  * GPU's array: calls that overlap on one GPU (two streams, two threads) race. */
 int shamx_sort_noalloc(void *stream, uint32_t n, uint32_t key_bits, uint32_t *keys,
@@ -217,6 +220,13 @@ int shamx_solver_destroy(shamx_solver *s);
  * `on_device` selects a device or host source/destination of n doubles */
 int shamx_solver_set_field(shamx_solver *s, int field, const double *src, int on_device);
 int shamx_solver_get_field(shamx_solver *s, int field, double *dst, int on_device);
+/* Storage order of the local particles, for restart dumps (the reference's dump holds its
+ * patch data in storage order, Model.hpp:959-991, so a continued run sums in the same order).
+ * get: host array of n caller particle ids, the order the next step runs in (a pending
+ * reorder included); set: store the particles in that order (a permutation of 0..n-1; only
+ * the identity when keep_storage_order = 1).  Field values per particle do not change. */
+int shamx_solver_get_storage_order(shamx_solver *s, uint32_t *order);
+int shamx_solver_set_storage_order(shamx_solver *s, const uint32_t *order);
 /* time state edges (Solver.hpp:92-148): tdc = {t, dt, cfl_multiplier} */
 int shamx_solver_get_time(shamx_solver *s, double *tdc3);
 int shamx_solver_set_time(shamx_solver *s, const double *tdc3);

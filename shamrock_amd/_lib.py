@@ -36,7 +36,7 @@ class ShamxConfig(C.Structure):
         ("mixed_precision", C.c_int32), ("av_kind", C.c_int32),
         ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("sigma_decay", C.c_double),
         ("constant_G", C.c_double), ("sink_softening", C.c_double),
-        ("keep_storage_order", C.c_int32),
+        ("keep_storage_order", C.c_int32), ("adjoint_fp32", C.c_int32),
     ]
 
 
@@ -97,6 +97,8 @@ SIGNATURES = {
     "shamx_solver_destroy": (_i, [_P]),
     "shamx_solver_set_field": (_i, [_P, _i, _P, _i]),
     "shamx_solver_get_field": (_i, [_P, _i, _P, _i]),
+    "shamx_solver_get_storage_order": (_i, [_P, _P]),
+    "shamx_solver_set_storage_order": (_i, [_P, _P]),
     "shamx_solver_get_time": (_i, [_P, _dp]),
     "shamx_solver_set_time": (_i, [_P, _dp]),
     "shamx_solver_add_sink": (_i, [_P, C.c_double, _dp, _dp, C.c_double]),

@@ -7,7 +7,11 @@ density rho_h(h) of the final state; gradients flow to v0 and u0.  `rollout_stat
 final density and internal energy (for losses on both, e.g. an entropy sum).  Sinks (free
 boundaries) are part of the differentiated state: their seeds are carried by the solver.  The time steps are those the
 solver chooses in the forward pass (or `dts` when given) and are held fixed when
-differentiating.  Memory: one state every `checkpoint_every` steps plus one segment; the
+differentiating.  A step whose corrector reran is differentiated through every pass (pass i
+evaluates the force from the velocities and u of pass i - 1; the solver keeps them).
+cfg["adjoint_fp32"] = 1 computes the adjoint's pair terms in FP32 (Kahan sums, FP64 r_ab);
+cfg["mixed_precision"] = 1 runs the forward steps in mixed precision, whose map is then
+differentiated as the FP64 step at the mixed trajectory's states.  Memory: one state every `checkpoint_every` steps plus one segment; the
 backward pass replays each segment from its checkpoint (the step is deterministic, so replays
 are bit-identical) and runs the adjoint of every step on the replayed intermediate products;
 the last step of a segment is run once (its products are the solver's when the replay of the
@@ -38,14 +42,16 @@ def _density(cfg, h):
     return cfg["gpart_mass"] * (hf / h) ** 3
 
 
-def _step(solver, dt):
+def _step(solver, dt, passes=None):
+    """One step (of prescribed dt when given); returns (dt, corrector passes).  A replay must
+    take the passes of the step it repeats."""
     if dt is not None:
         solver.set_time(dt=dt)
     log = solver.evolve_once()
-    if log["corrector_iterations"] != 1:
-        raise RuntimeError("rollout: a step reran its corrector; the adjoint needs one pass "
-                           "(lower the CFL numbers or prescribe the time steps)")
-    return log["dt"]
+    if passes is not None and log["corrector_iterations"] != passes:
+        raise RuntimeError("rollout: a replayed step took another number of corrector passes "
+                           "than the forward pass")
+    return log["dt"], log["corrector_iterations"]
 
 
 class _Rollout:
@@ -62,12 +68,14 @@ class _Rollout:
         for i, k in enumerate(("vx", "vy", "vz")):
             s.set_device(k, v0[:, i].detach().contiguous())
         s.set_device("u", u0.detach().contiguous())
-        self.ckpt, used = {}, []
+        self.ckpt, used, passes = {}, [], []
         for n in range(self.steps):
             if n % self.k == 0:
                 self.ckpt[n] = s.snapshot()
-            used.append(_step(s, None if self.dts is None else self.dts[n]))
-        self.dts_used = used
+            dt, p = _step(s, None if self.dts is None else self.dts[n])
+            used.append(dt)
+            passes.append(p)
+        self.dts_used, self.passes_used = used, passes
         self.solver = s
         rho = _density(self.cfg, s.get_device("h"))
         u = s.get_device("u")
@@ -97,13 +105,14 @@ class _Rollout:
             starts = []
             for n in seg[:-1]:
                 starts.append(s.snapshot())
-                _step(s, self.dts_used[n])
+                _step(s, self.dts_used[n], self.passes_used[n])
             # the last step of the segment runs once, and its adjoint reads its products
-            _step(s, self.dts_used[seg[-1]])
+            _step(s, self.dts_used[seg[-1]], self.passes_used[seg[-1]])
             s.adjoint_step(seeds)
             for n, st in zip(reversed(seg[:-1]), reversed(starts)):
                 s.restore(st)
-                _step(s, self.dts_used[n])  # replay: the adjoint reads this step's products
+                # replay: the adjoint reads this step's products
+                _step(s, self.dts_used[n], self.passes_used[n])
                 s.adjoint_step(seeds)
         torch.cuda.synchronize()
         vbar = torch.stack(seeds[3:6], dim=1)
@@ -155,4 +164,12 @@ def rollout_dts(cfg, fields, v0, u0, steps):
     return r.dts_used
 
 
-__all__ = ["rollout_density", "rollout_state", "rollout_dts", "FIELDS"]
+def rollout_passes(cfg, fields, v0, u0, steps, dts=None):
+    """The corrector passes (1 + reruns) of every step of the rollout."""
+    r = _Rollout(cfg, fields, steps, steps, dts)
+    r.forward(v0, u0)
+    r.solver.close()
+    return r.passes_used
+
+
+__all__ = ["rollout_density", "rollout_state", "rollout_dts", "rollout_passes", "FIELDS"]

@@ -56,74 +56,112 @@ namespace shamx {
             f64 *gx, *gy, *gz, *gvx, *gvy, *gvz, *gu, *gh, *gom;
         };
 
-        template<class K>
+        // a sum in the scalar T: plain for FP64 (the order of the FP64 sums is kept), Kahan
+        // compensated for FP32; value() in FP64
+        template<class T>
+        struct Acc;
+        template<>
+        struct Acc<f64> {
+            f64 s = 0;
+            __device__ void add(f64 x) { s += x; }
+            __device__ f64 value() const { return s; }
+        };
+        template<>
+        struct Acc<f32> {
+            f32 s = 0, c = 0;
+            __device__ void add(f32 x) {
+                f32 y = x - c, t = s + y;
+                c = (t - s) - y;
+                s = t;
+            }
+            __device__ f64 value() const { return (f64) s - (f64) c; }
+        };
+
+        // T = f64: every operation FP64.  T = f32 (the FP32 adjoint): r_ab and the neighbour
+        // criteria in FP64, the pair terms and their derivatives in FP32, Kahan-compensated
+        // sums, results stored in FP64.
+        template<class K, class T>
         __global__ void __launch_bounds__(ABLK)
             k_force_vjp(u32 n, Rows1 r, const FRecView *__restrict rec,
                         const f64 *__restrict omega_m, const f64 *__restrict sax,
                         const f64 *__restrict say, const f64 *__restrict saz,
-                        const f64 *__restrict sdu, PairParams pp, f64 lp_q, ForceGrad o) {
+                        const f64 *__restrict sdu, PairParams pp, f64 lp_q, int accum,
+                        ForceGrad o) {
             u32 j = blockIdx.x * ABLK + threadIdx.x;
             if (j >= n) return;
             u32 row = r.sel[j], a = r.outi[row];
             const FRecView A = rec[row];
-            const PairSide SA{{A.vx, A.vy, A.vz}, A.u, A.h, omega_m[a], A.cs};
-            const f64 seed_a[3] = {sax[row], say[row], saz[row]};
-            const f64 sdu_a     = sdu[row];
-            f64 gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gu = 0, gh = 0, gom = 0, gcs = 0;
+            const PairSideT<T> SA{{T(A.vx), T(A.vy), T(A.vz)}, T(A.u), T(A.h), T(omega_m[a]),
+                                  T(A.cs)};
+            const T seed_a[3] = {T(sax[row]), T(say[row]), T(saz[row])};
+            const T sdu_a     = T(sdu[row]);
+            Acc<T> gx[3], gv[3], gu, gh, gom, gcs;
             u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
             for (u32 k = k0; k < k1; k++) {
                 u32 b = r.ids[k];
                 if (b == row) continue;
                 const FRecView B = rec[b];
-                const PairSide SB{{B.vx, B.vy, B.vz}, B.u, B.h, omega_m[r.outi[b]], B.cs};
-                f64 rab[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
-                f64 acc[3], du, gr[3];
-                PairGrad g1, g2;
+                const PairSideT<T> SB{{T(B.vx), T(B.vy), T(B.vz)}, T(B.u), T(B.h),
+                                      T(omega_m[r.outi[b]]), T(B.cs)};
+                T rab[3] = {T(A.x - B.x), T(A.y - B.y), T(A.z - B.z)};
+                T acc[3], du, gr[3];
+                PairGradT<T> g1, g2;
                 // own term: a's outputs seeded by a's seeds
-                force_pair<K>(pp, rab, SA, SB, acc, du, seed_a, sdu_a, gr, g1, g2);
+                force_pair<K, T>(pp, rab, SA, SB, acc, du, seed_a, sdu_a, gr, g1, g2);
                 for (int c = 0; c < 3; c++) {
-                    gx[c] += gr[c];
-                    gv[c] += g1.v[c];
+                    gx[c].add(gr[c]);
+                    gv[c].add(g1.v[c]);
                 }
-                gu += g1.u;
-                gh += g1.h;
-                gom += g1.omega;
-                gcs += g1.cs;
+                gu.add(g1.u);
+                gh.add(g1.h);
+                gom.add(g1.omega);
+                gcs.add(g1.cs);
                 // mirrored term: b's outputs (pair b, a) seeded by b's seeds; a is side B
-                const f64 seed_b[3] = {sax[b], say[b], saz[b]};
-                f64 rba[3]          = {-rab[0], -rab[1], -rab[2]};
-                force_pair<K>(pp, rba, SB, SA, acc, du, seed_b, sdu[b], gr, g1, g2);
+                const T seed_b[3] = {T(sax[b]), T(say[b]), T(saz[b])};
+                T rba[3]          = {-rab[0], -rab[1], -rab[2]};
+                force_pair<K, T>(pp, rba, SB, SA, acc, du, seed_b, T(sdu[b]), gr, g1, g2);
                 for (int c = 0; c < 3; c++) {
-                    gx[c] -= gr[c];
-                    gv[c] += g2.v[c];
+                    gx[c].add(-gr[c]);
+                    gv[c].add(g2.v[c]);
                 }
-                gu += g2.u;
-                gh += g2.h;
-                gom += g2.omega;
-                gcs += g2.cs;
+                gu.add(g2.u);
+                gh.add(g2.h);
+                gom.add(g2.omega);
+                gcs.add(g2.cs);
             }
+            f64 px = gx[0].value(), py = gx[1].value(), pz = gx[2].value();
             if (pp.eos_kind == 2) {
                 // LP07: c_s = c_s0 (|x| / r0)^(-q), so d c_s / dx = -q c_s x / |x|^2
                 const f64 R2 = A.x * A.x + A.y * A.y + A.z * A.z;
-                const f64 c  = R2 > 0 ? gcs * (-lp_q * A.cs / R2) : 0.;
-                gx[0] += c * A.x;
-                gx[1] += c * A.y;
-                gx[2] += c * A.z;
+                const f64 c  = R2 > 0 ? gcs.value() * (-lp_q * A.cs / R2) : 0.;
+                px += c * A.x;
+                py += c * A.y;
+                pz += c * A.z;
             }
-            o.gx[a]  = gx[0];
-            o.gy[a]  = gx[1];
-            o.gz[a]  = gx[2];
-            o.gvx[a] = gv[0];
-            o.gvy[a] = gv[1];
-            o.gvz[a] = gv[2];
-            o.gu[a]  = gu;
-            o.gh[a]  = gh;
-            o.gom[a] = gom;
+            // position, h and Omega gradients add up over the corrector passes of a step (the
+            // passes share x, h and Omega); the velocity and u gradients are the pass's own
+            if (accum) {
+                o.gx[a] += px;
+                o.gy[a] += py;
+                o.gz[a] += pz;
+                o.gh[a] += gh.value();
+                o.gom[a] += gom.value();
+            } else {
+                o.gx[a]  = px;
+                o.gy[a]  = py;
+                o.gz[a]  = pz;
+                o.gh[a]  = gh.value();
+                o.gom[a] = gom.value();
+            }
+            o.gvx[a] = gv[0].value();
+            o.gvy[a] = gv[1].value();
+            o.gvz[a] = gv[2].value();
+            o.gu[a]  = gu.value();
         }
 
         // Omega_a = 1 + C_a sum_b m dW/dh(d_ab, h_a), C = h / (3 rho_h(h)) = h^4 / (3 m hfact^3),
         // over d_ab <= R h_a: x-gradient (own + mirrored) added to g, h-gradient to gh
-        template<class K>
+        template<class K, class T>
         __global__ void __launch_bounds__(ABLK)
             k_omega_vjp(u32 n, Rows1 r, const double4 *__restrict ps,
                         const f64 *__restrict s_omb, f64 mass, f64 hfact, f64 *__restrict gx,
@@ -137,7 +175,8 @@ namespace shamx {
             f64 rho_a = rho_h(mass, h_a, hfact);
             f64 C_a   = h_a / (3 * rho_a);
             f64 ob_a  = s_omb[row];
-            f64 S1 = 0, S2 = 0, g[3] = {0, 0, 0};
+            const T mT = T(mass), own_c = T(ob_a * C_a * mass);
+            Acc<T> S1, S2, g[3];
             u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
             for (u32 k = k0; k < k1; k++) {
                 u32 b     = r.ids[k];
@@ -146,29 +185,29 @@ namespace shamx {
                 f64 d     = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
                 f64 inv   = d > 1e-300 ? 1. / d : 0.;
                 if (d <= R * h_a) {
-                    KernDeriv<K> kd(d, h_a);
-                    S1 += mass * kd.Wh;
-                    S2 += mass * kd.Whh;
-                    f64 c = ob_a * C_a * mass * kd.Wrh * inv;
-                    for (int t = 0; t < 3; t++) g[t] += c * rr[t];
+                    KernDeriv<K, T> kd((T) d, (T) h_a);
+                    S1.add(mT * kd.Wh);
+                    S2.add(mT * kd.Whh);
+                    T c = own_c * kd.Wrh * T(inv);
+                    for (int t = 0; t < 3; t++) g[t].add(c * T(rr[t]));
                 }
                 f64 h_b = B.w;
                 if (b != row && d <= R * h_b) {
-                    KernDeriv<K> kd(d, h_b);
+                    KernDeriv<K, T> kd((T) d, (T) h_b);
                     f64 C_b = h_b / (3 * rho_h(mass, h_b, hfact));
-                    f64 c   = s_omb[b] * C_b * mass * kd.Wrh * inv;
-                    for (int t = 0; t < 3; t++) g[t] += c * rr[t];
+                    T c     = T(s_omb[b] * C_b * mass) * kd.Wrh * T(inv);
+                    for (int t = 0; t < 3; t++) g[t].add(c * T(rr[t]));
                 }
             }
-            gx[a] += g[0];
-            gy[a] += g[1];
-            gz[a] += g[2];
+            gx[a] += g[0].value();
+            gy[a] += g[1].value();
+            gz[a] += g[2].value();
             // dOmega/dh = (4 C / h) S1 + C S2
-            gh[a] += ob_a * ((4 * C_a / h_a) * S1 + C_a * S2);
+            gh[a] += ob_a * ((4 * C_a / h_a) * S1.value() + C_a * S2.value());
         }
 
         // x-gradient of sum_a lambda_a F_a, F_a = sum_b m W(d_ab, h_a) - rho_h(h_a)
-        template<class K>
+        template<class K, class T>
         __global__ void __launch_bounds__(ABLK)
             k_ift_vjp(u32 n, Rows1 r, const double4 *__restrict ps, const f64 *__restrict s_lam,
                       f64 mass, f64 *__restrict gx, f64 *__restrict gy, f64 *__restrict gz) {
@@ -178,7 +217,8 @@ namespace shamx {
             u32 row = r.sel[j], a = r.outi[row];
             double4 A = ps[row];
             f64 lam_a = s_lam[row];
-            f64 g[3]  = {0, 0, 0};
+            const T own_c = T(lam_a * mass);
+            Acc<T> g[3];
             u32 k0 = r.off[row], k1 = k0 + r.cnt[row];
             for (u32 k = k0; k < k1; k++) {
                 u32 b = r.ids[k];
@@ -187,15 +227,15 @@ namespace shamx {
                 f64 rr[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
                 f64 d     = sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
                 if (!(d > 0)) continue;
-                f64 c = 0;
-                if (d <= R * A.w) c += lam_a * mass * KernDeriv<K>(d, A.w).Wr;
-                if (d <= R * B.w) c += s_lam[b] * mass * KernDeriv<K>(d, B.w).Wr;
-                c /= d;
-                for (int t = 0; t < 3; t++) g[t] += c * rr[t];
+                T c = 0;
+                if (d <= R * A.w) c += own_c * KernDeriv<K, T>(T(d), T(A.w)).Wr;
+                if (d <= R * B.w) c += T(s_lam[b] * mass) * KernDeriv<K, T>(T(d), T(B.w)).Wr;
+                c /= T(d);
+                for (int t = 0; t < 3; t++) g[t].add(c * T(rr[t]));
             }
-            gx[a] += g[0];
-            gy[a] += g[1];
-            gz[a] += g[2];
+            gx[a] += g[0].value();
+            gy[a] += g[1].value();
+            gz[a] += g[2].value();
         }
 
         // lambda_a = -hbar_a / (3 rho_a Omega_a / h_a)
@@ -234,6 +274,40 @@ namespace shamx {
             f64 ub   = s.u[i];
             fdu[i]   = s.du[i] + hdt * ub;
             s.du[i]  = -hdt * ub;
+        }
+
+        // one corrector pass further back: pass i computed a_i = F(v_{i-1}, u_{i-1}) and
+        // v_i = v_{i-1} + dt/2 (a_i - a_{i-1}).  With the pass's own v and u gradients g, the
+        // seed of v_{i-1} is vbar_i + g, the seed of a_{i-1} (the previous pass's force) is
+        // dt/2 g, and the seed of the step start's a is -dt/2 vbar_1 (rewritten at each pass,
+        // the last write is pass 2's)
+        __global__ void k_pass_vjp(u32 n, f64 hdt, StepSeeds s, ForceGrad g,
+                                   f64 *__restrict fax, f64 *__restrict fay,
+                                   f64 *__restrict faz, f64 *__restrict fdu) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            f64 *fa[3]       = {fax, fay, faz};
+            const f64 gv[3]  = {g.gvx[i], g.gvy[i], g.gvz[i]};
+            for (int c = 0; c < 3; c++) {
+                f64 vb    = s.v[c][i] + gv[c];
+                s.v[c][i] = vb;
+                fa[c][i]  = hdt * gv[c];
+                s.a[c][i] = -hdt * vb;
+            }
+            f64 gu  = g.gu[i];
+            f64 ub  = s.u[i] + gu;
+            s.u[i]  = ub;
+            fdu[i]  = hdt * gu;
+            s.du[i] = -hdt * ub;
+        }
+
+        __global__ void k_accumulate3(u32 n, const f64 *__restrict ax, const f64 *__restrict ay,
+                                      const f64 *__restrict az, f64 *__restrict sum) {
+            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= n) return;
+            sum[i] += ax[i];
+            sum[n + i] += ay[i];
+            sum[2 * (size_t) n + i] += az[i];
         }
 
         // add the force / Omega / IFT gradients, then the predictor adjoint
@@ -310,6 +384,14 @@ namespace shamx {
                 throw std::invalid_argument("adjoint: available for the M4, M6, M8 and C2 kernels");
             }
         }
+        // kernel and scalar of the pair terms (FP64, or FP32 when A.fp32)
+        template<class F>
+        void dispatch_adjoint_scalar(const AdjointArgs &A, F &&f) {
+            dispatch_adjoint_kernel(A.kernel, [&](auto K) {
+                if (A.fp32) f(K, f32{});
+                else f(K, f64{});
+            });
+        }
     } // namespace
 
     void adjoint_gather_sorted(hipStream_t s, u32 nm, u32 N, const u32 *sort_index,
@@ -327,12 +409,16 @@ namespace shamx {
     }
 
     void adjoint_step(hipStream_t s, const AdjointArgs &A, f64 *const *seeds,
-                      DevArray<f64> &work, const SinkAdjoint *sk) {
+                      DevArray<f64> &work, const SinkAdjoint *sk, u32 passes,
+                      const std::function<void(u32)> *records_for_pass) {
         const u32 N = A.n, nm = A.nm;
         if (N == 0) return;
+        if (passes == 0) throw std::invalid_argument("adjoint: a step has at least one pass");
+        const bool sum_ext = sk && sk->count && passes > 1;
         // work: 4 force seeds (local) + 4 sorted force seeds + 9 gradients + 2 sorted (Omega
-        // bar, lambda) + lambda (local)
-        size_t need = (size_t) 4 * N + 4 * (size_t) nm + 9 * (size_t) N + 2 * (size_t) nm + N;
+        // bar, lambda) + lambda (local) [+ 3 sums of the force seeds over the passes]
+        size_t need = (size_t) 4 * N + 4 * (size_t) nm + 9 * (size_t) N + 2 * (size_t) nm + N +
+                      (sum_ext ? 3 * (size_t) N : 0);
         work.resize(need);
         f64 *w    = work.data();
         f64 *fa[4] = {w, w + N, w + 2 * (size_t) N, w + 3 * (size_t) N};
@@ -345,6 +431,8 @@ namespace shamx {
         f64 *s_omb = w, *s_lam = w + nm;
         w += 2 * (size_t) nm;
         f64 *lam = w;
+        w += N;
+        f64 *fext = sum_ext ? w : nullptr;
 
         StepSeeds ss{{seeds[0], seeds[1], seeds[2]},
                      {seeds[3], seeds[4], seeds[5]},
@@ -353,35 +441,50 @@ namespace shamx {
                      seeds[10]};
         const f64 dt = A.dt;
         k_corrector_vjp<<<div_up(N, 256), 256, 0, s>>>(N, dt / 2, ss, fa[0], fa[1], fa[2], fa[3]);
-        const f64 *fin[4] = {fa[0], fa[1], fa[2], fa[3]};
-        adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 4, fin, sf);
+        if (sum_ext) fill_f64(s, fext, 0., 3 * (size_t) N);
         PairParams pp{A.mass,     A.hfact,   A.gamma,    A.alpha_u, A.alpha_AV,
                       A.beta_AV,  A.eos_kind, A.cs_iso};
         Rows1 rf{A.lpos, A.sort_index, A.fcnt, A.offset, A.fids};
         Rows1 rc{A.lpos, A.sort_index, A.cnt, A.offset, A.ids};
-        dispatch_adjoint_kernel(A.kernel, [&](auto K) {
+        dispatch_adjoint_scalar(A, [&](auto K, auto T) {
             using KT = decltype(K);
-            k_force_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(
-                N, rf, (const FRecView *) A.frec, A.omega, sf[0], sf[1], sf[2], sf[3], pp,
-                A.lp_q, g);
+            using TT = decltype(T);
+            // the corrector passes from the last to the first, each on its own records
+            for (u32 i = passes; i >= 1; i--) {
+                if (records_for_pass) (*records_for_pass)(i);
+                if (sum_ext)
+                    k_accumulate3<<<div_up(N, 256), 256, 0, s>>>(N, fa[0], fa[1], fa[2], fext);
+                const f64 *fin[4] = {fa[0], fa[1], fa[2], fa[3]};
+                adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 4, fin, sf);
+                k_force_vjp<KT, TT><<<div_up(N, ABLK), ABLK, 0, s>>>(
+                    N, rf, (const FRecView *) A.frec, A.omega, sf[0], sf[1], sf[2], sf[3], pp,
+                    A.lp_q, i < passes ? 1 : 0, g);
+                if (i > 1)
+                    k_pass_vjp<<<div_up(N, 256), 256, 0, s>>>(N, dt / 2, ss, g, fa[0], fa[1],
+                                                                fa[2], fa[3]);
+            }
             // Omega bar of the merged particles, then the Omega adjoint
             const f64 *oin[1] = {g.gom};
             f64 *oout[1]      = {s_omb};
             adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 1, oin, oout);
-            k_omega_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(N, rc, A.ps, s_omb, A.mass,
-                                                             A.hfact, g.gx, g.gy, g.gz, g.gh);
+            k_omega_vjp<KT, TT><<<div_up(N, ABLK), ABLK, 0, s>>>(
+                N, rc, A.ps, s_omb, A.mass, A.hfact, g.gx, g.gy, g.gz, g.gh);
             // smoothing length through its density equation
             k_lambda<<<div_up(N, 256), 256, 0, s>>>(N, A.h, A.omega, g.gh, A.mass, A.hfact,
                                                     lam);
             const f64 *lin[1] = {lam};
             f64 *lout[1]      = {s_lam};
             adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 1, lin, lout);
-            k_ift_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(N, rc, A.ps, s_lam, A.mass, g.gx,
-                                                           g.gy, g.gz);
+            k_ift_vjp<KT, TT><<<div_up(N, ABLK), ABLK, 0, s>>>(N, rc, A.ps, s_lam, A.mass,
+                                                               g.gx, g.gy, g.gz);
         });
         if (sk && sk->count) {
-            // the force seeds of the local particles are also those of their external
-            // accelerations (a = hydro + a_ext); the gas gradients add to the positions'
+            // the force seeds of the local particles (summed over the passes) are also those of
+            // their external accelerations (a = hydro + a_ext, a_ext computed once per step);
+            // the gas gradients add to the positions'
+            const f64 *ex = sum_ext ? fext : fa[0];
+            const f64 *ey = sum_ext ? fext + N : fa[1];
+            const f64 *ez = sum_ext ? fext + 2 * (size_t) N : fa[2];
             const u32 nb = div_up(N, 256u);
             DevArray<f64> part(3 * (size_t) nb);
             std::vector<f64> hp(3 * (size_t) nb);
@@ -390,8 +493,8 @@ namespace shamx {
                           sk->mass[k],     sk->racc[k],             sk->G,
                           A.mass,          sk->soft,                sk->abar_gas[3 * k],
                           sk->abar_gas[3 * k + 1], sk->abar_gas[3 * k + 2]};
-                k_sink_gas_vjp<<<nb, 256, 0, s>>>(N, sk->x, sk->y, sk->z, a, fa[0], fa[1], fa[2],
-                                                  g.gx, g.gy, g.gz, part.data());
+                k_sink_gas_vjp<<<nb, 256, 0, s>>>(N, sk->x, sk->y, sk->z, a, ex, ey, ez, g.gx,
+                                                  g.gy, g.gz, part.data());
                 SHAMX_HIP(hipMemcpyAsync(hp.data(), part.data(), hp.size() * sizeof(f64),
                                          hipMemcpyDeviceToHost, s));
                 SHAMX_HIP(hipStreamSynchronize(s));
@@ -418,10 +521,9 @@ namespace shamx {
         f64 *lout[1]      = {s_lam};
         adjoint_gather_sorted(s, nm, N, A.sort_index, A.ghost_src, 1, lin, lout);
         Rows1 rc{A.lpos, A.sort_index, A.cnt, A.offset, A.ids};
-        dispatch_adjoint_kernel(A.kernel, [&](auto K) {
-            using KT = decltype(K);
-            k_ift_vjp<KT><<<div_up(N, ABLK), ABLK, 0, s>>>(N, rc, A.ps, s_lam, A.mass, gx, gy,
-                                                           gz);
+        dispatch_adjoint_scalar(A, [&](auto K, auto T) {
+            k_ift_vjp<decltype(K), decltype(T)><<<div_up(N, ABLK), ABLK, 0, s>>>(
+                N, rc, A.ps, s_lam, A.mass, gx, gy, gz);
         });
         SHAMX_HIP(hipGetLastError());
     }

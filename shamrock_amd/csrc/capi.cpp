@@ -257,6 +257,56 @@ struct shamx_solver {
         tree_valid   = false;
     }
 
+    // the storage order the next step will run in (the pending reorder applied to pid), as
+    // particle ids of the caller's order; the storage itself is left as it is
+    std::vector<u32> next_storage_order() {
+        if (cfg.keep_storage_order || !tree_valid || tree.n_obj != N + G || N == 0)
+            return host_pid();
+        const u32 Nm = N + G;
+        reo_perm.resize(N);
+        reo_flag.resize(Nm);
+        reo_off.resize(Nm);
+        reo_scan.resize(std::max(reo_scan.size(), scan_scratch_bytes(Nm)));
+        local_sorted_order(stream, Nm, N, tree.sort_index.data(), reo_perm.data(),
+                           reo_flag.data(), reo_off.data(), reo_scan.data());
+        std::vector<u32> perm(N), p = host_pid(), out(N);
+        SHAMX_HIP(hipMemcpyAsync(perm.data(), reo_perm.data(), N * sizeof(u32),
+                                 hipMemcpyDeviceToHost, stream));
+        SHAMX_HIP(hipStreamSynchronize(stream));
+        for (u32 j = 0; j < N; j++) out[j] = p[perm[j]];
+        return out;
+    }
+
+    // store the local particles in the given order (order[j] = caller id held by slot j); every
+    // field keeps its values per particle, and the next step does not reorder again
+    void set_storage_order(const u32 *order) {
+        std::vector<u8> seen(N, 0);
+        bool identity = true;
+        for (u32 j = 0; j < N; j++) {
+            need(order[j] < N && !seen[order[j]],
+                 "storage order must be a permutation of the particle ids");
+            seen[order[j]] = 1;
+            identity &= order[j] == j;
+        }
+        if (identity && pid_identity) return;
+        need(!cfg.keep_storage_order, "shamx_solver_set_storage_order: the solver keeps the "
+                                      "caller's storage order (keep_storage_order = 1)");
+        DevArray<f64> caller;
+        caller.resize(std::max<u32>(N, 1));
+        pid_tmp.resize(std::max<u32>(N, 1));
+        SHAMX_HIP(hipMemcpyAsync(pid_tmp.data(), order, N * sizeof(u32), hipMemcpyHostToDevice,
+                                 stream));
+        for (auto *f : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &ax, &ay, &az, &du, &alpha_av,
+                        &soundspeed, &divv, &curlx, &curly, &curlz, &dtdivv, &axe, &aye, &aze}) {
+            field_to_caller(*f, caller.data(), true);
+            perm_gather_f64(stream, N, pid_tmp.data(), caller.data(), f->data());
+        }
+        pid.swap(pid_tmp);
+        pid_identity = identity;
+        tree_valid   = false;
+        SHAMX_HIP(hipStreamSynchronize(stream));
+    }
+
     // field in the caller's particle order <-> storage order
     void field_from_caller(DevArray<f64> &f, const f64 *src, bool on_device) {
         if (pid_identity) {
@@ -502,6 +552,7 @@ struct shamx_solver {
                 throw std::runtime_error("the corrector has made over 50 loops, either their is "
                                          "a bug, either you are using a dt that is too large");
             b = timer.begin(stream);
+            if (cfg.keep_storage_order) record_pass_inputs(corr_iter);
             FieldPtrs fp{{h.data(), u.data(), vx.data(), vy.data(), vz.data(), omega.data()}, 6};
             gather_ghosts(stream, N, G, ghost_src.data(), fp);
             // EOS of the merged set fused with the sorted force records (a corrector rerun
@@ -764,12 +815,70 @@ struct shamx_solver {
 
     // the adjoint of the last step reads the step's own intermediate products
     DevArray<f64> adj_work;
+    // the adjoint's tape: the inputs (local v and u) of every corrector pass of the last step,
+    // kept when the storage keeps the caller's order (the adjoint's mode); the FP64 force
+    // records of one pass rebuilt from them, and the step's end state (v, u) kept meanwhile
+    std::vector<DevArray<f64>> tape;
+    DevArray<unsigned char> adj_frec;
+    DevArray<f64> adj_keep;
+
+    void record_pass_inputs(u32 pass) {
+        if (tape.size() <= pass) tape.resize(pass + 1);
+        DevArray<f64> &t = tape[pass];
+        t.resize(4 * (size_t) std::max<u32>(N, 1));
+        const f64 *src[4] = {vx.data(), vy.data(), vz.data(), u.data()};
+        for (int k = 0; k < 4; k++)
+            SHAMX_HIP(hipMemcpyAsync(t.data() + k * (size_t) N, src[k], N * sizeof(f64),
+                                     hipMemcpyDeviceToDevice, stream));
+    }
+
+    // the step's state as corrector pass `pass` (1-based) saw it: local v and u from the tape,
+    // the images' from their sources, P, c_s and the FP64 force records (adj_frec) from them
+    void adjoint_records(u32 pass) {
+        const DevArray<f64> &t = tape[pass - 1];
+        f64 *dst[4] = {vx.data(), vy.data(), vz.data(), u.data()};
+        for (int k = 0; k < 4; k++)
+            SHAMX_HIP(hipMemcpyAsync(dst[k], t.data() + k * (size_t) N, N * sizeof(f64),
+                                     hipMemcpyDeviceToDevice, stream));
+        FieldPtrs fp{{h.data(), u.data(), vx.data(), vy.data(), vz.data(), omega.data()}, 6};
+        gather_ghosts(stream, N, G, ghost_src.data(), fp);
+        adj_frec.resize(force_record_bytes(0) * (size_t) (N + G));
+        pack_force(stream, cfg.kernel, N + G, tree.sort_index.data(), x.data(), y.data(),
+                   z.data(), h.data(), vx.data(), vy.data(), vz.data(), u.data(), omega.data(),
+                   P.data(), cs.data(), adj_frec.data(), cfg.gpart_mass,
+                   EosArgs{cfg.eos_kind, cfg.gamma, cfg.eos_cs, cfg.eos_cs0, cfg.eos_q,
+                           cfg.eos_r0},
+                   0, false);
+    }
+
+    // the adjoint of the last step over all its corrector passes, on FP64 records rebuilt
+    // from the tape (so a mixed-precision step is differentiated as the FP64 step map at its
+    // own state); the step's end state is put back afterwards
+    void adjoint_passes(f64 *const *seeds11, const SinkAdjoint *sk) {
+        AdjointArgs A = adjoint_args(); // the configuration checks come first
+        const u32 r   = last.corrector_iterations;
+        need(r >= 1 && tape.size() >= r,
+             "adjoint: the last step kept no corrector inputs (run it with keep_storage_order = 1)");
+        adj_keep.resize(4 * (size_t) std::max<u32>(N, 1));
+        f64 *cur[4] = {vx.data(), vy.data(), vz.data(), u.data()};
+        for (int k = 0; k < 4; k++)
+            SHAMX_HIP(hipMemcpyAsync(adj_keep.data() + k * (size_t) N, cur[k], N * sizeof(f64),
+                                     hipMemcpyDeviceToDevice, stream));
+        adj_frec.resize(force_record_bytes(0) * (size_t) (N + G));
+        A.frec = adj_frec.data();
+        std::function<void(u32)> prep = [this](u32 pass) { adjoint_records(pass); };
+        adjoint_step(stream, A, seeds11, adj_work, sk, r, &prep);
+        // end state: images, P and c_s as the last pass left them, then the local v and u
+        adjoint_records(r);
+        for (int k = 0; k < 4; k++)
+            SHAMX_HIP(hipMemcpyAsync(cur[k], adj_keep.data() + k * (size_t) N, N * sizeof(f64),
+                                     hipMemcpyDeviceToDevice, stream));
+    }
     AdjointArgs adjoint_args() {
         need(N > 0 && tree.L > 0, "adjoint: run evolve_once first");
         need(cfg.eos_kind == 0 || cfg.eos_kind == 1 || (cfg.eos_kind == 2 && !cfg.periodic),
              "adjoint: available for the adiabatic and isothermal equations of state, and for "
              "LP07 with free boundaries (periodic images sit at other radii)");
-        need(!cfg.mixed_precision, "adjoint: the step must run in FP64");
         need(cfg.av_kind == 0, "adjoint: available for the constant artificial viscosity");
         need(pid_identity, "adjoint: the solver must keep the caller's storage order "
                            "(keep_storage_order = 1)");
@@ -801,7 +910,8 @@ struct shamx_solver {
                            frec.data(),
                            nb.ps.data(),
                            h.data(),
-                           omega.data()};
+                           omega.data(),
+                           cfg.adjoint_fp32 ? 1 : 0};
     }
 
     // sink-sink gravity reversed (SinkParticlesUpdate.cpp:155-200): a_ext,i = -sum_j G M_j
@@ -854,7 +964,7 @@ struct shamx_solver {
         SinkAdjoint sa{(u32) S,         pmid.data(), mass.data(), racc.data(),
                        cfg.constant_G,  cfg.sink_softening,       ab_gas.data(),
                        pbar_gas.data(), x.data(),    y.data(),    z.data()};
-        adjoint_step(stream, adjoint_args(), seeds11, adj_work, &sa);
+        adjoint_passes(seeds11, &sa);
         SHAMX_HIP(hipStreamSynchronize(stream));
         for (size_t i = 0; i < 3 * S; i++) pbar[i] += pbar_gas[i];
         sink_ext_vjp(pmid.data(), ab_ext.data(), pbar.data());
@@ -1226,6 +1336,21 @@ int shamx_solver_get_field(shamx_solver *s, int field, double *dst, int on_devic
     });
 }
 
+int shamx_solver_get_storage_order(shamx_solver *s, uint32_t *order) {
+    return guarded([&] {
+        need(s != nullptr && order != nullptr, "shamx_solver_get_storage_order: null argument");
+        std::vector<u32> p = s->next_storage_order();
+        std::copy(p.begin(), p.end(), order);
+    });
+}
+
+int shamx_solver_set_storage_order(shamx_solver *s, const uint32_t *order) {
+    return guarded([&] {
+        need(s != nullptr && order != nullptr, "shamx_solver_set_storage_order: null argument");
+        s->set_storage_order(order);
+    });
+}
+
 int shamx_solver_get_time(shamx_solver *s, double *tdc3) {
     return guarded([&] {
         tdc3[0] = s->t;
@@ -1316,11 +1441,9 @@ int shamx_solver_density_force(shamx_solver *s, shamx_step_log *log) {
 int shamx_solver_adjoint_step(shamx_solver *s, double *const *seeds11) {
     return guarded([&] {
         need(s && seeds11, "shamx_solver_adjoint_step: null argument");
-        need(s->last.corrector_iterations == 1,
-             "adjoint: the last step reran its corrector (lower the CFL)");
         for (int k = 0; k < 11; k++) need(seeds11[k] != nullptr, "adjoint: null seed array");
         SHAMX_HIP(hipDeviceSynchronize()); // seeds may still be being written on another stream
-        if (s->sinks.empty()) adjoint_step(s->stream, s->adjoint_args(), seeds11, s->adj_work);
+        if (s->sinks.empty()) s->adjoint_passes(seeds11, nullptr);
         else s->adjoint_step_with_sinks(seeds11);
         SHAMX_HIP(hipStreamSynchronize(s->stream));
     });

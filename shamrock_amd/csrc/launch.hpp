@@ -5,6 +5,8 @@
 // return a count to the host say so and synchronise the stream.
 #pragma once
 
+#include <functional>
+
 #include "common.hpp"
 #include "devbuf.hpp"
 
@@ -276,6 +278,7 @@ namespace shamx {
         const double4 *ps;   // (x, y, z, final h), sorted
         const f64 *h;     // final h, local particles
         const f64 *omega; // final Omega, merged (local particles then their images)
+        int fp32;         // pair terms of the adjoint in FP32 (Kahan sums) on FP64 r_ab
     };
     // the sinks of a step for its adjoint (free boundaries): the positions the sink-gas forces
     // used, masses, accretion radii and force constants (host), the seeds of the sinks'
@@ -291,8 +294,12 @@ namespace shamx {
     };
     // seeds (11 device arrays of n: x y z vx vy vz u ax ay az du) at the end of the step are
     // replaced by the seeds at its start; with sinks, their sink-gas terms are reversed too
+    // passes: the corrector passes of the step (1 + reruns); records_for_pass(i) makes A.frec
+    // hold the FP64 records of pass i (1-based) before its force adjoint runs (when null,
+    // A.frec holds those of a single pass)
     void adjoint_step(hipStream_t s, const AdjointArgs &A, f64 *const *seeds,
-                      DevArray<f64> &work, const SinkAdjoint *sk = nullptr);
+                      DevArray<f64> &work, const SinkAdjoint *sk = nullptr, u32 passes = 1,
+                      const std::function<void(u32)> *records_for_pass = nullptr);
     // adds to (gx, gy, gz) the position gradient of sum_a rho_bar_a rho_h(h_a(x))
     void adjoint_density(hipStream_t s, const AdjointArgs &A, const f64 *rho_bar, f64 *gx,
                          f64 *gy, f64 *gz, DevArray<f64> &work);

@@ -19,6 +19,7 @@
 #include "launch.hpp"
 #include "tree_layout.hpp"
 
+#include <cstdlib>
 #include <mutex>
 
 namespace shamx {
@@ -140,40 +141,76 @@ namespace shamx {
         // --------------------------------------------------------------------- radix sort
         // Onesweep (one global histogram pass, then one scatter kernel per 8-bit digit with
         // decoupled look-back across tiles).  Ranks inside a tile are computed with wave-wide
-        // peer masks built from ballots, in (wave, item, lane) order, so the sort is stable.
+        // peer masks built from ballots, in (wave, item, lane) order, so the sort is stable;
+        // each tile is staged in LDS in digit order before it is written out.
         constexpr int RS_BLOCK = 256;
         constexpr int RS_WAVES = RS_BLOCK / WAVE;
-        constexpr int RS_ITEMS = 16;
-        constexpr int RS_TILE  = RS_BLOCK * RS_ITEMS;
+        // keys per thread: 16 (8 measured slower at 2.5 M and 16.7 M keys: 206 vs 168 us and
+        // 804 vs 607 us per sort; kept as a tuning knob)
+        constexpr int RS_ITEMS       = 16;
+        constexpr int RS_ITEMS_SMALL = 8;
+        constexpr int RS_TILE_SMALL  = RS_BLOCK * RS_ITEMS_SMALL;
         constexpr int RS_BINS  = 256;
         constexpr u32 ST_AGG = 1u << 30, ST_PREFIX = 2u << 30, ST_VALUE = (1u << 30) - 1;
         constexpr u64 RS_SPIN_LIMIT = 200000000ull; // 2 s of the 100 MHz constant clock
+        constexpr int RS_LOOKBACK   = 8; // predecessors read per look-back window
 
+        // digit counts of every pass: each wave counts into its own LDS histograms (a quarter
+        // of the contention on the atomics), 4 keys per load where the keys are 16-byte aligned
         __global__ void __launch_bounds__(RS_BLOCK)
             k_rs_hist(u32 n, const u32 *__restrict keys, u32 passes, u32 *__restrict hist) {
-            __shared__ u32 h[4][RS_BINS];
-            for (int i = threadIdx.x; i < 4 * RS_BINS; i += RS_BLOCK) (&h[0][0])[i] = 0;
+            __shared__ u32 h[RS_WAVES][4][RS_BINS];
+            for (int i = threadIdx.x; i < RS_WAVES * 4 * RS_BINS; i += RS_BLOCK)
+                (&h[0][0][0])[i] = 0;
             __syncthreads();
-            for (u32 i = blockIdx.x * RS_BLOCK + threadIdx.x; i < n; i += gridDim.x * RS_BLOCK) {
-                u32 k = keys[i];
-                for (u32 p = 0; p < passes; p++) atomicAdd(&h[p][(k >> (8 * p)) & 0xff], 1u);
+            u32(*hw)[RS_BINS] = h[threadIdx.x / WAVE];
+            auto count        = [&](u32 k) {
+#pragma unroll
+                for (u32 p = 0; p < 4; p++)
+                    if (p < passes) atomicAdd(&hw[p][(k >> (8 * p)) & 0xffu], 1u);
+            };
+            const u32 stride = gridDim.x * RS_BLOCK;
+            const u32 n4     = ((uintptr_t) keys & 15) == 0 ? n / 4 : 0;
+            const uint4 *k4  = (const uint4 *) keys;
+            for (u32 i = blockIdx.x * RS_BLOCK + threadIdx.x; i < n4; i += stride) {
+                uint4 q = k4[i];
+                count(q.x);
+                count(q.y);
+                count(q.z);
+                count(q.w);
             }
+            for (u32 i = 4 * n4 + blockIdx.x * RS_BLOCK + threadIdx.x; i < n; i += stride)
+                count(keys[i]);
             __syncthreads();
             for (int i = threadIdx.x; i < (int) passes * RS_BINS; i += RS_BLOCK) {
-                u32 v = (&h[0][0])[i];
+                u32 v = 0;
+#pragma unroll
+                for (int w = 0; w < RS_WAVES; w++) v += (&h[w][0][0])[i];
                 if (v) atomicAdd(&hist[i], v);
             }
         }
 
-        __global__ void k_rs_hist_scan(u32 passes, u32 *__restrict hist) {
-            // one thread per pass: 256 bins, exclusive
-            u32 p = threadIdx.x;
-            if (p >= passes) return;
-            u32 run = 0;
-            for (int d = 0; d < RS_BINS; d++) {
-                u32 v                 = hist[p * RS_BINS + d];
-                hist[p * RS_BINS + d] = run;
-                run += v;
+        // one wave per pass: exclusive scan of the 256 digit counts (4 per lane)
+        __global__ void __launch_bounds__(WAVE) k_rs_hist_scan(u32 *__restrict hist) {
+            u32 *h         = hist + blockIdx.x * RS_BINS;
+            const u32 lane = threadIdx.x;
+            u32 v[4], sum = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                v[q] = h[4 * lane + q];
+                sum += v[q];
+            }
+            u32 inc = sum;
+#pragma unroll
+            for (int o = 1; o < WAVE; o <<= 1) {
+                u32 t = __shfl_up(inc, o, WAVE);
+                if (lane >= (u32) o) inc += t;
+            }
+            u32 run = inc - sum;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                h[4 * lane + q] = run;
+                run += v[q];
             }
         }
 
@@ -182,7 +219,11 @@ namespace shamx {
             return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         }
 
-        // ITEMS keys per thread: tiles of RS_BLOCK * ITEMS keys
+        // ITEMS keys per thread: tiles of RS_BLOCK * ITEMS keys.  Each tile ranks its keys by
+        // digit (wave peer masks, (wave, item, lane) order: stable), publishes its digit counts,
+        // stages its keys and values in LDS in digit order, looks back for the counts of the
+        // earlier tiles, then writes each digit's run out with consecutive lanes on consecutive
+        // addresses (whole 64-byte lines instead of one scattered 4-byte store per key).
         template<int ITEMS>
         __global__ void __launch_bounds__(RS_BLOCK)
             k_rs_onesweep(u32 n, u32 shift, const u32 *__restrict keys_in,
@@ -190,16 +231,20 @@ namespace shamx {
                           u32 *__restrict vals_out, const u32 *__restrict digit_base,
                           u32 *__restrict status, u32 *__restrict tile_counter, int first,
                           u32 *__restrict err) {
+            constexpr u32 TILE = RS_BLOCK * ITEMS;
             __shared__ u32 s_tile;
             __shared__ u32 wave_hist[RS_WAVES][RS_BINS];
-            __shared__ u32 tile_prefix[RS_BINS];
+            __shared__ u32 s_start[RS_BINS]; // tile-local start of each digit's run
+            __shared__ u32 s_dst[RS_BINS];   // output position of the run's key 0, minus s_start
+            __shared__ u32 s_wsum[RS_WAVES];
+            __shared__ u64 s_kv[TILE];       // (value << 32 | key) in digit order
             const u32 lane = __lane_id(), wid = threadIdx.x / WAVE;
             for (int i = threadIdx.x; i < RS_WAVES * RS_BINS; i += RS_BLOCK)
                 (&wave_hist[0][0])[i] = 0;
             if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
             __syncthreads();
             const u32 tile = s_tile;
-            const u64 base = (u64) tile * (RS_BLOCK * ITEMS) + (u64) wid * (WAVE * ITEMS);
+            const u64 base = (u64) tile * TILE + (u64) wid * (WAVE * ITEMS);
             const u64 lt   = lanemask_lt();
 
             u32 key[ITEMS], val[ITEMS], rank[ITEMS];
@@ -227,62 +272,102 @@ namespace shamx {
             }
             __syncthreads();
 
-            // per digit: exclusive scan across waves, tile aggregate, decoupled look-back
-            {
-                const u32 d = threadIdx.x; // RS_BLOCK == RS_BINS
-                u32 run     = 0;
+            // one thread per digit (RS_BLOCK == RS_BINS): offsets of the waves inside the
+            // digit's run, the tile's count published at once, then the runs' starts
+            const u32 d = threadIdx.x;
+            u32 run     = 0;
 #pragma unroll
-                for (int w = 0; w < RS_WAVES; w++) {
-                    u32 t           = wave_hist[w][d];
-                    wave_hist[w][d] = run;
-                    run += t;
-                }
-                u32 *st = status + (u64) tile * RS_BINS + d;
-                if (tile == 0) {
-                    __hip_atomic_store(st, ST_PREFIX | run, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    tile_prefix[d] = digit_base[d];
-                } else {
-                    __hip_atomic_store(st, ST_AGG | run, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    u32 excl = 0;
-                    i64 j    = (i64) tile - 1;
-                    u64 t0   = wall_clock64();
-                    while (j >= 0) {
-                        u32 s = __hip_atomic_load(status + (u64) j * RS_BINS + d,
-                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if ((s & ~ST_VALUE) == 0) { // predecessor not published yet
-                            // predecessors always publish; a wait this long means a bug:
-                            // flag it and leave instead of hanging the device
-                            if (wall_clock64() - t0 > RS_SPIN_LIMIT) {
-                                atomicOr(err, 1u);
-                                break;
-                            }
-                            continue;
-                        }
-                        excl += s & ST_VALUE;
-                        if ((s & ~ST_VALUE) == ST_PREFIX) break;
-                        j--;
-                    }
-                    __hip_atomic_store(st, ST_PREFIX | (excl + run), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    tile_prefix[d] = digit_base[d] + excl;
-                }
+            for (int w = 0; w < RS_WAVES; w++) {
+                u32 t           = wave_hist[w][d];
+                wave_hist[w][d] = run;
+                run += t;
             }
+            u32 *st = status + (u64) tile * RS_BINS + d;
+            __hip_atomic_store(st, (tile == 0 ? ST_PREFIX : ST_AGG) | run, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            u32 inc = run;
+#pragma unroll
+            for (int o = 1; o < WAVE; o <<= 1) {
+                u32 t = __shfl_up(inc, o, WAVE);
+                if (lane >= (u32) o) inc += t;
+            }
+            if (lane == WAVE - 1) s_wsum[wid] = inc;
+            __syncthreads();
+            u32 start = inc - run;
+            for (u32 w = 0; w < wid; w++) start += s_wsum[w];
+            s_start[d] = start;
             __syncthreads();
 
+            // stage the tile in digit order (while earlier tiles finish publishing)
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
                 u64 idx = base + (u64) k * WAVE + lane;
                 if (idx < n) {
-                    u32 d   = (key[k] >> shift) & 0xffu;
-                    u32 dst = tile_prefix[d] + wave_hist[wid][d] + rank[k];
-                    if (dst < n) {
-                        keys_out[dst] = key[k];
-                        vals_out[dst] = val[k];
-                    } else {
-                        atomicOr(err, 2u);
+                    u32 dk = (key[k] >> shift) & 0xffu;
+                    s_kv[s_start[dk] + wave_hist[wid][dk] + rank[k]] =
+                        ((u64) val[k] << 32) | key[k];
+                }
+            }
+
+            // decoupled look-back for the counts of digit d in the earlier tiles, RS_LOOKBACK
+            // predecessors read at once (independent loads: one memory latency per window)
+            u32 excl = 0;
+            if (tile > 0) {
+                i64 j  = (i64) tile - 1;
+                u64 t0 = wall_clock64();
+                while (j >= 0) {
+                    u32 sv[RS_LOOKBACK];
+#pragma unroll
+                    for (int q = 0; q < RS_LOOKBACK; q++)
+                        sv[q] = (j - q >= 0)
+                                    ? __hip_atomic_load(status + (u64) (j - q) * RS_BINS + d,
+                                                        __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                                    : ST_PREFIX; // before tile 0: a prefix of 0
+                    // take the window in order up to the first prefix or unpublished entry
+                    int used = 0;
+                    bool done = false, wait = false;
+#pragma unroll
+                    for (int q = 0; q < RS_LOOKBACK; q++) {
+                        if (done || wait) continue;
+                        const u32 flag = sv[q] & ~ST_VALUE;
+                        if (flag == 0) {
+                            wait = true;
+                        } else {
+                            excl += sv[q] & ST_VALUE;
+                            used++;
+                            done = flag == ST_PREFIX;
+                        }
                     }
+                    if (done) break;
+                    j -= used;
+                    // predecessors always publish; a wait this long means a bug: flag it and
+                    // leave instead of hanging the device
+                    if (wait && wall_clock64() - t0 > RS_SPIN_LIMIT) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                }
+                __hip_atomic_store(st, ST_PREFIX | (excl + run), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_dst[d] = digit_base[d] + excl - start;
+            __syncthreads();
+
+            // write the runs out: consecutive positions of the tile go to consecutive
+            // addresses of their digit's output run
+            const u64 t0    = (u64) tile * TILE;
+            const u32 count = (u32) min((u64) TILE, (u64) n - t0);
+#pragma unroll 4
+            for (u32 i = threadIdx.x; i < count; i += RS_BLOCK) {
+                u64 kv  = s_kv[i];
+                u32 k   = (u32) kv;
+                u32 dst = s_dst[(k >> shift) & 0xffu] + i;
+                if (dst < n) {
+                    keys_out[dst] = k;
+                    vals_out[dst] = (u32) (kv >> 32);
+                } else {
+                    atomicOr(err, 2u);
                 }
             }
         }
@@ -577,7 +662,8 @@ namespace shamx {
         // scratch of radix_sort_pairs for n keys: 4 digit histograms, 8 counters / error
         // words, then 4 passes of per-tile digit status words
         constexpr size_t rs_scratch_bytes(u32 n) {
-            size_t nt = ((size_t) (n > 0 ? n : 1) + RS_TILE - 1) / RS_TILE;
+            // tiles counted at the smaller tile size: enough status words for either
+            size_t nt = ((size_t) (n > 0 ? n : 1) + RS_TILE_SMALL - 1) / RS_TILE_SMALL;
             return (4 * RS_BINS + 8 + 4 * nt * RS_BINS) * sizeof(u32);
         }
     } // namespace
@@ -588,22 +674,33 @@ namespace shamx {
                           u32 *keys_alt, u32 *vals_alt, void *scratch, bool vals_given) {
         if (n == 0) return;
         if (key_bits == 0 || key_bits > 32) throw std::invalid_argument("radix sort key_bits");
+        if (n >= (1u << 30)) throw std::invalid_argument("radix sort: n must be below 2^30");
+        // SHAMX_SORT_ITEMS = 8 or 16 forces the keys per thread (a test and tuning knob)
+        const char *ie   = std::getenv("SHAMX_SORT_ITEMS");
+        const int items  = ie ? std::atoi(ie) : 0;
+        const bool small = items == RS_ITEMS_SMALL;
         u32 passes   = div_up(key_bits, 8);
-        u32 nt       = div_up(n, RS_TILE);
+        u32 nt       = div_up(n, RS_BLOCK * (small ? RS_ITEMS_SMALL : RS_ITEMS));
         u32 *hist    = (u32 *) scratch;
         u32 *ctr     = hist + 4 * RS_BINS;
         u32 *err     = ctr + 4;
         u32 *status  = ctr + 8;
         fill_u32(s, hist, 0, 4 * RS_BINS + 8 + (size_t) passes * nt * RS_BINS);
-        u32 hb = std::max<u32>(1, std::min<u32>(512, div_up(n, RS_BLOCK * 16)));
+        u32 hb = std::max<u32>(1, std::min<u32>(1024, div_up(n, RS_BLOCK * 16)));
         k_rs_hist<<<hb, RS_BLOCK, 0, s>>>(n, keys, passes, hist);
-        k_rs_hist_scan<<<1, 64, 0, s>>>(passes, hist);
+        k_rs_hist_scan<<<passes, WAVE, 0, s>>>(hist);
         u32 *kin = keys, *vin = vals, *kout = keys_alt, *vout = vals_alt;
         for (u32 p = 0; p < passes; p++) {
-            // (32 keys per thread measured slower: 0.90 vs 0.80 ms at 16.7 M keys)
-            k_rs_onesweep<RS_ITEMS><<<nt, RS_BLOCK, 0, s>>>(
-                n, 8 * p, kin, vin, kout, vout, hist + p * RS_BINS,
-                status + (u64) p * nt * RS_BINS, ctr + p, (p == 0 && !vals_given) ? 1 : 0, err);
+            if (small)
+                k_rs_onesweep<RS_ITEMS_SMALL><<<nt, RS_BLOCK, 0, s>>>(
+                    n, 8 * p, kin, vin, kout, vout, hist + p * RS_BINS,
+                    status + (u64) p * nt * RS_BINS, ctr + p, (p == 0 && !vals_given) ? 1 : 0,
+                    err);
+            else
+                k_rs_onesweep<RS_ITEMS><<<nt, RS_BLOCK, 0, s>>>(
+                    n, 8 * p, kin, vin, kout, vout, hist + p * RS_BINS,
+                    status + (u64) p * nt * RS_BINS, ctr + p, (p == 0 && !vals_given) ? 1 : 0,
+                    err);
             std::swap(kin, kout);
             std::swap(vin, vout);
         }

@@ -251,6 +251,9 @@ class SPHModel:
         self.cfg = SolverConfig(kernel)
         self.host = {k: np.zeros(0) for k in FIELDS}
         self.sinks = np.zeros((0, 17))
+        # storage order of the particles (caller ids per slot) to restore when the solver is
+        # built again, so that a rebuilt or reloaded solver sums in the same order
+        self.storage_order = None
         self.gpu: GpuSolver | None = None
         self.t, self.dt, self.cfl_mult = 0.0, 0.0, 1e-2
         self.logs: list[dict] = []
@@ -336,6 +339,7 @@ class SPHModel:
             for k in self.host:
                 self.host[k] = self.gpu.get(k)
             self.sinks = self.gpu.sink_records()
+            self.storage_order = self.gpu.storage_order()
             tm = self.gpu.time()
             self.t, self.dt, self.cfl_mult = tm["t"], tm["dt"], tm["cfl_mult"]
 
@@ -353,6 +357,7 @@ class SPHModel:
                    u=np.asarray(upart, float))
         for k in self.host:
             self.host[k] = np.concatenate([self.host[k], add.get(k, np.zeros(n))])
+        self.storage_order = None
 
     def get_total_part_count(self):
         return int(self.host["x"].size) if self.gpu is None else self.gpu.n
@@ -418,11 +423,11 @@ class SPHModel:
         self._sync_from_gpu()
         write_dump(fname, self.kernel, self.cfg.d, dict(t=self.t, dt=self.dt,
                                                          cfl_mult=self.cfl_mult),
-                   self.sinks, self.host, self.step_count)
+                   self.sinks, self.host, self.step_count, self.storage_order)
 
     def load_from_dump(self, fname):
         """Model::load_from_dump (Model.hpp:906-957) for files written by dump."""
-        kernel, cfg, tm, sinks, fields, steps = read_dump(fname)
+        kernel, cfg, tm, sinks, fields, steps, order = read_dump(fname)
         if kernel != self.kernel:
             raise ValueError(f"dump holds a {kernel} model, this model is {self.kernel}")
         self._invalidate()
@@ -431,6 +436,7 @@ class SPHModel:
         self.t, self.dt, self.cfl_mult = tm["t"], tm["dt"], tm["cfl_mult"]
         self.sinks = sinks
         self.host = {k: fields.get(k, np.zeros(fields["x"].size)) for k in FIELDS}
+        self.storage_order = order
         self.step_count = steps
 
     # --- sinks ---------------------------------------------------------------------------
@@ -523,6 +529,8 @@ class SPHModel:
                                    "Please set it using either model.set_particle_mass(pmass) "
                                    "or cfg.set_particle_mass(pmass)")
             self.gpu = GpuSolver(self.cfg.d, self.host)
+            if self.storage_order is not None and self.storage_order.size == self.gpu.n:
+                self.gpu.set_storage_order(self.storage_order)
             self.gpu.set_sink_records(self.sinks)
             self.gpu.set_time(self.t, self.dt, self.cfl_mult)
         return self.gpu
@@ -590,19 +598,23 @@ class SPHModel:
 
 
 _DUMP_MAGIC = b"SHAMXDMP"
-_DUMP_VERSION = 1
+_DUMP_VERSION = 2
 
 
-def write_dump(fname, kernel, cfg, time_state, sinks, fields, step_count=0):
+def write_dump(fname, kernel, cfg, time_state, sinks, fields, step_count=0,
+               storage_order=None):
     """Restart file: 8-byte magic, u64 version, u64 header length, a JSON header (kernel,
-    config, time edges, step count, sink records, field names and particle count), then each
-    field as n little-endian float64 values in header order."""
+    config, time edges, step count, sink records, field names, particle count and whether a
+    storage order follows), then each field as n little-endian float64 values in header order,
+    then (if present) the storage order as n little-endian u32 particle ids."""
     n = int(np.asarray(fields["x"]).size)
     names = [k for k in FIELDS if k in fields]
     cfg = {k: (list(v) if isinstance(v, (list, tuple, np.ndarray)) else v)
            for k, v in cfg.items() if k != "sinks"}
+    has_order = storage_order is not None and np.asarray(storage_order).size == n
     head = dict(kernel=kernel, config=cfg, time=dict(time_state), step_count=int(step_count),
-                sinks=np.asarray(sinks, float).reshape(-1, 17).tolist(), n=n, fields=names)
+                sinks=np.asarray(sinks, float).reshape(-1, 17).tolist(), n=n, fields=names,
+                storage_order=bool(has_order))
     hb = json.dumps(head, sort_keys=True).encode()
     with open(fname, "wb") as f:
         f.write(_DUMP_MAGIC)
@@ -613,15 +625,18 @@ def write_dump(fname, kernel, cfg, time_state, sinks, fields, step_count=0):
             if a.size != n:
                 raise ValueError(f"field {k} has {a.size} values, expected {n}")
             f.write(a.tobytes())
+        if has_order:
+            f.write(np.ascontiguousarray(storage_order, "<u4").tobytes())
 
 
 def read_dump(fname):
-    """Inverse of write_dump: (kernel, config, time edges, sink records, fields, step count)."""
+    """Inverse of write_dump: (kernel, config, time edges, sink records, fields, step count,
+    storage order or None).  Version-1 files (no storage order) are read too."""
     with open(fname, "rb") as f:
         if f.read(8) != _DUMP_MAGIC:
             raise ValueError(f"{fname}: not a dump written by this library")
         version, hlen = np.frombuffer(f.read(16), "<u8")
-        if int(version) != _DUMP_VERSION:
+        if int(version) not in (1, _DUMP_VERSION):
             raise ValueError(f"{fname}: dump version {int(version)} is not supported")
         head = json.loads(f.read(int(hlen)).decode())
         n = int(head["n"])
@@ -631,9 +646,17 @@ def read_dump(fname):
             if a.size != n:
                 raise ValueError(f"{fname}: truncated field {k}")
             fields[k] = a.astype(np.float64)
+        order = None
+        if head.get("storage_order", False):
+            order = np.frombuffer(f.read(4 * n), "<u4")
+            if order.size != n:
+                raise ValueError(f"{fname}: truncated storage order")
+            order = order.astype(np.uint32)
+            if not np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32)):
+                raise ValueError(f"{fname}: the storage order is not a permutation")
     sinks = np.asarray(head["sinks"], float).reshape(-1, 17)
     return (head["kernel"], head["config"], head["time"], sinks, fields,
-            int(head.get("step_count", 0)))
+            int(head.get("step_count", 0)), order)
 
 
 _KERNELS = ("M4", "M6", "M8", "C2", "C4", "C6")

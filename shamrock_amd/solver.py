@@ -36,6 +36,7 @@ def make_config(cfg: dict) -> _lib.ShamxConfig:
         setattr(c, k, int(cfg[k]))
     c.av_kind = int(cfg.get("av_kind", 0))
     c.keep_storage_order = int(cfg.get("keep_storage_order", 0))
+    c.adjoint_fp32 = int(cfg.get("adjoint_fp32", 0))
     for k, d in (("alpha_min", 0.0), ("alpha_max", 1.0), ("sigma_decay", 0.1),
                  ("constant_G", G_SI), ("sink_softening", 0.0)):
         setattr(c, k, float(cfg.get(k, d)))
@@ -75,6 +76,19 @@ class GpuSolver:
         a = np.zeros(self.n)
         check(lib().shamx_solver_get_field(self.h, FIELDS.index(name), a.ctypes.data, 0))
         return a
+
+    def storage_order(self) -> np.ndarray:
+        """Caller particle id of every storage slot, in the order the next step runs in."""
+        out = np.zeros(self.n, np.uint32)
+        check(lib().shamx_solver_get_storage_order(self.h, out.ctypes.data))
+        return out
+
+    def set_storage_order(self, order):
+        """Store the particles in `order` (a permutation of the caller's ids)."""
+        a = np.ascontiguousarray(order, np.uint32)
+        if a.size != self.n:
+            raise ValueError(f"storage order has {a.size} ids, the solver {self.n} particles")
+        check(lib().shamx_solver_set_storage_order(self.h, a.ctypes.data))
 
     def get_device(self, name, out=None):
         """Device copy of a field as a torch tensor (float64, cuda)."""

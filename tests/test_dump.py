@@ -71,3 +71,24 @@ def test_dump_refuses_foreign_and_mismatched_files(tmp_path):
     cut.write_bytes(raw[:-8])
     with pytest.raises(ValueError, match="truncated"):
         model_mod.read_dump(str(cut))
+
+
+def test_dump_keeps_the_storage_order(tmp_path):
+    """A solver's storage order (caller ids per slot) is written after the fields and read
+    back; a file whose order is not a permutation is refused."""
+    n = 7
+    fields = {k: np.arange(n, dtype=float) + i for i, k in enumerate(model_mod.FIELDS)}
+    order = np.array([3, 0, 6, 1, 5, 2, 4], np.uint32)
+    f = str(tmp_path / "ordered.shamx")
+    model_mod.write_dump(f, "M6", {"gpart_mass": 1.0}, dict(t=0.0, dt=0.0, cfl_mult=0.3),
+                         np.zeros((0, 17)), fields, 4, order)
+    out = model_mod.read_dump(f)
+    np.testing.assert_array_equal(out[6], order)
+    assert out[5] == 4
+    for k in model_mod.FIELDS:
+        np.testing.assert_array_equal(out[4][k], fields[k])
+    bad = str(tmp_path / "bad_order.shamx")
+    model_mod.write_dump(bad, "M6", {"gpart_mass": 1.0}, dict(t=0.0, dt=0.0, cfl_mult=0.3),
+                         np.zeros((0, 17)), fields, 4, np.array([0, 0, 1, 2, 3, 4, 5]))
+    with pytest.raises(ValueError, match="permutation"):
+        model_mod.read_dump(bad)

@@ -22,7 +22,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 from shamrock_amd import setups  # noqa: E402
-from shamrock_amd.adjoint import rollout_density, rollout_dts  # noqa: E402
+from shamrock_amd.adjoint import rollout_density, rollout_dts, rollout_passes  # noqa: E402
 
 DEV = "cuda"
 
@@ -110,7 +110,9 @@ def test_adjoint_kh64_sixty_four_components():
     initial velocities vs central differences on 64 random components.  Same perturbed start as
     the small cases (see the module docstring).  Measured: max error 2.3e-4 of the largest
     difference quotient, median 4e-7; the tolerance leaves room for one pair crossing an
-    artificial-viscosity kink inside a difference step."""
+    artificial-viscosity kink inside a difference step.  The FP32 adjoint (the workload's
+    precision) of the same rollout is held to rel-L2 1e-4 of the FP64 gradient and to the same
+    central differences."""
     say("[adjoint kh64]")
     cfg, f = setups.kelvin_helmholtz(64)
     cfg = dict(cfg, epsilon_h=1e-12, cfl_cour=0.1, cfl_force=0.1)
@@ -134,6 +136,15 @@ def test_adjoint_kh64_sixty_four_components():
     vv = v0.clone().requires_grad_(True)
     loss(vv).backward()
     g = vv.grad.cpu().numpy()
+    # the FP32 adjoint (pair terms in FP32, Kahan sums, FP64 r_ab) of the same FP64 rollout
+    cfg32 = dict(cfg, adjoint_fp32=1)
+    v32 = v0.clone().requires_grad_(True)
+    rho32 = rollout_density(cfg32, f, v32, u0, steps, checkpoint_every=k, dts=dts)
+    ((rho32 - target) ** 2).mean().backward()
+    g32 = v32.grad.cpu().numpy()
+    rel32 = float(np.linalg.norm(g32 - g) / np.linalg.norm(g))
+    say(f"[adjoint kh64] FP32 adjoint vs FP64 adjoint: rel-L2 {rel32:.2e}")
+    assert rel32 < 1e-4
     comps = [(int(i), int(d)) for i, d in zip(rng.integers(0, n, 64), rng.integers(0, 3, 64))]
     fd = []
     with torch.no_grad():
@@ -147,10 +158,13 @@ def test_adjoint_kh64_sixty_four_components():
     ad = np.array([g[i, d] for i, d in comps])
     scale = np.abs(fd).max()
     err = np.abs(ad - fd) / scale
+    err32 = np.abs(np.array([g32[i, d] for i, d in comps]) - fd) / scale
     say(f"[adjoint kh64] max |fd| {scale:.3e} max rel err {err.max():.2e} "
-        f"median {np.median(err):.2e}")
+        f"median {np.median(err):.2e}; FP32 adjoint max {err32.max():.2e} "
+        f"median {np.median(err32):.2e}")
     assert scale > 0
     assert err.max() < 1e-3 and np.median(err) < 1e-5
+    assert err32.max() < 1e-3 and np.median(err32) < 1e-4
 
 
 def test_adjoint_lp07_matches_central_differences():
@@ -298,3 +312,66 @@ def test_fit_initial_velocity_to_target_density():
             v -= lr * g
     say(f"[adjoint fit] losses {losses}")
     assert losses[-1] < losses[0]
+
+
+def test_adjoint_through_corrector_reruns():
+    """Steps whose corrector reruns (a Sedov blast at 12^3, at the time steps the solver picks or
+    multiples of them, until a pass's velocity change is above the 1e-2 tolerance) are
+    differentiated through every pass: pass i evaluates the force from the velocities and u of
+    pass i - 1.  The
+    gradient of sum c_a rho_a w.r.t. v0 and u0 is compared with central differences, for the
+    FP64 and the FP32 adjoint."""
+    say("[adjoint reruns]")
+    cfg, f = setups.sedov(12)
+    cfg = dict(cfg, epsilon_h=1e-12)
+    rng = np.random.default_rng(5)
+    v = np.stack([f["vx"], f["vy"], f["vz"]], 1)
+    v0 = torch.tensor(v + 1e-3 * rng.standard_normal(v.shape), device=DEV)
+    u0 = torch.tensor(f["u"] * (1 + 0.05 * rng.standard_normal(f["u"].size)), device=DEV)
+    steps = 4
+    base = rollout_dts(cfg, f, v0, u0, steps)
+    for factor in (1, 2, 4):
+        dts = [factor * d for d in base]
+        passes = rollout_passes(cfg, f, v0, u0, steps, dts)
+        say(f"[adjoint reruns] dt x {factor}: corrector passes per step {passes}")
+        if max(passes) >= 2:
+            break
+    assert max(passes) >= 2
+    rng = np.random.default_rng(3)
+    c = torch.tensor(rng.normal(size=v0.shape[0]), device=DEV)
+    grads = {}
+    for fp32 in (0, 1):
+        cf = dict(cfg, adjoint_fp32=fp32)
+        v = v0.clone().requires_grad_(True)
+        u = u0.clone().requires_grad_(True)
+        _loss(cf, f, v, u, c, steps, dts).backward()
+        grads[fp32] = (v.grad.cpu().numpy(), u.grad.cpu().numpy())
+    n = v0.shape[0]
+    comps = [(int(i), int(d)) for i, d in zip(rng.integers(0, n, 32), rng.integers(0, 3, 32))]
+    comps_u = [int(i) for i in rng.integers(0, n, 8)]
+    fd_v, fd_u = [], []
+    with torch.no_grad():
+        for i, d in comps:
+            e = 1e-6
+            vp, vm = v0.clone(), v0.clone()
+            vp[i, d] += e
+            vm[i, d] -= e
+            fd_v.append((_loss(cfg, f, vp, u0, c, steps, dts) -
+                         _loss(cfg, f, vm, u0, c, steps, dts)).item() / (2 * e))
+        for i in comps_u:
+            e = 1e-5
+            up, um = u0.clone(), u0.clone()
+            up[i] += e
+            um[i] -= e
+            fd_u.append((_loss(cfg, f, v0, up, c, steps, dts) -
+                         _loss(cfg, f, v0, um, c, steps, dts)).item() / (2 * e))
+    fd_v, fd_u = np.array(fd_v), np.array(fd_u)
+    scale_v, scale_u = np.abs(fd_v).max(), np.abs(fd_u).max()
+    assert scale_v > 0 and scale_u > 0
+    for fp32, (gv, gu) in grads.items():
+        err_v = np.abs(np.array([gv[i, d] for i, d in comps]) - fd_v).max() / scale_v
+        err_u = np.abs(np.array([gu[i] for i in comps_u]) - fd_u).max() / scale_u
+        say(f"[adjoint reruns] fp32={fp32}: rel err v {err_v:.2e}, u {err_u:.2e}")
+        assert err_v < 1e-4 and err_u < 1e-4
+    g64, g32 = grads[0][0], grads[1][0]
+    assert np.linalg.norm(g32 - g64) / np.linalg.norm(g64) < 1e-4

@@ -143,7 +143,10 @@ def test_disc_4M_momentum_four_steps():
         g.evolve_once()
     p1, s1 = _momentum(g, cfg)
     g.close()
-    assert np.all(np.abs(p1 - p0) <= 1e-12 * np.maximum(s0, s1)), (p0, p1, s0)
+    # one scale for the three components (the disc's vertical momentum starts at zero, and
+    # rounding of the in-plane terms reaches it through the pair directions)
+    scale = max(s0.max(), s1.max())
+    assert np.all(np.abs(p1 - p0) <= 1e-12 * scale), (p0, p1, s0, s1)
 
 
 def test_disc_mixed_precision_error_bound():

@@ -113,6 +113,27 @@ def test_radix_sort_stable(ctx, n, bits, seed):
     np.testing.assert_array_equal(k.cpu().numpy().view(np.uint32), keys[ref])
 
 
+@pytest.mark.parametrize("items", ["8", "16"])
+@pytest.mark.parametrize("n,bits,seed", [(1, 30, 0), (4097, 30, 2), (300_001, 32, 6),
+                                         (1_000_003, 30, 3)])
+def test_radix_sort_stable_tile_sizes(ctx, n, bits, seed, items, monkeypatch):
+    """Both tile sizes of the sort (16 keys per thread, the default, and 8) give the stable
+    order."""
+    monkeypatch.setenv("SHAMX_SORT_ITEMS", items)
+    say(f"[radix_sort] n={n} bits={bits} items={items}")
+    rng = np.random.default_rng(seed)
+    keys = rng.integers(0, 1 << bits, n, dtype=np.uint64).astype(np.uint32)
+    if n > 100:
+        keys[: n // 3] = keys[n // 3: 2 * (n // 3)]
+    k = dev(keys.view(np.int32))
+    perm = torch.zeros(n, dtype=torch.int32, device=DEV)
+    check(lib().shamx_sort(ctx, stream(), n, bits, k.data_ptr(), perm.data_ptr()))
+    torch.cuda.synchronize()
+    ref = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(perm.cpu().numpy().view(np.uint32), ref)
+    np.testing.assert_array_equal(k.cpu().numpy().view(np.uint32), keys[ref])
+
+
 def _tree_gpu(ctx, pos, h, level, htol, split=False):
     """Tree through shamx_tree_build, or (split) through its two halves
     shamx_tree_reduce_karras + shamx_tree_aabb_hmax."""
