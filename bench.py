@@ -137,27 +137,47 @@ def scale_probe(n=16_777_216, reps=5):
 
     t_copy = timed(lambda: keys.copy_(codes))
     t_sort = timed(sort) - t_copy
-    t_rk = timed(lambda: check(lib().shamx_tree_reduce_karras(
-        ctx, sp, n, x.data_ptr(), y.data_ptr(), z.data_ptr(), 3)))
-    t_ab = timed(lambda: check(lib().shamx_tree_aabb_hmax(
-        ctx, sp, x.data_ptr(), y.data_ptr(), z.data_ptr(), h.data_ptr(), 1.1)))
+
+    def tree_time(x, y, z, h, t_sort_in):
+        # the tree's own sort (timed alone on the same keys) is not part of the tree's time
+        t_rk = timed(lambda: check(lib().shamx_tree_reduce_karras(
+            ctx, sp, n, x.data_ptr(), y.data_ptr(), z.data_ptr(), 3)))
+        t_ab = timed(lambda: check(lib().shamx_tree_aabb_hmax(
+            ctx, sp, x.data_ptr(), y.data_ptr(), z.data_ptr(), h.data_ptr(), 1.1)))
+        return t_rk - t_sort_in + t_ab
+
+    # the tree on particles stored in the order of the previous build's Morton sort (what the
+    # solver keeps between steps: it reorders its storage at the start of every step), and on
+    # the caller's order (here random: every per-particle read goes through the permutation)
+    t_tree_random = tree_time(x, y, z, h, t_sort)
+    sort()
+    order = perm.long()
+    xs, ys, zs, hs = x[order], y[order], z[order], h[order]
+    codes.copy_(keys)  # the codes of the Morton-ordered particles (sorted keys)
+    t_sort_sorted = timed(sort) - t_copy
+    t_tree = tree_time(xs, ys, zs, hs, t_sort_sorted)
     sz = (C.c_uint32 * 4)()
     check(lib().shamx_tree_sizes(ctx, sz))
     L, I = int(sz[1]), int(sz[2])
     lib().shamx_ctx_destroy(ctx)
-    t_tree = t_rk - t_sort + t_ab
     sort_b = SORT_BYTES_PER_KEY * n
     tree_b = TREE_BYTES[0] * n + TREE_BYTES[1] * L + TREE_BYTES[2] * I
     out = {}
     for k, b, t, alg in (("sort", sort_b, t_sort, f"{SORT_BYTES_PER_KEY} B x {n} keys"),
                          ("tree", tree_b, t_tree,
-                          f"132 B x {n} + 68 B x {L} leaves + 182 B x {I} cells")):
+                          f"132 B x {n} + 68 B x {L} leaves + 182 B x {I} cells, particles "
+                          "stored in the previous build's Morton order (the solver's storage)"),
+                         ("tree_random_order", tree_b, t_tree_random,
+                          f"132 B x {n} + 68 B x {L} leaves + 182 B x {I} cells, particles "
+                          "stored in random order")):
         gbs = b / t / 1e9
         out[f"{k}_plummer_16M"] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                                    "avg_ms": t * 1e3, "algorithmic": alg}
     log(f"[bench] scale probe n={n}: sort {t_sort * 1e3:.3f} ms, tree {t_tree * 1e3:.3f} ms "
+        f"(Morton-ordered storage), {t_tree_random * 1e3:.3f} ms (random order) "
         f"(L={L}, I={I})")
+    del xs, ys, zs, hs, order
     del x, y, z, h, keys, perm, codes
     torch.cuda.empty_cache()
     out["pass_plummer_16M"] = plummer_pass(n, pos, hh)

@@ -267,6 +267,16 @@ int shamx_solver_stream(shamx_solver *s, void **stream);
  * shamx_solver_adjoint_density starts (it sets them to zero): the gradients of the particle
  * seeds then include the paths through the sinks. */
 int shamx_solver_adjoint_step(shamx_solver *s, double *const *seeds11);
+/* No counterpart in the reference (the adjoint's product store): products_save keeps what the
+ * adjoint of the last step reads (tree order, images, neighbour rows, sorted positions, the
+ * positions, h and Omega at the end of the step, the velocities and u every corrector pass
+ * started from, the step log and the sinks) in slot `slot`; products_load puts them back, so
+ * that shamx_solver_adjoint_step can run for that step without running it again.  After a load
+ * the solver's fields hold the saved step's end positions: set the fields (a restore) before
+ * the next evolve_once.  products_free releases every slot.  keep_storage_order = 1 only. */
+int shamx_solver_products_save(shamx_solver *s, uint32_t slot);
+int shamx_solver_products_load(shamx_solver *s, uint32_t slot);
+int shamx_solver_products_free(shamx_solver *s);
 /* adds to xbar, ybar, zbar (device, N) the position gradient of sum_a rho_bar_a rho_a at the
  * current state, rho_a = m (hfact / h_a)^3 with h_a from the density equation; starts a
  * backward pass (the sinks' seeds are set to zero) */

@@ -98,6 +98,9 @@ SIGNATURES = {
     "shamx_solver_set_field": (_i, [_P, _i, _P, _i]),
     "shamx_solver_get_field": (_i, [_P, _i, _P, _i]),
     "shamx_solver_get_storage_order": (_i, [_P, _P]),
+    "shamx_solver_products_save": (_i, [_P, C.c_uint32]),
+    "shamx_solver_products_load": (_i, [_P, C.c_uint32]),
+    "shamx_solver_products_free": (_i, [_P]),
     "shamx_solver_set_storage_order": (_i, [_P, _P]),
     "shamx_solver_get_time": (_i, [_P, _dp]),
     "shamx_solver_set_time": (_i, [_P, _dp]),

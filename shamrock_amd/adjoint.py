@@ -55,11 +55,16 @@ def _step(solver, dt, passes=None):
 
 
 class _Rollout:
-    def __init__(self, cfg, fields, steps, checkpoint_every, dts):
+    def __init__(self, cfg, fields, steps, checkpoint_every, dts, store_products=True):
         # the adjoint reads the step's products in the caller's storage order
         self.cfg, self.fields, self.steps = dict(cfg, keep_storage_order=1), fields, int(steps)
         self.k = max(1, int(checkpoint_every))
         self.dts = None if dts is None else [float(d) for d in dts]
+        # keep every step's products of a segment (the solver's product store) so that the
+        # backward pass runs each step once; without it, each step but the last of a segment
+        # runs twice (once to lay down the segment's states, once before its adjoint)
+        self.store = bool(store_products)
+        self.last_c0 = ((self.steps - 1) // self.k) * self.k if self.steps else 0
 
     def forward(self, v0, u0):
         torch = _torch()
@@ -75,6 +80,8 @@ class _Rollout:
             dt, p = _step(s, None if self.dts is None else self.dts[n])
             used.append(dt)
             passes.append(p)
+            if self.store and n >= self.last_c0:
+                s.products_save(n - self.last_c0)
         self.dts_used, self.passes_used = used, passes
         self.solver = s
         rho = _density(self.cfg, s.get_device("h"))
@@ -92,6 +99,21 @@ class _Rollout:
         s.adjoint_density(rho_bar.contiguous(), seeds[0], seeds[1], seeds[2])
         if u_bar is not None:
             seeds[6] += u_bar
+        if self.store:
+            for c0 in sorted(self.ckpt, reverse=True):
+                seg = list(range(c0, min(c0 + self.k, self.steps)))
+                if c0 != self.last_c0:  # the forward pass kept the last segment's products
+                    s.restore(self.ckpt[c0])
+                    for n in seg:
+                        _step(s, self.dts_used[n], self.passes_used[n])
+                        s.products_save(n - c0)
+                for n in reversed(seg):
+                    s.products_load(n - c0)
+                    s.adjoint_step(seeds)
+            s.products_free()
+            torch.cuda.synchronize()
+            vbar = torch.stack(seeds[3:6], dim=1)
+            return vbar, seeds[6], torch.stack(seeds[0:3], dim=1)
         first = True
         for c0 in sorted(self.ckpt, reverse=True):
             seg = list(range(c0, min(c0 + self.k, self.steps)))
@@ -124,8 +146,8 @@ def _make_function():
 
     class RolloutState(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, v0, u0, cfg, fields, steps, checkpoint_every, dts):
-            r = _Rollout(cfg, fields, steps, checkpoint_every, dts)
+        def forward(ctx, v0, u0, cfg, fields, steps, checkpoint_every, dts, store_products):
+            r = _Rollout(cfg, fields, steps, checkpoint_every, dts, store_products)
             rho, u = r.forward(v0, u0)
             ctx.rollout = r
             return rho, u
@@ -134,7 +156,7 @@ def _make_function():
         def backward(ctx, rho_bar, u_bar):
             vbar, ubar, _ = ctx.rollout.backward(rho_bar, u_bar)
             ctx.rollout.solver.close()
-            return vbar, ubar, None, None, None, None, None
+            return vbar, ubar, None, None, None, None, None, None
 
     return RolloutState
 
@@ -142,23 +164,25 @@ def _make_function():
 _FN = None
 
 
-def rollout_state(cfg, fields, v0, u0, steps, checkpoint_every=5, dts=None):
+def rollout_state(cfg, fields, v0, u0, steps, checkpoint_every=5, dts=None,
+                  store_products=True):
     """Final-state density and internal energy after `steps` steps, differentiable w.r.t. v0
     (N x 3) and u0 (N)."""
     global _FN
     if _FN is None:
         _FN = _make_function()
-    return _FN.apply(v0, u0, cfg, fields, steps, checkpoint_every, dts)
+    return _FN.apply(v0, u0, cfg, fields, steps, checkpoint_every, dts, store_products)
 
 
-def rollout_density(cfg, fields, v0, u0, steps, checkpoint_every=5, dts=None):
+def rollout_density(cfg, fields, v0, u0, steps, checkpoint_every=5, dts=None,
+                    store_products=True):
     """Final-state density after `steps` steps, differentiable w.r.t. v0 (N x 3) and u0 (N)."""
-    return rollout_state(cfg, fields, v0, u0, steps, checkpoint_every, dts)[0]
+    return rollout_state(cfg, fields, v0, u0, steps, checkpoint_every, dts, store_products)[0]
 
 
 def rollout_dts(cfg, fields, v0, u0, steps):
     """The time steps the solver takes from this state (to hold them fixed in comparisons)."""
-    r = _Rollout(cfg, fields, steps, steps, None)
+    r = _Rollout(cfg, fields, steps, steps, None, store_products=False)
     r.forward(v0, u0)
     r.solver.close()
     return r.dts_used
@@ -166,7 +190,7 @@ def rollout_dts(cfg, fields, v0, u0, steps):
 
 def rollout_passes(cfg, fields, v0, u0, steps, dts=None):
     """The corrector passes (1 + reruns) of every step of the rollout."""
-    r = _Rollout(cfg, fields, steps, steps, dts)
+    r = _Rollout(cfg, fields, steps, steps, dts, store_products=False)
     r.forward(v0, u0)
     r.solver.close()
     return r.passes_used

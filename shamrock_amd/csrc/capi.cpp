@@ -822,6 +822,101 @@ struct shamx_solver {
     DevArray<unsigned char> adj_frec;
     DevArray<f64> adj_keep;
 
+    // The adjoint's product store: what the adjoint of a step reads (its tree order, images,
+    // neighbour rows, sorted positions, end-of-step positions, h and Omega, the tape of its
+    // corrector passes, its log and sinks), saved after the step and loaded back before its
+    // adjoint, so a backward pass runs every step of a segment once instead of twice.
+    struct ProductSlot {
+        bool valid = false;
+        u32 G = 0, L = 0, n_obj = 0, passes = 0;
+        shamx_step_log last{};
+        std::vector<Sink> sinks;
+        std::vector<f64> sink_pos_start;
+        DevArray<u32> sort_index, ghost_src, lpos, cnt, offset, ids, fcnt, fids;
+        DevArray<double4> ps;
+        DevArray<f64> x, y, z, h, omega;
+        std::vector<DevArray<f64>> tape;
+    };
+    std::vector<ProductSlot> slots;
+
+    template<class T>
+    void copy_dev(DevArray<T> &dst, const DevArray<T> &src, size_t n) {
+        dst.resize(std::max<size_t>(n, 1));
+        if (n)
+            SHAMX_HIP(hipMemcpyAsync(dst.data(), src.data(), n * sizeof(T),
+                                     hipMemcpyDeviceToDevice, stream));
+    }
+
+    void products_save(u32 slot) {
+        need(cfg.keep_storage_order, "products: the solver must keep the caller's storage order "
+                                     "(keep_storage_order = 1)");
+        need(N > 0 && tree.L > 0 && last.corrector_iterations >= 1 &&
+                 tape.size() >= last.corrector_iterations,
+             "products: run evolve_once first");
+        if (slots.size() <= slot) slots.resize(slot + 1);
+        ProductSlot &p = slots[slot];
+        const u32 Nm   = N + G;
+        p.G = G;
+        p.L = tree.L;
+        p.n_obj  = tree.n_obj;
+        p.passes = last.corrector_iterations;
+        p.last   = last;
+        p.sinks  = sinks;
+        p.sink_pos_start = sink_pos_start;
+        copy_dev(p.sort_index, tree.sort_index, tree.n_obj);
+        copy_dev(p.ghost_src, ghost_src, G);
+        using U32Pair = std::pair<DevArray<u32> *, DevArray<u32> *>;
+        for (auto [d, sarr] : {U32Pair{&p.lpos, &nb.lpos}, U32Pair{&p.cnt, &nb.cnt},
+                               U32Pair{&p.offset, &nb.offset}, U32Pair{&p.ids, &nb.ids},
+                               U32Pair{&p.fcnt, &nb.fcnt}, U32Pair{&p.fids, &nb.fids}})
+            copy_dev(*d, *sarr, sarr->size());
+        copy_dev(p.ps, nb.ps, nb.ps.size());
+        using F64Pair = std::pair<DevArray<f64> *, DevArray<f64> *>;
+        for (auto [d, sarr] : {F64Pair{&p.x, &x}, F64Pair{&p.y, &y}, F64Pair{&p.z, &z},
+                               F64Pair{&p.h, &h}, F64Pair{&p.omega, &omega}})
+            copy_dev(*d, *sarr, Nm);
+        if (p.tape.size() < p.passes) p.tape.resize(p.passes);
+        for (u32 i = 0; i < p.passes; i++) copy_dev(p.tape[i], tape[i], 4 * (size_t) N);
+        p.valid = true;
+        SHAMX_HIP(hipStreamSynchronize(stream));
+    }
+
+    // the solver then holds the saved step's products (its fields x, y, z, h, Omega included:
+    // the forward state is left for the adjoint, and a restore of the fields must follow
+    // before the next step)
+    void products_load(u32 slot) {
+        need(slot < slots.size() && slots[slot].valid, "products: nothing saved in this slot");
+        ProductSlot &p = slots[slot];
+        const u32 Nm   = N + p.G;
+        need(Nm <= cap, "products: the saved step has more images than the solver holds");
+        G           = p.G;
+        tree.L      = p.L;
+        tree.n_obj  = p.n_obj;
+        last        = p.last;
+        sinks       = p.sinks;
+        sink_pos_start = p.sink_pos_start;
+        copy_dev(tree.sort_index, p.sort_index, p.n_obj);
+        copy_dev(ghost_src, p.ghost_src, p.G);
+        using U32Pair = std::pair<DevArray<u32> *, DevArray<u32> *>;
+        for (auto [d, sarr] : {U32Pair{&nb.lpos, &p.lpos}, U32Pair{&nb.cnt, &p.cnt},
+                               U32Pair{&nb.offset, &p.offset}, U32Pair{&nb.ids, &p.ids},
+                               U32Pair{&nb.fcnt, &p.fcnt}, U32Pair{&nb.fids, &p.fids}})
+            copy_dev(*d, *sarr, sarr->size());
+        copy_dev(nb.ps, p.ps, p.ps.size());
+        using F64Pair = std::pair<DevArray<f64> *, DevArray<f64> *>;
+        for (auto [d, sarr] : {F64Pair{&x, &p.x}, F64Pair{&y, &p.y}, F64Pair{&z, &p.z},
+                               F64Pair{&h, &p.h}, F64Pair{&omega, &p.omega}}) {
+            // the merged arrays keep their capacity (cap >= N + G): copy into place
+            if (Nm)
+                SHAMX_HIP(hipMemcpyAsync(d->data(), sarr->data(), Nm * sizeof(f64),
+                                         hipMemcpyDeviceToDevice, stream));
+        }
+        if (tape.size() < p.passes) tape.resize(p.passes);
+        for (u32 i = 0; i < p.passes; i++) copy_dev(tape[i], p.tape[i], 4 * (size_t) N);
+        tree_valid = false; // the tree no longer describes the storage of the forward state
+        SHAMX_HIP(hipStreamSynchronize(stream));
+    }
+
     void record_pass_inputs(u32 pass) {
         if (tape.size() <= pass) tape.resize(pass + 1);
         DevArray<f64> &t = tape[pass];
@@ -1446,6 +1541,28 @@ int shamx_solver_adjoint_step(shamx_solver *s, double *const *seeds11) {
         if (s->sinks.empty()) s->adjoint_passes(seeds11, nullptr);
         else s->adjoint_step_with_sinks(seeds11);
         SHAMX_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int shamx_solver_products_save(shamx_solver *s, uint32_t slot) {
+    return guarded([&] {
+        need(s != nullptr, "shamx_solver_products_save: null solver");
+        s->products_save(slot);
+    });
+}
+
+int shamx_solver_products_load(shamx_solver *s, uint32_t slot) {
+    return guarded([&] {
+        need(s != nullptr, "shamx_solver_products_load: null solver");
+        s->products_load(slot);
+    });
+}
+
+int shamx_solver_products_free(shamx_solver *s) {
+    return guarded([&] {
+        need(s != nullptr, "shamx_solver_products_free: null solver");
+        SHAMX_HIP(hipStreamSynchronize(s->stream));
+        s->slots.clear();
     });
 }
 

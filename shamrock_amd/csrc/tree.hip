@@ -158,8 +158,13 @@ namespace shamx {
         // digit counts of every pass: each wave counts into its own LDS histograms (a quarter
         // of the contention on the atomics), 4 keys per load where the keys are 16-byte aligned
         __global__ void __launch_bounds__(RS_BLOCK)
-            k_rs_hist(u32 n, const u32 *__restrict keys, u32 passes, u32 *__restrict hist) {
+            k_rs_hist(u32 n, const u32 *__restrict keys, u32 passes, u32 *__restrict hist,
+                      u32 *__restrict status0, u32 n_status0) {
             __shared__ u32 h[RS_WAVES][4][RS_BINS];
+            // the first scatter pass's status words start at zero
+            for (u32 i = blockIdx.x * RS_BLOCK + threadIdx.x; i < n_status0;
+                 i += gridDim.x * RS_BLOCK)
+                status0[i] = 0u;
             for (int i = threadIdx.x; i < RS_WAVES * 4 * RS_BINS; i += RS_BLOCK)
                 (&h[0][0][0])[i] = 0;
             __syncthreads();
@@ -229,7 +234,8 @@ namespace shamx {
             k_rs_onesweep(u32 n, u32 shift, const u32 *__restrict keys_in,
                           const u32 *__restrict vals_in, u32 *__restrict keys_out,
                           u32 *__restrict vals_out, const u32 *__restrict digit_base,
-                          u32 *__restrict status, u32 *__restrict tile_counter, int first,
+                          u32 *__restrict status, u32 *__restrict next_status,
+                          u32 *__restrict tile_counter, int first, u32 nbits,
                           u32 *__restrict err) {
             constexpr u32 TILE = RS_BLOCK * ITEMS;
             __shared__ u32 s_tile;
@@ -244,31 +250,50 @@ namespace shamx {
             if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
             __syncthreads();
             const u32 tile = s_tile;
+            // the next pass's status words of this tile start at zero (read only after this
+            // launch has ended)
+            if (next_status) next_status[(u64) tile * RS_BINS + threadIdx.x] = 0u;
             const u64 base = (u64) tile * TILE + (u64) wid * (WAVE * ITEMS);
             const u64 lt   = lanemask_lt();
 
-            u32 key[ITEMS], val[ITEMS], rank[ITEMS];
+            // ranks inside the wave: peers (valid lanes holding the same digit) from one ballot
+            // per digit bit; the lowest peer adds the group's count to the wave's digit counter
+            // with one LDS atomic whose return value is the count of the earlier items (LDS
+            // operations of a wave complete in issue order, so the atomics of all items are
+            // issued back to back and waited for once), then broadcasts it to its peers
+            u32 key[ITEMS], val[ITEMS], rank[ITEMS], src[ITEMS];
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
                 u64 idx    = base + (u64) k * WAVE + lane;
                 bool valid = idx < n;
                 key[k]     = valid ? keys_in[idx] : 0u;
                 val[k]     = valid ? (first ? (u32) idx : vals_in[idx]) : 0u;
-                u32 d      = (key[k] >> shift) & 0xffu;
-                // peers: valid lanes holding the same digit
-                u64 peers = __ballot(valid);
+            }
 #pragma unroll
-                for (int b = 0; b < 8; b++) {
+            for (int k = 0; k < ITEMS; k++) {
+                u64 idx    = base + (u64) k * WAVE + lane;
+                bool valid = idx < n;
+                u32 d      = (key[k] >> shift) & 0xffu;
+                u64 peers  = __ballot(valid);
+#pragma unroll
+                for (u32 b = 0; b < 8; b++) {
+                    if (b >= nbits) break; // digit bits above the key's width are all 0
                     u64 bb = __ballot((d >> b) & 1u);
                     peers &= ((d >> b) & 1u) ? bb : ~bb;
                 }
                 u32 before = (u32) __popcll(peers & lt);
-                u32 cnt    = (u32) __popcll(peers);
-                u32 prev   = valid ? wave_hist[wid][d] : 0u;
-                rank[k]    = prev + before;
-                // the lowest peer writes the updated count after every peer has read it
-                if (valid && before == 0) wave_hist[wid][d] = prev + cnt;
-                __builtin_amdgcn_wave_barrier();
+                u32 prev   = 0;
+                if (valid && before == 0) prev = atomicAdd(&wave_hist[wid][d], (u32) __popcll(peers));
+                rank[k] = prev + before;
+                src[k]  = peers ? (u32) __builtin_ctzll(peers) : lane;
+            }
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) {
+                // mine: the leader's count of its digit in the earlier items, or a peer's
+                // position among its peers; the leader's value is added to its peers'
+                u32 mine = rank[k];
+                u32 lead = (u32) __shfl((int) mine, (int) src[k], WAVE);
+                rank[k]  = (lane == src[k]) ? mine : lead + mine;
             }
             __syncthreads();
 
@@ -685,22 +710,26 @@ namespace shamx {
         u32 *ctr     = hist + 4 * RS_BINS;
         u32 *err     = ctr + 4;
         u32 *status  = ctr + 8;
-        fill_u32(s, hist, 0, 4 * RS_BINS + 8 + (size_t) passes * nt * RS_BINS);
+        // digit counts and tile counters start at zero; the status words of pass 0 are zeroed
+        // by the histogram kernel, those of pass p + 1 by pass p's tiles
+        fill_u32(s, hist, 0, 4 * RS_BINS + 8);
         u32 hb = std::max<u32>(1, std::min<u32>(1024, div_up(n, RS_BLOCK * 16)));
-        k_rs_hist<<<hb, RS_BLOCK, 0, s>>>(n, keys, passes, hist);
+        k_rs_hist<<<hb, RS_BLOCK, 0, s>>>(n, keys, passes, hist, status, nt * RS_BINS);
         k_rs_hist_scan<<<passes, WAVE, 0, s>>>(hist);
         u32 *kin = keys, *vin = vals, *kout = keys_alt, *vout = vals_alt;
         for (u32 p = 0; p < passes; p++) {
+            u32 *st      = status + (u64) p * nt * RS_BINS;
+            u32 *st_next = p + 1 < passes ? st + (u64) nt * RS_BINS : nullptr;
+            const u32 nbits = std::min<u32>(8, key_bits - 8 * p);
+            const int first = (p == 0 && !vals_given) ? 1 : 0;
             if (small)
                 k_rs_onesweep<RS_ITEMS_SMALL><<<nt, RS_BLOCK, 0, s>>>(
-                    n, 8 * p, kin, vin, kout, vout, hist + p * RS_BINS,
-                    status + (u64) p * nt * RS_BINS, ctr + p, (p == 0 && !vals_given) ? 1 : 0,
-                    err);
+                    n, 8 * p, kin, vin, kout, vout, hist + p * RS_BINS, st, st_next, ctr + p,
+                    first, nbits, err);
             else
                 k_rs_onesweep<RS_ITEMS><<<nt, RS_BLOCK, 0, s>>>(
-                    n, 8 * p, kin, vin, kout, vout, hist + p * RS_BINS,
-                    status + (u64) p * nt * RS_BINS, ctr + p, (p == 0 && !vals_given) ? 1 : 0,
-                    err);
+                    n, 8 * p, kin, vin, kout, vout, hist + p * RS_BINS, st, st_next, ctr + p,
+                    first, nbits, err);
             std::swap(kin, kout);
             std::swap(vin, vout);
         }

@@ -90,6 +90,17 @@ class GpuSolver:
             raise ValueError(f"storage order has {a.size} ids, the solver {self.n} particles")
         check(lib().shamx_solver_set_storage_order(self.h, a.ctypes.data))
 
+    def products_save(self, slot):
+        """Keep what the adjoint of the last step reads in `slot` (the adjoint's product store)."""
+        check(lib().shamx_solver_products_save(self.h, int(slot)))
+
+    def products_load(self, slot):
+        """Put the products of `slot` back (the fields then hold that step's end positions)."""
+        check(lib().shamx_solver_products_load(self.h, int(slot)))
+
+    def products_free(self):
+        check(lib().shamx_solver_products_free(self.h))
+
     def get_device(self, name, out=None):
         """Device copy of a field as a torch tensor (float64, cuda)."""
         import torch

@@ -272,16 +272,20 @@ def test_adjoint_rejects_lp07_with_periodic_images():
 
 
 def test_adjoint_independent_of_checkpoint_interval():
+    """The gradient does not depend on the checkpoint interval, nor on whether the backward
+    pass takes the steps' products from the product store or from replays: bit for bit."""
     say("[adjoint ckpt]")
     cfg, f, v0, u0 = _case("random_box")
     steps = 5
     dts = rollout_dts(cfg, f, v0, u0, steps)
     c = torch.tensor(np.random.default_rng(1).normal(size=v0.shape[0]), device=DEV)
     grads = []
-    for k in (1, 2, 5):
+    for k, store in ((1, True), (2, True), (5, True), (2, False), (5, False)):
         v = v0.clone().requires_grad_(True)
         u = u0.clone().requires_grad_(True)
-        _loss(cfg, f, v, u, c, steps, dts, k=k).backward()
+        rho = rollout_density(cfg, f, v, u, steps, checkpoint_every=k, dts=dts,
+                              store_products=store)
+        (rho * c).sum().backward()
         grads.append((v.grad.cpu().numpy(), u.grad.cpu().numpy()))
     for g in grads[1:]:
         np.testing.assert_array_equal(g[0], grads[0][0])
@@ -314,29 +318,41 @@ def test_fit_initial_velocity_to_target_density():
     assert losses[-1] < losses[0]
 
 
+def _rerun_case():
+    """A rollout whose correctors rerun: the jittered random box (u ~ 1, v ~ 0.05) at multiples
+    of the solver's time steps, else a Sedov blast at 12^3."""
+    for name, factors in (("random_box", (2, 4, 8)), ("sedov", (1, 2))):
+        if name == "random_box":
+            cfg, f = setups.random_box(1000, seed=4, jitter=0.3)
+        else:
+            cfg, f = setups.sedov(12)
+        cfg = dict(cfg, epsilon_h=1e-12)
+        rng = np.random.default_rng(5)
+        v = np.stack([f["vx"], f["vy"], f["vz"]], 1)
+        v0 = torch.tensor(v + 1e-3 * rng.standard_normal(v.shape), device=DEV)
+        u0 = torch.tensor(f["u"] * (1 + 0.05 * rng.standard_normal(f["u"].size)), device=DEV)
+        steps = 4
+        base = rollout_dts(cfg, f, v0, u0, steps)
+        for factor in factors:
+            dts = [factor * d for d in base]
+            passes = rollout_passes(cfg, f, v0, u0, steps, dts)
+            say(f"[adjoint reruns] {name} dt x {factor}: corrector passes per step {passes}")
+            if max(passes) >= 2:
+                return name, cfg, f, v0, u0, steps, dts, passes
+    return None
+
+
 def test_adjoint_through_corrector_reruns():
-    """Steps whose corrector reruns (a Sedov blast at 12^3, at the time steps the solver picks or
-    multiples of them, until a pass's velocity change is above the 1e-2 tolerance) are
-    differentiated through every pass: pass i evaluates the force from the velocities and u of
-    pass i - 1.  The
-    gradient of sum c_a rho_a w.r.t. v0 and u0 is compared with central differences, for the
-    FP64 and the FP32 adjoint."""
+    """Steps whose corrector reruns (time steps a multiple of those the solver picks, until a
+    pass's velocity change is above the 1e-2 tolerance) are differentiated through every pass:
+    pass i evaluates the force from the velocities and u of pass i - 1.  The gradient of
+    sum c_a rho_a w.r.t. v0 and u0 is compared with central differences (relative steps for u;
+    a component whose perturbed runs take other numbers of passes is a jump of the map, not a
+    derivative, and is left out), for the FP64 and the FP32 adjoint."""
     say("[adjoint reruns]")
-    cfg, f = setups.sedov(12)
-    cfg = dict(cfg, epsilon_h=1e-12)
-    rng = np.random.default_rng(5)
-    v = np.stack([f["vx"], f["vy"], f["vz"]], 1)
-    v0 = torch.tensor(v + 1e-3 * rng.standard_normal(v.shape), device=DEV)
-    u0 = torch.tensor(f["u"] * (1 + 0.05 * rng.standard_normal(f["u"].size)), device=DEV)
-    steps = 4
-    base = rollout_dts(cfg, f, v0, u0, steps)
-    for factor in (1, 2, 4):
-        dts = [factor * d for d in base]
-        passes = rollout_passes(cfg, f, v0, u0, steps, dts)
-        say(f"[adjoint reruns] dt x {factor}: corrector passes per step {passes}")
-        if max(passes) >= 2:
-            break
-    assert max(passes) >= 2
+    case = _rerun_case()
+    assert case is not None, "no rollout with corrector reruns"
+    name, cfg, f, v0, u0, steps, dts, passes = case
     rng = np.random.default_rng(3)
     c = torch.tensor(rng.normal(size=v0.shape[0]), device=DEV)
     grads = {}
@@ -347,30 +363,46 @@ def test_adjoint_through_corrector_reruns():
         _loss(cf, f, v, u, c, steps, dts).backward()
         grads[fp32] = (v.grad.cpu().numpy(), u.grad.cpu().numpy())
     n = v0.shape[0]
+    uabs = np.abs(u0.cpu().numpy())
     comps = [(int(i), int(d)) for i, d in zip(rng.integers(0, n, 32), rng.integers(0, 3, 32))]
-    comps_u = [int(i) for i in rng.integers(0, n, 8)]
-    fd_v, fd_u = [], []
+    comps_u = [int(i) for i in rng.choice(np.argsort(uabs)[-n // 4:], 12, replace=False)]
+    fd_v, fd_u, keep_v, keep_u = [], [], [], []
+
+    def same_passes(vv, uu):
+        return rollout_passes(cfg, f, vv, uu, steps, dts) == passes
+
     with torch.no_grad():
         for i, d in comps:
             e = 1e-6
             vp, vm = v0.clone(), v0.clone()
             vp[i, d] += e
             vm[i, d] -= e
+            keep_v.append(same_passes(vp, u0) and same_passes(vm, u0))
             fd_v.append((_loss(cfg, f, vp, u0, c, steps, dts) -
                          _loss(cfg, f, vm, u0, c, steps, dts)).item() / (2 * e))
         for i in comps_u:
-            e = 1e-5
+            e = 1e-5 * uabs[i]
             up, um = u0.clone(), u0.clone()
             up[i] += e
             um[i] -= e
+            keep_u.append(same_passes(v0, up) and same_passes(v0, um))
+            # the loss change per relative change of u_i
             fd_u.append((_loss(cfg, f, v0, up, c, steps, dts) -
-                         _loss(cfg, f, v0, um, c, steps, dts)).item() / (2 * e))
-    fd_v, fd_u = np.array(fd_v), np.array(fd_u)
+                         _loss(cfg, f, v0, um, c, steps, dts)).item() / (2 * e) * uabs[i])
+    keep_v, keep_u = np.array(keep_v), np.array(keep_u)
+    say(f"[adjoint reruns] {name}: components kept v {keep_v.sum()}/{keep_v.size}, "
+        f"u {keep_u.sum()}/{keep_u.size}")
+    assert keep_v.sum() >= 16 and keep_u.sum() >= 6
+    fd_v, fd_u = np.array(fd_v)[keep_v], np.array(fd_u)[keep_u]
+    cv = [cd for cd, k in zip(comps, keep_v) if k]
+    cu = [i for i, k in zip(comps_u, keep_u) if k]
     scale_v, scale_u = np.abs(fd_v).max(), np.abs(fd_u).max()
     assert scale_v > 0 and scale_u > 0
     for fp32, (gv, gu) in grads.items():
-        err_v = np.abs(np.array([gv[i, d] for i, d in comps]) - fd_v).max() / scale_v
-        err_u = np.abs(np.array([gu[i] for i in comps_u]) - fd_u).max() / scale_u
+        ad_v = np.array([gv[i, d] for i, d in cv])
+        ad_u = np.array([gu[i] * uabs[i] for i in cu])
+        err_v = np.abs(ad_v - fd_v).max() / scale_v
+        err_u = np.abs(ad_u - fd_u).max() / scale_u
         say(f"[adjoint reruns] fp32={fp32}: rel err v {err_v:.2e}, u {err_u:.2e}")
         assert err_v < 1e-4 and err_u < 1e-4
     g64, g32 = grads[0][0], grads[1][0]
