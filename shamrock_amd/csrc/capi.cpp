@@ -158,7 +158,12 @@ struct shamx_solver {
     // per-particle gathers of the tree, records and outputs read and write nearly contiguous
     // memory.  pid[j] = particle id (the caller's order) of storage slot j; field getters and
     // setters and the inspection getters translate, so every result keeps the caller's order.
-    DevArray<u32> pid, pid_tmp, reo_perm, reo_flag, reo_off;
+    DevArray<u32> pid, pid_tmp, reo_perm, reo_flag, reo_off, reo_count;
+    // the next step's reorder, prepared after the step's last tree build: reo_perm holds the
+    // permutation and reo_moved (read with the corrector's reductions, no sync of its own) how
+    // many particles it moves; reo_known says both describe the current storage and tree
+    u32 reo_moved  = 0;
+    bool reo_known = false;
     DevArray<unsigned char> reo_scan;
     DevArray<f64> tmp_f64;
     bool pid_identity = true; // pid[j] = j (nothing reordered yet)
@@ -224,10 +229,13 @@ struct shamx_solver {
                    tree.sort_index.data()};
     }
 
-    // permute every per-particle field of the local particles into the sorted order of the
-    // last tree build (one gather per field; the images are rebuilt by the step anyway)
-    void reorder_storage() {
-        if (cfg.keep_storage_order || !tree_valid || tree.n_obj != N + G || N == 0) return;
+    // the permutation from the storage to the last tree's sorted order of the local particles
+    // (in reo_perm), and whether the storage is far enough from it to reorder: the reorder
+    // moves every field (22 gathers, about 0.2 ms at 2.1 M particles), so it waits until at
+    // least 1/32 of the particles have changed places since the last one
+    bool pending_reorder() {
+        if (cfg.keep_storage_order || !tree_valid || tree.n_obj != N + G || N == 0) return false;
+        if (reo_known) return (u64) reo_moved * 32 >= N;
         const u32 Nm = N + G;
         reo_perm.resize(N);
         reo_flag.resize(Nm);
@@ -235,6 +243,33 @@ struct shamx_solver {
         reo_scan.resize(std::max(reo_scan.size(), scan_scratch_bytes(Nm)));
         local_sorted_order(stream, Nm, N, tree.sort_index.data(), reo_perm.data(),
                            reo_flag.data(), reo_off.data(), reo_scan.data());
+        const u32 moved = perm_moved(stream, N, reo_perm.data(), reo_flag.data());
+        return (u64) moved * 32 >= N;
+    }
+
+    // after the step's last tree build: the permutation of the next reorder and its count of
+    // moved particles on the device (the count is read with the corrector's reductions)
+    bool prepare_reorder() {
+        reo_known = false;
+        if (cfg.keep_storage_order || !tree_valid || tree.n_obj != N + G || N == 0) return false;
+        const u32 Nm = N + G;
+        reo_perm.resize(N);
+        reo_flag.resize(Nm);
+        reo_off.resize(Nm);
+        reo_count.resize(1);
+        reo_scan.resize(std::max(reo_scan.size(), scan_scratch_bytes(Nm)));
+        local_sorted_order(stream, Nm, N, tree.sort_index.data(), reo_perm.data(),
+                           reo_flag.data(), reo_off.data(), reo_scan.data());
+        perm_moved_async(stream, N, reo_perm.data(), reo_count.data());
+        return true;
+    }
+
+    // permute every per-particle field of the local particles into the sorted order of the
+    // last tree build (one gather per field; the images are rebuilt by the step anyway)
+    void reorder_storage() {
+        const bool go = pending_reorder();
+        reo_known     = false;
+        if (!go) return;
         const u32 *perm = reo_perm.data();
         for (auto *f : {&x, &y, &z, &h, &vx, &vy, &vz, &u, &ax, &ay, &az, &du, &alpha_av,
                         &soundspeed, &divv, &curlx, &curly, &curlz, &dtdivv, &axe, &aye, &aze}) {
@@ -260,15 +295,7 @@ struct shamx_solver {
     // the storage order the next step will run in (the pending reorder applied to pid), as
     // particle ids of the caller's order; the storage itself is left as it is
     std::vector<u32> next_storage_order() {
-        if (cfg.keep_storage_order || !tree_valid || tree.n_obj != N + G || N == 0)
-            return host_pid();
-        const u32 Nm = N + G;
-        reo_perm.resize(N);
-        reo_flag.resize(Nm);
-        reo_off.resize(Nm);
-        reo_scan.resize(std::max(reo_scan.size(), scan_scratch_bytes(Nm)));
-        local_sorted_order(stream, Nm, N, tree.sort_index.data(), reo_perm.data(),
-                           reo_flag.data(), reo_off.data(), reo_scan.data());
+        if (!pending_reorder()) return host_pid();
         std::vector<u32> perm(N), p = host_pid(), out(N);
         SHAMX_HIP(hipMemcpyAsync(perm.data(), reo_perm.data(), N * sizeof(u32),
                                  hipMemcpyDeviceToHost, stream));
@@ -386,18 +413,19 @@ struct shamx_solver {
             G = 0;
             return;
         }
+        // interaction range max h * htol_up_coarse * R, from the max key on the device
         fill_u64(stream, red.data(), ORDERED_NEG_INF, 1);
         max_key_f64(stream, N, h.data(), red.data());
-        f64 hmax      = read_key(0);
-        f64 int_range = hmax * cfg.htol_up_coarse * Rkern;
-        f64 box6[6]   = {cfg.box_min[0], cfg.box_min[1], cfg.box_min[2],
-                         cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
+        f64 box6[6] = {cfg.box_min[0], cfg.box_min[1], cfg.box_min[2],
+                       cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
         u32 g = periodic_ghosts(stream, N, x.data(), y.data(), z.data(), h.data(), box6,
-                                int_range, ghost_src, ghost_oid, gwork, gscratch, cap - N);
+                                red.data(), cfg.htol_up_coarse, Rkern, ghost_src, ghost_oid,
+                                gwork, gscratch, cap - N);
         if (g > cap - N) {
             grow(N + g);
             g = periodic_ghosts(stream, N, x.data(), y.data(), z.data(), h.data(), box6,
-                                int_range, ghost_src, ghost_oid, gwork, gscratch, cap - N);
+                                red.data(), cfg.htol_up_coarse, Rkern, ghost_src, ghost_oid,
+                                gwork, gscratch, cap - N);
         }
         G = g;
     }
@@ -406,6 +434,7 @@ struct shamx_solver {
     // require_converged false, a last subcycle that did not converge is kept as it is (h of
     // the unconverged particles clamped to the cache radius) instead of failing the step.
     void prestep(u32 max_sub, bool require_converged) {
+        reo_known = false; // the tree built here replaces the one a prepared reorder follows
         u32 hstep = 0, hmax = max_sub;
         last.h_iterations = 0;
         for (; hstep < hmax; hstep++) {
@@ -543,6 +572,7 @@ struct shamx_solver {
         }
         (void) box6;
         prestep((u32) cfg.h_max_subcycles, true);
+        const bool count_pending = prepare_reorder();
 
         f64 next_cfl = 0;
         u32 corr_iter = 0;
@@ -611,7 +641,11 @@ struct shamx_solver {
             timer.end(stream, PH_CORR, b);
             u64 k[5];
             SHAMX_HIP(hipMemcpyAsync(k, red.data() + 3, 40, hipMemcpyDeviceToHost, stream));
+            if (count_pending)
+                SHAMX_HIP(hipMemcpyAsync(&reo_moved, reo_count.data(), sizeof(u32),
+                                         hipMemcpyDeviceToHost, stream));
             SHAMX_HIP(hipStreamSynchronize(stream));
+            if (count_pending) reo_known = true;
             f64 dt_min         = ordered_to_f64(k[0]);
             last.n_interacting = k[4];
             f64 max_e2 = ordered_to_f64(k[1]);

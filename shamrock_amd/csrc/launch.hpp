@@ -20,6 +20,10 @@ namespace shamx {
     size_t scan_scratch_bytes(u32 n);
     void exclusive_scan_u32(hipStream_t s, const u32 *in, u32 *out, u32 n, u32 *total,
                             void *scratch);
+    // the same over the first min(n_max, *n_dev) values, a length known on the device only
+    // (no synchronisation; scratch sized for n_max)
+    void exclusive_scan_u32_devn(hipStream_t s, const u32 *in, u32 *out, u32 n_max,
+                                 const u32 *n_dev, u32 *total, void *scratch);
 
     // ------------------------------------------------------------------ tree.hip
     // bounding box of n positions widened outward by one ulp (BuildTrees.cpp:38-55);
@@ -246,9 +250,11 @@ namespace shamx {
                 u64 *dt_min_key, DevArray<u64> &part);
     // periodic image selection (BasicSPHGhosts.cpp:260-579): returns G (synchronises) and
     // fills src ids + appends positions/h of the images after the n local entries.
+    // The interaction range is (hmax * f1) * f2 with hmax the ordered key at *hmax_key (device).
     u32 periodic_ghosts(hipStream_t s, u32 n, f64 *x, f64 *y, f64 *z, f64 *h, const f64 *box6,
-                        f64 int_range, DevArray<u32> &ghost_src, DevArray<u8> &ghost_off_id,
-                        DevArray<u32> &work, DevArray<unsigned char> &scratch, u32 cap);
+                        const u64 *hmax_key, f64 f1, f64 f2, DevArray<u32> &ghost_src,
+                        DevArray<u8> &ghost_off_id, DevArray<u32> &work,
+                        DevArray<unsigned char> &scratch, u32 cap);
     // ghost field refresh: dst[n + g] = src[ghost_src[g]] (+ offset for positions)
     struct FieldPtrs {
         f64 *p[8];
@@ -260,6 +266,10 @@ namespace shamx {
     void local_sorted_order(hipStream_t s, u32 nm, u32 n, const u32 *sort_index, u32 *perm,
                             u32 *flag, u32 *off, void *scan_scratch);
     // dst[j] = src[perm[j]] (gather) and dst[perm[j]] = src[j] (scatter), j < n
+    // entries of perm[0..n) that differ from their index (synchronises to return the count)
+    u32 perm_moved(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev);
+    // the same count left in *count_dev (no synchronisation)
+    void perm_moved_async(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev);
     void perm_gather_f64(hipStream_t s, u32 n, const u32 *perm, const f64 *src, f64 *dst);
     void perm_scatter_f64(hipStream_t s, u32 n, const u32 *perm, const f64 *src, f64 *dst);
     void perm_gather_u32(hipStream_t s, u32 n, const u32 *perm, const u32 *src, u32 *dst);

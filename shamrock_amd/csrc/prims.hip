@@ -43,8 +43,10 @@ namespace shamx {
         }
 
         __global__ void __launch_bounds__(SCAN_BLOCK)
-            k_tile_sums(const u32 *__restrict in, u32 n, u32 *__restrict tile_sums) {
+            k_tile_sums(const u32 *__restrict in, u32 n, u32 *__restrict tile_sums,
+                        const u32 *__restrict n_dev) {
             __shared__ u32 lds_waves[SCAN_BLOCK / WAVE];
+            if (n_dev) n = min(n, *n_dev); // the length known on the device only
             u64 base = (u64) blockIdx.x * SCAN_TILE;
             u32 s = 0;
 #pragma unroll
@@ -75,10 +77,12 @@ namespace shamx {
 
         __global__ void __launch_bounds__(SCAN_BLOCK)
             k_scan_tiles_apply(const u32 *__restrict in, u32 *__restrict out, u32 n,
-                               const u32 *__restrict tile_base) {
+                               const u32 *__restrict tile_base, const u32 *__restrict n_dev) {
             __shared__ u32 tile[SCAN_TILE];
             __shared__ u32 lds_waves[SCAN_BLOCK / WAVE];
+            if (n_dev) n = min(n, *n_dev);
             u64 base = (u64) blockIdx.x * SCAN_TILE;
+            if (base >= n) return; // block-uniform
 #pragma unroll
             for (int k = 0; k < SCAN_ITEMS; k++) {
                 u64 i                               = base + (u64) k * SCAN_BLOCK + threadIdx.x;
@@ -131,9 +135,23 @@ namespace shamx {
         }
         u32 nt          = div_up(n, SCAN_TILE);
         u32 *tile_sums  = (u32 *) scratch;
-        k_tile_sums<<<nt, SCAN_BLOCK, 0, s>>>(in, n, tile_sums);
+        k_tile_sums<<<nt, SCAN_BLOCK, 0, s>>>(in, n, tile_sums, nullptr);
         k_scan_tiles<<<1, SCAN_BLOCK, 0, s>>>(tile_sums, nt, total);
-        k_scan_tiles_apply<<<nt, SCAN_BLOCK, 0, s>>>(in, out, n, tile_sums);
+        k_scan_tiles_apply<<<nt, SCAN_BLOCK, 0, s>>>(in, out, n, tile_sums, nullptr);
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    void exclusive_scan_u32_devn(hipStream_t s, const u32 *in, u32 *out, u32 n_max,
+                                 const u32 *n_dev, u32 *total, void *scratch) {
+        if (n_max == 0) {
+            if (total) fill_u32(s, total, 0, 1);
+            return;
+        }
+        u32 nt         = div_up(n_max, SCAN_TILE);
+        u32 *tile_sums = (u32 *) scratch;
+        k_tile_sums<<<nt, SCAN_BLOCK, 0, s>>>(in, n_max, tile_sums, n_dev);
+        k_scan_tiles<<<1, SCAN_BLOCK, 0, s>>>(tile_sums, nt, total);
+        k_scan_tiles_apply<<<nt, SCAN_BLOCK, 0, s>>>(in, out, n_max, tile_sums, n_dev);
         SHAMX_HIP(hipGetLastError());
     }
 

@@ -354,19 +354,23 @@ namespace shamx {
         // rows of the sweep list whose particle still iterates (eps > 1e-6: the sweep's own
         // test); the eps of the rows dropped is final for the subcycle and goes into the
         // frozen max / min
+        // (grid-stride over few blocks: one pair of same-address atomics per block)
         __global__ void __launch_bounds__(BLK)
             k_h_active_flags(u32 n, const u32 *__restrict sel, const u32 *__restrict outi,
                              const f64 *__restrict eps, u32 *__restrict flag,
                              u64 *__restrict st) {
             __shared__ u64 lds[2][BLK / WAVE];
-            u32 j = blockIdx.x * BLK + threadIdx.x;
             u64 kmax = ORDERED_NEG_INF, kmin = ORDERED_POS_INF;
-            if (j < n) {
+            for (u32 j = blockIdx.x * BLK + threadIdx.x; j < n; j += gridDim.x * BLK) {
                 u32 row = sel ? sel[j] : j;
                 f64 e   = eps[outi ? outi[row] : row];
                 bool keep = e > 1e-6;
                 flag[j]   = keep ? 1u : 0u;
-                if (!keep) kmax = kmin = f64_to_ordered(e);
+                if (!keep) {
+                    u64 k = f64_to_ordered(e);
+                    kmax  = umax64(kmax, k);
+                    kmin  = umin64(kmin, k);
+                }
             }
             kmax = block_reduce(kmax, umax64, lds[0]);
             kmin = block_reduce(kmin, umin64, lds[1]);
@@ -1300,6 +1304,34 @@ namespace shamx {
             u32 active[26];
         };
 
+        // the 26 image offsets and cut volumes of the periodic box (BasicSPHGhosts.cpp:260-509),
+        // offset o in (x, y, z) nesting order without (0, 0, 0), from the interaction range
+        // (hmax * f1) * f2 with hmax read from the device (no host round trip)
+        struct Box6 {
+            f64 v[6];
+        };
+        __global__ void k_cut_boxes(Box6 box, const u64 *__restrict hmax_key, f64 f1, f64 f2,
+                                    CutBoxes *__restrict cb) {
+            const int o = threadIdx.x;
+            if (o >= 26) return;
+            const int q     = o < 13 ? o : o + 1;
+            const int io[3] = {q / 9 - 1, (q / 3) % 3 - 1, q % 3 - 1};
+            const f64 int_range = (ordered_to_f64(*hmax_key) * f1) * f2;
+            bool nonempty = true;
+            for (int k = 0; k < 3; k++) {
+                f64 bmin = box.v[k], bmax = box.v[3 + k];
+                f64 bsize = bmax - bmin, pmin = bmin, pmax = bsize + bmin;
+                f64 off = io[k] * bsize;
+                f64 rlo = pmin - int_range, rhi = pmax + int_range;
+                f64 ilo = fmax(rlo, pmin + off), ihi = fmin(rhi, pmax + off);
+                if (!(ihi >= ilo)) nonempty = false;
+                cb->off[o][k] = off;
+                cb->lo[o][k]  = fmax(pmin, rlo - off);
+                cb->hi[o][k]  = fmin(pmax, rhi - off);
+            }
+            cb->active[o] = nonempty ? 1u : 0u;
+        }
+
         __device__ inline bool in_cut(const CutBoxes &cb, int o, f64 px, f64 py, f64 pz) {
             return cb.active[o] && (cb.lo[o][0] <= px) && (px < cb.hi[o][0]) &&
                    (cb.lo[o][1] <= py) && (py < cb.hi[o][1]) && (cb.lo[o][2] <= pz) &&
@@ -1495,8 +1527,8 @@ namespace shamx {
                 DevArray<u32> &dst = (sel == b.sel_a.data()) ? b.sel_b : b.sel_a;
                 dst.resize(std::max<u32>(keep, 1));
                 b.scan.resize(std::max(b.scan.size(), scan_scratch_bytes(n_act)));
-                k_h_active_flags<<<div_up(n_act, BLK), BLK, 0, s>>>(n_act, sel, c.outi, eps,
-                                                                    b.flag.data(), b.st.data());
+                k_h_active_flags<<<std::min<u32>(256, div_up(n_act, BLK)), BLK, 0, s>>>(
+                    n_act, sel, c.outi, eps, b.flag.data(), b.st.data());
                 exclusive_scan_u32(s, b.flag.data(), b.off.data(), n_act, nullptr, b.scan.data());
                 k_h_compact_rows<<<div_up(n_act, 256), 256, 0, s>>>(n_act, sel, b.flag.data(),
                                                                    b.off.data(), dst.data());
@@ -1723,35 +1755,10 @@ namespace shamx {
     // offset o = (xo, yo, zo) in lexicographic order without (0,0,0); interf volume =
     // sender box ∩ (receiver box expanded by int_range - offset); membership half-open.
     u32 periodic_ghosts(hipStream_t s, u32 n, f64 *x, f64 *y, f64 *z, f64 *h, const f64 *box6,
-                        f64 int_range, DevArray<u32> &ghost_src, DevArray<u8> &ghost_off_id,
-                        DevArray<u32> &work, DevArray<unsigned char> &scratch, u32 cap) {
-        CutBoxes cb{};
-        f64 bmin[3] = {box6[0], box6[1], box6[2]}, bmax[3] = {box6[3], box6[4], box6[5]};
-        f64 bsize[3], pmin[3], pmax[3];
-        for (int k = 0; k < 3; k++) {
-            bsize[k] = bmax[k] - bmin[k];
-            pmin[k]  = bmin[k];
-            pmax[k]  = bsize[k] + bmin[k];
-        }
-        int o = 0;
-        for (int xo = -1; xo <= 1; xo++)
-            for (int yo = -1; yo <= 1; yo++)
-                for (int zo = -1; zo <= 1; zo++) {
-                    if (xo == 0 && yo == 0 && zo == 0) continue;
-                    int io[3] = {xo, yo, zo};
-                    bool nonempty = true;
-                    for (int k = 0; k < 3; k++) {
-                        f64 off = io[k] * bsize[k];
-                        f64 rlo = pmin[k] - int_range, rhi = pmax[k] + int_range;
-                        f64 ilo = std::fmax(rlo, pmin[k] + off), ihi = std::fmin(rhi, pmax[k] + off);
-                        if (!(ihi >= ilo)) nonempty = false;
-                        cb.off[o][k] = off;
-                        cb.lo[o][k]  = std::fmax(pmin[k], rlo - off);
-                        cb.hi[o][k]  = std::fmin(pmax[k], rhi - off);
-                    }
-                    cb.active[o] = nonempty ? 1u : 0u;
-                    o++;
-                }
+                        const u64 *hmax_key, f64 f1, f64 f2, DevArray<u32> &ghost_src,
+                        DevArray<u8> &ghost_off_id, DevArray<u32> &work,
+                        DevArray<unsigned char> &scratch, u32 cap) {
+        Box6 box{{box6[0], box6[1], box6[2], box6[3], box6[4], box6[5]}};
         u32 nb = div_up(std::max<u32>(n, 1), BLK);
         // work: [cutboxes (as u32 words)] [26*nb counts] [26*nb bases] [total]
         size_t cb_words = (sizeof(CutBoxes) + 3) / 4;
@@ -1762,7 +1769,7 @@ namespace shamx {
         u32 *bases       = counts + (size_t) 26 * nb;
         u32 *total       = bases + (size_t) 26 * nb;
         scratch.resize(std::max<size_t>(scratch.size(), scan_scratch_bytes(26 * nb)));
-        SHAMX_HIP(hipMemcpyAsync(cbd, &cb, sizeof(CutBoxes), hipMemcpyHostToDevice, s));
+        k_cut_boxes<<<1, 32, 0, s>>>(box, hmax_key, f1, f2, cbd);
         k_ghost_count<<<nb, BLK, 0, s>>>(n, x, y, z, cbd, counts, nb);
         exclusive_scan_u32(s, counts, bases, 26 * nb, total, scratch.data());
         u32 G = 0;
@@ -1791,6 +1798,22 @@ namespace shamx {
             u32 k = blockIdx.x * blockDim.x + threadIdx.x;
             if (k < nm && flag[k]) perm[off[k]] = sort_index[k];
         }
+        // entries of perm that are not the identity (few blocks: one atomic per block)
+        __global__ void __launch_bounds__(256)
+            k_perm_moved(u32 n, const u32 *__restrict perm, u32 *__restrict count) {
+            u32 c = 0;
+            for (u32 j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256)
+                c += perm[j] != j ? 1u : 0u;
+            c = wave_reduce(c, [](u32 a, u32 b) { return a + b; });
+            __shared__ u32 lds[256 / WAVE];
+            if (__lane_id() == 0) lds[threadIdx.x / WAVE] = c;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                u32 t = 0;
+                for (int w = 0; w < 256 / WAVE; w++) t += lds[w];
+                if (t) atomicAdd(count, t);
+            }
+        }
         template<class T>
         __global__ void k_perm_gather(u32 n, const u32 *__restrict perm, const T *__restrict src,
                                       T *__restrict dst) {
@@ -1804,6 +1827,21 @@ namespace shamx {
             if (j < n) dst[perm[j]] = src[j];
         }
     } // namespace
+
+    void perm_moved_async(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev) {
+        fill_u32(s, count_dev, 0, 1);
+        if (n) k_perm_moved<<<std::min<u32>(512, div_up(n, 256)), 256, 0, s>>>(n, perm, count_dev);
+        SHAMX_HIP(hipGetLastError());
+    }
+
+    u32 perm_moved(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev) {
+        if (n == 0) return 0;
+        perm_moved_async(s, n, perm, count_dev);
+        u32 c = 0;
+        SHAMX_HIP(hipMemcpyAsync(&c, count_dev, sizeof(u32), hipMemcpyDeviceToHost, s));
+        SHAMX_HIP(hipStreamSynchronize(s));
+        return c;
+    }
 
     void local_sorted_order(hipStream_t s, u32 nm, u32 n, const u32 *sort_index, u32 *perm,
                             u32 *flag, u32 *off, void *scan_scratch) {

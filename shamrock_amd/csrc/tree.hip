@@ -432,6 +432,33 @@ namespace shamx {
             s_out[i] = (!(d0 < dm && dmm < dm) && s_in[i]) ? 1u : 0u;
         }
 
+        // sycl_reduction_iteration on the compacted list of kept positions: F[0..K) are the
+        // positions whose split flag is set (K on the device); keep[j] is F[j]'s new flag.  The
+        // reference walks from F[j] to the nearest set flags on either side (reduction_alg.cpp:
+        // 196-244): those are F[j - 1], F[j - 2] and F[j + 1], with the wrapped u32 values
+        // (-1, -2) the walks stop at when no flag is left before, and n when none is after.
+        __global__ void k_reduc_keep(u32 n, const u32 *__restrict m, const u32 *__restrict F,
+                                     const u32 *__restrict K_dev, u32 *__restrict keep) {
+            const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+            const u32 K = *K_dev;
+            if (j >= K) return;
+            const u32 i       = F[j];
+            const u32 before1 = j >= 1 ? F[j - 1] : 0xFFFFFFFFu;
+            const u32 before2 = j >= 2 ? F[j - 2] : (j == 1 ? 0xFFFFFFFFu : 0xFFFFFFFEu);
+            const u32 next1   = j + 1 < K ? F[j + 1] : n;
+            i32 d0  = kdelta((i32) i, (i32) next1, n, m);
+            i32 dm  = kdelta((i32) i, (i32) before1, n, m);
+            i32 dmm = kdelta((i32) before1, (i32) before2, n, m);
+            keep[j] = !(d0 < dm && dmm < dm) ? 1u : 0u;
+        }
+        // F_out[off[j]] = F_in[j] for the kept j < K (K on the device)
+        __global__ void k_compact_list(const u32 *__restrict K_dev, const u32 *__restrict keep,
+                                       const u32 *__restrict off, const u32 *__restrict F_in,
+                                       u32 *__restrict F_out) {
+            const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+            if (j < *K_dev && keep[j]) F_out[off[j]] = F_in[j];
+        }
+
         __global__ void k_compact(u32 n, const u32 *__restrict flag, const u32 *__restrict off,
                                   u32 *__restrict map) {
             u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -795,7 +822,7 @@ namespace shamx {
                                    scan_scratch_bytes(n)});
         t.scratch.resize(scratch);
         t.bbox6.resize(6);
-        t.counters.resize(8);
+        t.counters.resize(12);
         for (auto *a : {&t.codes, &t.sorted, &t.sort_index, &t.keys_alt, &t.vals_alt, &t.split_a,
                         &t.split_b})
             a->resize(n);
@@ -836,26 +863,35 @@ namespace shamx {
 
     void tree_stage_karras(hipStream_t s, TreeBuffers &t, u32 reduction_level) {
         u32 n = t.n_obj;
-        // leaf reduction
+        // leaf reduction (reduction_alg.cpp:413-456) on the compacted list of split positions:
+        // each iteration tests the positions still kept and compacts them, so it costs the
+        // number of kept positions, not n (the lengths stay on the device: no host round trip)
         u32 g = div_up(n, 256);
         k_split<<<g, 256, 0, s>>>(n, t.sorted.data(), t.split_a.data());
-        u32 *sin = t.split_a.data(), *sout = t.split_b.data();
         t.red_cnt.resize(std::max<u32>(n, 1));
         t.red_pos.resize(std::max<u32>(n, 1));
-        u32 *total = t.counters.data() + 6;
+        u32 *distinct = t.counters.data() + 1; // distinct codes (positions where codes change)
+        u32 *L_dev    = t.counters.data() + 0;
+        u32 *K_tmp[2] = {t.counters.data() + 8, t.counters.data() + 9};
+        exclusive_scan_u32(s, t.split_a.data(), t.red_cnt.data(), n, distinct, t.scratch.data());
+        u32 *F_in = reduction_level > 0 ? t.red_pos.data() : t.reduc_map.data();
+        k_compact<<<g, 256, 0, s>>>(n, t.split_a.data(), t.red_cnt.data(), F_in);
+        const u32 *K_in = distinct;
+        if (reduction_level == 0)
+            SHAMX_HIP(hipMemcpyAsync(L_dev, distinct, sizeof(u32), hipMemcpyDeviceToDevice, s));
         for (u32 it = 1; it <= reduction_level; it++) {
-            exclusive_scan_u32(s, sin, t.red_cnt.data(), n, total, t.scratch.data());
-            if (it == 1) // distinct codes (the split flags mark code changes)
-                SHAMX_HIP(hipMemcpyAsync(t.counters.data() + 1, total, sizeof(u32),
-                                         hipMemcpyDeviceToDevice, s));
-            k_compact<<<g, 256, 0, s>>>(n, sin, t.red_cnt.data(), t.red_pos.data());
-            k_reduc_iter<<<g, 256, 0, s>>>(n, t.sorted.data(), sin, t.red_cnt.data(),
-                                          t.red_pos.data(), total, sout);
-            std::swap(sin, sout);
+            const bool last = it == reduction_level;
+            u32 *F_out = last ? t.reduc_map.data()
+                              : (F_in == t.red_pos.data() ? t.split_b.data() : t.red_pos.data());
+            u32 *K_out = last ? L_dev : K_tmp[it % 2];
+            k_reduc_keep<<<g, 256, 0, s>>>(n, t.sorted.data(), F_in, K_in, t.split_a.data());
+            exclusive_scan_u32_devn(s, t.split_a.data(), t.red_cnt.data(), n, K_in, K_out,
+                                    t.scratch.data());
+            k_compact_list<<<g, 256, 0, s>>>(K_in, t.split_a.data(), t.red_cnt.data(), F_in,
+                                             F_out);
+            F_in = F_out;
+            K_in = K_out;
         }
-        // sin holds the final split table; sout is free for the offsets
-        exclusive_scan_u32(s, sin, sout, n, t.counters.data(), t.scratch.data());
-        k_compact<<<g, 256, 0, s>>>(n, sin, sout, t.reduc_map.data());
         u32 hc[4];
         SHAMX_HIP(hipMemcpyAsync(hc, t.counters.data(), 4 * sizeof(u32), hipMemcpyDeviceToHost, s));
         SHAMX_HIP(hipStreamSynchronize(s));
