@@ -780,7 +780,7 @@ namespace shamx {
             }
         }
         void max_u32(hipStream_t s, u32 n, const u32 *v, u32 *out) {
-            if (n) k_max_u32<<<std::min<u32>(div_up(n, 256), 1024), 256, 0, s>>>(n, v, out);
+            if (n) k_max_u32<<<std::min<u32>(div_up(n, 256), 256), 256, 0, s>>>(n, v, out);
         }
 
         // the stage-2 wave items (one per 64 sorted positions of every leaf): item -> leaf map,

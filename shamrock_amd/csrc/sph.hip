@@ -1289,13 +1289,15 @@ namespace shamx {
             if (threadIdx.x == 0) part[blockIdx.x] = key;
         }
 
+        // (one same-address atomic per block: a few hundred blocks)
         __global__ void __launch_bounds__(BLK) k_max_key(u32 n, const f64 *__restrict v,
                                                          u64 *__restrict key) {
+            __shared__ u64 lds[BLK / WAVE];
             u64 k = ORDERED_NEG_INF;
             for (u32 i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK)
                 k = umax64(k, f64_to_ordered(v[i]));
-            k = wave_reduce(k, umax64);
-            if (__lane_id() == 0) atomic_max_key(key, k);
+            k = block_reduce(k, umax64, lds);
+            if (threadIdx.x == 0) atomic_max_key(key, k);
         }
 
         // ------------------------------------------------------------- periodic images
@@ -1746,7 +1748,7 @@ namespace shamx {
 
     void max_key_f64(hipStream_t s, u32 n, const f64 *v, u64 *key) {
         if (n == 0) return;
-        u32 nb = std::min<u32>(div_up(n, BLK), 1024);
+        u32 nb = std::min<u32>(div_up(n, BLK), 256);
         k_max_key<<<nb, BLK, 0, s>>>(n, v, key);
         SHAMX_HIP(hipGetLastError());
     }

@@ -653,21 +653,32 @@ namespace shamx {
         // depth of every internal node (root 0) by walking the parent links; max depth to
         // maxd, broken links or depth >= DEPTH_MAX to err
         constexpr u32 DEPTH_MAX = 1024;
-        __global__ void k_node_depth(u32 I, const u32 *__restrict parent, u16 *__restrict depth,
-                                     u32 *__restrict maxd, u32 *__restrict err) {
-            u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-            u32 d = 0;
-            if (i < I) {
-                u32 p = parent[i];
+        // (grid-stride over few blocks: the maximum depth leaves each block in one atomic)
+        __global__ void __launch_bounds__(256)
+            k_node_depth(u32 I, const u32 *__restrict parent, u16 *__restrict depth,
+                         u32 *__restrict maxd, u32 *__restrict err) {
+            __shared__ u32 lds[256 / WAVE];
+            u32 dmax = 0;
+            bool broken = false;
+            for (u32 i = blockIdx.x * 256 + threadIdx.x; i < I; i += gridDim.x * 256) {
+                u32 d = 0, p = parent[i];
                 while (p < I && d < DEPTH_MAX) {
                     d++;
                     p = parent[p];
                 }
-                if (p != NO_PARENT) atomicOr(err, 1u);
+                broken |= p != NO_PARENT;
                 depth[i] = (u16) min(d, DEPTH_MAX - 1);
+                dmax     = max(dmax, d);
             }
-            d = wave_reduce(d, [](u32 p, u32 q) { return p > q ? p : q; });
-            if (__lane_id() == 0) atomicMax(maxd, d);
+            if (broken) atomicOr(err, 1u);
+            dmax = wave_reduce(dmax, [](u32 p, u32 q) { return p > q ? p : q; });
+            if (__lane_id() == 0) lds[threadIdx.x / WAVE] = dmax;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                u32 m = 0;
+                for (int w = 0; w < 256 / WAVE; w++) m = max(m, lds[w]);
+                atomicMax(maxd, m);
+            }
         }
 
         // boxes and h max of the internal nodes at depth d from their two children (all of
@@ -944,8 +955,8 @@ namespace shamx {
             // internal boxes level by level, deepest first (the kernel boundary between two
             // levels orders the writes of one level before the reads of the next)
             u32 *maxd = t.counters.data() + 4, *err = t.counters.data() + 5;
-            k_node_depth<<<div_up(I, 256), 256, 0, s>>>(I, t.parent.data(), t.depth.data(), maxd,
-                                                       err);
+            k_node_depth<<<std::min<u32>(1024, div_up(I, 256)), 256, 0, s>>>(
+                I, t.parent.data(), t.depth.data(), maxd, err);
             u32 hd[2];
             SHAMX_HIP(hipMemcpyAsync(hd, maxd, 2 * sizeof(u32), hipMemcpyDeviceToHost, s));
             SHAMX_HIP(hipStreamSynchronize(s));
