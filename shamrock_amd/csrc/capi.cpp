@@ -152,6 +152,11 @@ struct shamx_solver {
     DevArray<u8> ghost_oid;
     DevArray<unsigned char> gscratch;
     DevArray<u64> red, part;
+    // host copies of the step's reductions and counts (pinned, so the reads of one decision
+    // queue on the stream and share one synchronisation)
+    PinnedWords<u64, 8> hostw;
+    PinnedWords<u32, 4> hostc;
+    PinnedWords<u64, 3 * 64> hosts; // the sinks' gas accelerations (up to 64 sinks)
     HLoopBuffers hloop;
     // storage order: the local particles are kept in the order of the last tree's sort (the
     // reference's ParticleReordering, Solver.cpp:2043-2048, here at every step), so that the
@@ -212,10 +217,9 @@ struct shamx_solver {
     }
 
     f64 read_key(int slot) {
-        u64 k;
-        SHAMX_HIP(hipMemcpyAsync(&k, red.data() + slot, 8, hipMemcpyDeviceToHost, stream));
+        hostw.read_async(stream, red.data() + slot, 1);
         SHAMX_HIP(hipStreamSynchronize(stream));
-        return ordered_to_f64(k);
+        return ordered_to_f64(hostw[0]);
     }
 
     // sorted-layout CSR: local particles in sorted order, records and rows by sorted position
@@ -243,7 +247,7 @@ struct shamx_solver {
         reo_scan.resize(std::max(reo_scan.size(), scan_scratch_bytes(Nm)));
         local_sorted_order(stream, Nm, N, tree.sort_index.data(), reo_perm.data(),
                            reo_flag.data(), reo_off.data(), reo_scan.data());
-        const u32 moved = perm_moved(stream, N, reo_perm.data(), reo_flag.data());
+        const u32 moved = perm_moved(stream, N, reo_perm.data(), reo_flag.data(), hostc.data());
         return (u64) moved * 32 >= N;
     }
 
@@ -420,12 +424,12 @@ struct shamx_solver {
                        cfg.box_max[0], cfg.box_max[1], cfg.box_max[2]};
         u32 g = periodic_ghosts(stream, N, x.data(), y.data(), z.data(), h.data(), box6,
                                 red.data(), cfg.htol_up_coarse, Rkern, ghost_src, ghost_oid,
-                                gwork, gscratch, cap - N);
+                                gwork, gscratch, cap - N, hostc.data());
         if (g > cap - N) {
             grow(N + g);
             g = periodic_ghosts(stream, N, x.data(), y.data(), z.data(), h.data(), box6,
                                 red.data(), cfg.htol_up_coarse, Rkern, ghost_src, ghost_oid,
-                                gwork, gscratch, cap - N);
+                                gwork, gscratch, cap - N, hostc.data());
         }
         G = g;
     }
@@ -523,8 +527,13 @@ struct shamx_solver {
                      k == 0, axe.data(), aye.data(), aze.data(),
                      sink_red.data() + 3 * (size_t) k, part);
         }
-        std::vector<u64> r(3 * (size_t) S);
-        SHAMX_HIP(hipMemcpyAsync(r.data(), sink_red.data(), r.size() * 8, hipMemcpyDeviceToHost,
+        std::vector<u64> big;
+        u64 *r = hosts.data();
+        if (3 * (size_t) S > hosts.size()) {
+            big.resize(3 * (size_t) S);
+            r = big.data();
+        }
+        SHAMX_HIP(hipMemcpyAsync(r, sink_red.data(), 3 * (size_t) S * 8, hipMemcpyDeviceToHost,
                                  stream));
         SHAMX_HIP(hipStreamSynchronize(stream));
         for (u32 k = 0; k < S; k++)
@@ -639,13 +648,14 @@ struct shamx_solver {
                       az.data(), ax_old.data(), ay_old.data(), az_old.data(), u.data(), du.data(),
                       du_old.data(), cfg.gpart_mass, red.data() + 4, part);
             timer.end(stream, PH_CORR, b);
-            u64 k[5];
-            SHAMX_HIP(hipMemcpyAsync(k, red.data() + 3, 40, hipMemcpyDeviceToHost, stream));
-            if (count_pending)
-                SHAMX_HIP(hipMemcpyAsync(&reo_moved, reo_count.data(), sizeof(u32),
-                                         hipMemcpyDeviceToHost, stream));
+            const u64 *k = hostw.data();
+            hostw.read_async(stream, red.data() + 3, 5);
+            if (count_pending) hostc.read_async(stream, reo_count.data(), 1, 2);
             SHAMX_HIP(hipStreamSynchronize(stream));
-            if (count_pending) reo_known = true;
+            if (count_pending) {
+                reo_moved = hostc[2];
+                reo_known = true;
+            }
             f64 dt_min         = ordered_to_f64(k[0]);
             last.n_interacting = k[4];
             f64 max_e2 = ordered_to_f64(k[1]);
@@ -756,8 +766,8 @@ struct shamx_solver {
                       ax_old.data(), ay_old.data(), az_old.data(), du_old.data(), vsig.data(),
                       dtc.data(), fa, red.data() + 3, red.data() + 7, part);
         timer.end(stream, PH_FORCE, b);
-        u64 k[5];
-        SHAMX_HIP(hipMemcpyAsync(k, red.data() + 3, 40, hipMemcpyDeviceToHost, stream));
+        const u64 *k = hostw.data();
+        hostw.read_async(stream, red.data() + 3, 5);
         SHAMX_HIP(hipStreamSynchronize(stream));
         auto t1 = std::chrono::steady_clock::now();
         f64 ph[PH_N] = {0};
@@ -1199,6 +1209,8 @@ int shamx_tree_build(shamx_ctx *ctx, void *stream, uint32_t n, const double *x, 
     return guarded([&] {
         need(ctx != nullptr, "shamx_tree_build: null context");
         tree_build((hipStream_t) stream, ctx->tree, n, x, y, z, h, reduction_level, htol);
+        SHAMX_HIP(hipStreamSynchronize((hipStream_t) stream));
+        tree_check(ctx->tree);
     });
 }
 
@@ -1218,6 +1230,8 @@ int shamx_tree_aabb_hmax(shamx_ctx *ctx, void *stream, const double *x, const do
     return guarded([&] {
         need(ctx && ctx->tree.L > 0, "shamx_tree_aabb_hmax: build the tree first");
         tree_stage_aabb((hipStream_t) stream, ctx->tree, x, y, z, h, htol);
+        SHAMX_HIP(hipStreamSynchronize((hipStream_t) stream));
+        tree_check(ctx->tree);
     });
 }
 

@@ -7,6 +7,7 @@
 
 #include "common.hpp"
 
+#include <stdexcept>
 #include <utility>
 
 namespace shamx {
@@ -66,6 +67,39 @@ namespace shamx {
         size_t size() const { return size_; }
         size_t capacity() const { return cap_; }
         size_t bytes() const { return size_ * sizeof(T); }
+    };
+
+    // N pinned host words for small device-to-host reads (sizes, counts, reductions).  A read
+    // into pageable memory is staged through a runtime buffer and blocks the host by itself;
+    // a read into pinned memory is queued on the stream like a kernel, so the reads of one
+    // decision share a single stream synchronisation.  Allocated on first use.
+    template<class T, size_t N>
+    class PinnedWords {
+        T *ptr_ = nullptr;
+
+        public:
+        PinnedWords() = default;
+        PinnedWords(const PinnedWords &)            = delete;
+        PinnedWords &operator=(const PinnedWords &) = delete;
+        PinnedWords(PinnedWords &&o) noexcept : ptr_(std::exchange(o.ptr_, nullptr)) {}
+        PinnedWords &operator=(PinnedWords &&o) noexcept {
+            std::swap(ptr_, o.ptr_);
+            return *this;
+        }
+        ~PinnedWords() {
+            if (ptr_) (void) hipHostFree(ptr_);
+        }
+        T *data() {
+            if (!ptr_) SHAMX_HIP(hipHostMalloc((void **) &ptr_, N * sizeof(T), hipHostMallocDefault));
+            return ptr_;
+        }
+        T &operator[](size_t i) { return data()[i]; }
+        static constexpr size_t size() { return N; }
+        // queue a read of n <= N words from device memory (synchronise the stream before use)
+        void read_async(hipStream_t s, const T *src, size_t n, size_t at = 0) {
+            if (at + n > N) throw std::logic_error("PinnedWords: read past the end");
+            if (n) SHAMX_HIP(hipMemcpyAsync(data() + at, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+        }
     };
 
 } // namespace shamx

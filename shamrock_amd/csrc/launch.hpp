@@ -65,6 +65,8 @@ namespace shamx {
         DevArray<u16> depth;           // I (root 0)
         DevArray<f64> nodes;           // (I + L) * NODE_F64 records (tree_layout.hpp)
         DevArray<u32> counters;        // small device scalars
+        PinnedWords<u32, 8> host;      // their host copies
+        bool err_pending = false;      // host[5] holds the last box pass's flags (tree_check)
         DevArray<unsigned char> scratch;
         // order runs of equal codes by the position inside the cell (set from the previous
         // build's distinct-code count when allow_fine_ties; the phase entries keep the plain
@@ -88,6 +90,10 @@ namespace shamx {
     void tree_stage_karras(hipStream_t s, TreeBuffers &t, u32 reduction_level);
     void tree_stage_aabb(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
                          const f64 *z, const f64 *h, f64 htol);
+    // throws if the last box pass flagged the tree (broken parent links, a child index out of
+    // range, a cell deeper than the merge levels); valid once the stream has been synchronised
+    // after that pass (the neighbour build and the next tree build do it)
+    void tree_check(TreeBuffers &t);
 
     // ------------------------------------------------------------------ neigh.hip
     struct NeighBuffers {
@@ -117,11 +123,12 @@ namespace shamx {
         bool leaf_overflowed = false; // the last build had leaf lists past their capacity
         u32 n_mismatch = 0;
         DevArray<u32> counters;
+        PinnedWords<u32, 16> host; // host copies of counters
         DevArray<unsigned char> scratch;
     };
     // NeighbourCache::start_neighbors_cache_2stages (NeighbourCache.cpp:222-604).
     // Synchronises twice to size the leaf list and the particle list.
-    void neigh_build(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
+    void neigh_build(hipStream_t s, NeighBuffers &nb, TreeBuffers &t, u32 n_local,
                      const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
                      f64 Rkern, bool sorted_layout);
 
@@ -156,6 +163,7 @@ namespace shamx {
     // eps values and sweep count equal those of one sweep per host read.
     struct HLoopBuffers {
         DevArray<u64> st, part;
+        PinnedWords<u64, 16> host; // host copy of st
         DevArray<u32> sel_a, sel_b, flag, off;
         DevArray<unsigned char> scan;
     };
@@ -249,12 +257,13 @@ namespace shamx {
                 const f64 *ay, const f64 *az, f64 C_cour, f64 C_force, f64 *dt_cfl,
                 u64 *dt_min_key, DevArray<u64> &part);
     // periodic image selection (BasicSPHGhosts.cpp:260-579): returns G (synchronises) and
-    // fills src ids + appends positions/h of the images after the n local entries.
+    // fills src ids + appends positions/h of the images after the n local entries (count_host:
+    // one pinned word for the count).
     // The interaction range is (hmax * f1) * f2 with hmax the ordered key at *hmax_key (device).
     u32 periodic_ghosts(hipStream_t s, u32 n, f64 *x, f64 *y, f64 *z, f64 *h, const f64 *box6,
                         const u64 *hmax_key, f64 f1, f64 f2, DevArray<u32> &ghost_src,
                         DevArray<u8> &ghost_off_id, DevArray<u32> &work,
-                        DevArray<unsigned char> &scratch, u32 cap);
+                        DevArray<unsigned char> &scratch, u32 cap, u32 *count_host);
     // ghost field refresh: dst[n + g] = src[ghost_src[g]] (+ offset for positions)
     struct FieldPtrs {
         f64 *p[8];
@@ -266,8 +275,9 @@ namespace shamx {
     void local_sorted_order(hipStream_t s, u32 nm, u32 n, const u32 *sort_index, u32 *perm,
                             u32 *flag, u32 *off, void *scan_scratch);
     // dst[j] = src[perm[j]] (gather) and dst[perm[j]] = src[j] (scatter), j < n
-    // entries of perm[0..n) that differ from their index (synchronises to return the count)
-    u32 perm_moved(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev);
+    // entries of perm[0..n) that differ from their index (synchronises to return the count;
+    // count_host: one pinned word)
+    u32 perm_moved(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev, u32 *count_host);
     // the same count left in *count_dev (no synchronisation)
     void perm_moved_async(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev);
     void perm_gather_f64(hipStream_t s, u32 n, const u32 *perm, const f64 *src, f64 *dst);

@@ -903,7 +903,7 @@ namespace shamx {
         u32 grow_cap(u32 need) { return (need + need / 8 + 16 + 15) & ~15u; }
     } // namespace
 
-    void neigh_build(hipStream_t s, NeighBuffers &nb, const TreeBuffers &t, u32 n_local,
+    void neigh_build(hipStream_t s, NeighBuffers &nb, TreeBuffers &t, u32 n_local,
                      const f64 *x, const f64 *y, const f64 *z, const f64 *h, f64 htol,
                      f64 Rkern, bool sorted_layout) {
         u32 L = t.L, I = t.I, nm = t.n_obj;
@@ -922,7 +922,7 @@ namespace shamx {
         u32 *err = nb.counters.data() + 4;
         fill_u32(s, nb.counters.data(), 0, 16);
         f64 Rker2 = Rkern * Rkern;
-        u32 hc[16];
+        u32 *hc = nb.host.data(); // 16 pinned words: the reads queue on the stream
 
         // sorted copy and inverse permutation (independent of the leaf lists)
         nb.ps.resize(nm);
@@ -940,6 +940,7 @@ namespace shamx {
             SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
                                      hipMemcpyDeviceToHost, s));
             SHAMX_HIP(hipStreamSynchronize(s));
+            tree_check(t);
             if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
             nb.n_leaf_pairs = hc[0];
             nb.leaf_stride  = 0;
@@ -1045,6 +1046,7 @@ namespace shamx {
                 SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 8 * sizeof(u32),
                                          hipMemcpyDeviceToHost, s));
                 SHAMX_HIP(hipStreamSynchronize(s));
+                tree_check(t);
                 if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
                 nb.n_leaf_pairs = hc[0];
                 nb.leaf_ids.resize(std::max<u64>(nb.n_leaf_pairs, 1));
@@ -1074,6 +1076,7 @@ namespace shamx {
                 SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
                                          hipMemcpyDeviceToHost, s));
                 SHAMX_HIP(hipStreamSynchronize(s));
+                tree_check(t);
                 if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
                 if (hc[12]) resolve_leaves(cl, hc[12], hc[13]);
             }
@@ -1084,6 +1087,7 @@ namespace shamx {
                     SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
                                              hipMemcpyDeviceToHost, s));
                     SHAMX_HIP(hipStreamSynchronize(s));
+                    tree_check(t);
                     if (hc[4])
                         throw std::runtime_error("neighbour cache: tree traversal stack overflow");
                     if (hc[12]) resolve_leaves(cl, hc[12], hc[13]);
@@ -1118,6 +1122,7 @@ namespace shamx {
             SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
                                      hipMemcpyDeviceToHost, s));
             SHAMX_HIP(hipStreamSynchronize(s));
+            tree_check(t);
             check_walk_flags(hc[4]);
             if (!resolve_first && hc[12]) {
                 // leaf lists overflowed for the first time: stage 2 read truncated lists

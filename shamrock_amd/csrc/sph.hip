@@ -1501,7 +1501,8 @@ namespace shamx {
         const u32 *sel = c.sel;
         u32 n_act      = n;
         u32 done = 0, batch = 1;
-        u64 host[HL_WORDS];
+        static_assert(HL_WORDS <= decltype(b.host)::size(), "h-loop words exceed the host copy");
+        u64 *host = b.host.data();
         while (done < max_iter) {
             u32 k = std::min(batch, max_iter - done);
             for (u32 i = 0; i < k; i++) {
@@ -1516,7 +1517,7 @@ namespace shamx {
                 k_h_loop_ctl<<<1, 64, 0, s>>>(b.st.data(), epsilon_h);
             }
             SHAMX_HIP(hipGetLastError());
-            SHAMX_HIP(hipMemcpyAsync(host, b.st.data(), sizeof(host), hipMemcpyDeviceToHost, s));
+            b.host.read_async(s, b.st.data(), HL_WORDS);
             SHAMX_HIP(hipStreamSynchronize(s));
             done += k;
             res.host_reads++;
@@ -1759,7 +1760,7 @@ namespace shamx {
     u32 periodic_ghosts(hipStream_t s, u32 n, f64 *x, f64 *y, f64 *z, f64 *h, const f64 *box6,
                         const u64 *hmax_key, f64 f1, f64 f2, DevArray<u32> &ghost_src,
                         DevArray<u8> &ghost_off_id, DevArray<u32> &work,
-                        DevArray<unsigned char> &scratch, u32 cap) {
+                        DevArray<unsigned char> &scratch, u32 cap, u32 *count_host) {
         Box6 box{{box6[0], box6[1], box6[2], box6[3], box6[4], box6[5]}};
         u32 nb = div_up(std::max<u32>(n, 1), BLK);
         // work: [cutboxes (as u32 words)] [26*nb counts] [26*nb bases] [total]
@@ -1774,9 +1775,9 @@ namespace shamx {
         k_cut_boxes<<<1, 32, 0, s>>>(box, hmax_key, f1, f2, cbd);
         k_ghost_count<<<nb, BLK, 0, s>>>(n, x, y, z, cbd, counts, nb);
         exclusive_scan_u32(s, counts, bases, 26 * nb, total, scratch.data());
-        u32 G = 0;
-        SHAMX_HIP(hipMemcpyAsync(&G, total, sizeof(u32), hipMemcpyDeviceToHost, s));
+        SHAMX_HIP(hipMemcpyAsync(count_host, total, sizeof(u32), hipMemcpyDeviceToHost, s));
         SHAMX_HIP(hipStreamSynchronize(s));
+        const u32 G = *count_host;
         if (G > cap) return G; // caller grows the merged arrays and calls again
         ghost_src.resize(std::max<u32>(G, 1));
         ghost_off_id.resize(std::max<u32>(G, 1));
@@ -1836,13 +1837,12 @@ namespace shamx {
         SHAMX_HIP(hipGetLastError());
     }
 
-    u32 perm_moved(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev) {
+    u32 perm_moved(hipStream_t s, u32 n, const u32 *perm, u32 *count_dev, u32 *count_host) {
         if (n == 0) return 0;
         perm_moved_async(s, n, perm, count_dev);
-        u32 c = 0;
-        SHAMX_HIP(hipMemcpyAsync(&c, count_dev, sizeof(u32), hipMemcpyDeviceToHost, s));
+        SHAMX_HIP(hipMemcpyAsync(count_host, count_dev, sizeof(u32), hipMemcpyDeviceToHost, s));
         SHAMX_HIP(hipStreamSynchronize(s));
-        return c;
+        return *count_host;
     }
 
     void local_sorted_order(hipStream_t s, u32 nm, u32 n, const u32 *sort_index, u32 *perm,

@@ -650,16 +650,20 @@ namespace shamx {
             }
         }
 
-        // depth of every internal node (root 0) by walking the parent links; max depth to
-        // maxd, broken links or depth >= DEPTH_MAX to err
+        // Internal cells are merged in bands of levels, deepest first, over MERGE_LEVELS levels
+        // with no host read of the actual depth: the leaf codes are distinct 32-bit codes (runs of
+        // equal codes are merged into one leaf), every child's range shares a strictly longer
+        // prefix than its parent's, so no internal cell lies deeper than 31.
+        constexpr u32 MERGE_LEVELS = 32;
+        // cycle guard of the parent walk
         constexpr u32 DEPTH_MAX = 1024;
-        // (grid-stride over few blocks: the maximum depth leaves each block in one atomic)
+
+        // depth of every internal node (root 0) by walking the parent links; err bit 0: broken
+        // links, bit 2: a cell at depth >= MERGE_LEVELS
         __global__ void __launch_bounds__(256)
             k_node_depth(u32 I, const u32 *__restrict parent, u16 *__restrict depth,
-                         u32 *__restrict maxd, u32 *__restrict err) {
-            __shared__ u32 lds[256 / WAVE];
-            u32 dmax = 0;
-            bool broken = false;
+                         u32 *__restrict err) {
+            bool broken = false, deep = false;
             for (u32 i = blockIdx.x * 256 + threadIdx.x; i < I; i += gridDim.x * 256) {
                 u32 d = 0, p = parent[i];
                 while (p < I && d < DEPTH_MAX) {
@@ -667,40 +671,74 @@ namespace shamx {
                     p = parent[p];
                 }
                 broken |= p != NO_PARENT;
+                deep |= d >= MERGE_LEVELS;
                 depth[i] = (u16) min(d, DEPTH_MAX - 1);
-                dmax     = max(dmax, d);
             }
-            if (broken) atomicOr(err, 1u);
-            dmax = wave_reduce(dmax, [](u32 p, u32 q) { return p > q ? p : q; });
-            if (__lane_id() == 0) lds[threadIdx.x / WAVE] = dmax;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                u32 m = 0;
-                for (int w = 0; w < 256 / WAVE; w++) m = max(m, lds[w]);
-                atomicMax(maxd, m);
+            const bool any_broken = __ballot(broken) != 0, any_deep = __ballot(deep) != 0;
+            if (__lane_id() == 0 && (any_broken || any_deep))
+                atomicOr(err, (any_broken ? 1u : 0u) | (any_deep ? 4u : 0u));
+        }
+
+        // box and h max of a cell: [0..2] min, [3..5] max, [6] max rint
+        struct Rec7 {
+            f64 v[7];
+        };
+        __device__ inline Rec7 rec7_load(const f64 *__restrict nodes, u32 c) {
+            const f64 *p = nodes + (u64) c * NODE_F64;
+            Rec7 r;
+#pragma unroll
+            for (int k = 0; k < 7; k++) r.v[k] = p[k];
+            return r;
+        }
+        __device__ inline Rec7 rec7_merge(const Rec7 &a, const Rec7 &b) {
+            Rec7 r;
+#pragma unroll
+            for (int k = 0; k < 3; k++) r.v[k] = fmin(a.v[k], b.v[k]);
+#pragma unroll
+            for (int k = 3; k < 7; k++) r.v[k] = fmax(a.v[k], b.v[k]);
+            return r;
+        }
+        // the record of cell c as merged from the complete cells rem levels below it (or from
+        // leaves above that): cells at depth >= the band's end were merged by an earlier launch.
+        // Only the children links of the cells on the way are read; this kernel never writes
+        // them.  R bounds the recursion at compile time (rem <= R).
+        template<int R>
+        __device__ inline Rec7 band_subtree(const f64 *__restrict nodes, u32 I, u32 L, u32 c,
+                                            u32 rem, bool &bad) {
+            if constexpr (R == 0) {
+                return rec7_load(nodes, c);
+            } else {
+                if (c >= I || rem == 0) return rec7_load(nodes, c);
+                u32 lt, rt;
+                node_children(nodes, c, lt, rt);
+                if (lt >= I + L || rt >= I + L) {
+                    bad = true;
+                    return rec7_load(nodes, c);
+                }
+                return rec7_merge(band_subtree<R - 1>(nodes, I, L, lt, rem - 1, bad),
+                                  band_subtree<R - 1>(nodes, I, L, rt, rem - 1, bad));
             }
         }
 
-        // boxes and h max of the internal nodes at depth d from their two children (all of
-        // them complete: leaves, or internal nodes of depth d + 1 merged by the previous launch).
-        // min / max are exact, so this equals any other merge order bit for bit.
-        __global__ void k_merge_level(u32 I, u32 L, u32 d, const u16 *__restrict depth,
-                                      f64 *__restrict nodes, u32 *__restrict err) {
+        // boxes and h max of the internal cells of depth [b0, b0 + BAND), each from its
+        // descendants of depth b0 + BAND (merged by the previous launch) and the leaves above
+        // them.  min / max are exact, so this equals any other merge order bit for bit.
+        constexpr u32 BAND = 4;
+        __global__ void k_merge_band(u32 I, u32 L, u32 b0, const u16 *__restrict depth,
+                                     f64 *__restrict nodes, u32 *__restrict err) {
             u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-            if (i >= I || depth[i] != d) return;
-            u32 lt, rt;
-            node_children(nodes, i, lt, rt);
-            if (lt >= I + L || rt >= I + L) {
+            if (i >= I) return;
+            const u32 d = depth[i];
+            if (d < b0 || d >= b0 + BAND) return;
+            bool bad        = false;
+            const Rec7 m    = band_subtree<BAND>(nodes, I, L, i, b0 + BAND - d, bad);
+            if (bad) {
                 atomicOr(err, 2u);
                 return;
             }
-            const f64 *A = nodes + (u64) lt * NODE_F64;
-            const f64 *B = nodes + (u64) rt * NODE_F64;
-            f64 *P       = nodes + (u64) i * NODE_F64;
+            f64 *P = nodes + (u64) i * NODE_F64;
 #pragma unroll
-            for (int k = 0; k < 3; k++) P[k] = fmin(A[k], B[k]);
-#pragma unroll
-            for (int k = 3; k < 7; k++) P[k] = fmax(A[k], B[k]);
+            for (int k = 0; k < 7; k++) P[k] = m.v[k];
         }
     } // namespace
 
@@ -903,9 +941,10 @@ namespace shamx {
             F_in = F_out;
             K_in = K_out;
         }
-        u32 hc[4];
-        SHAMX_HIP(hipMemcpyAsync(hc, t.counters.data(), 4 * sizeof(u32), hipMemcpyDeviceToHost, s));
+        u32 *hc = t.host.data();
+        t.host.read_async(s, t.counters.data(), 4);
         SHAMX_HIP(hipStreamSynchronize(s));
+        tree_check(t); // the previous build's, if nothing has read it since
         if (hc[2] | hc[3]) throw std::runtime_error("radix sort: look-back or scatter inconsistency");
         // the next builds order runs of equal codes spatially when codes repeat on average
         // more than 1.5 times (clustered sets whose cells hold many particles); unique codes
@@ -952,20 +991,31 @@ namespace shamx {
             I, t.reduc_map.data(), t.sort_index.data(), x, y, z, h, htol, t.nodes.data(),
             t.big_leaves.data(), n_big);
         if (I > 0) {
-            // internal boxes level by level, deepest first (the kernel boundary between two
-            // levels orders the writes of one level before the reads of the next)
-            u32 *maxd = t.counters.data() + 4, *err = t.counters.data() + 5;
+            // internal boxes band by band, deepest first (the kernel boundary between two
+            // bands orders the writes of one band before the reads of the next)
+            u32 *err = t.counters.data() + 5;
             k_node_depth<<<std::min<u32>(1024, div_up(I, 256)), 256, 0, s>>>(
-                I, t.parent.data(), t.depth.data(), maxd, err);
-            u32 hd[2];
-            SHAMX_HIP(hipMemcpyAsync(hd, maxd, 2 * sizeof(u32), hipMemcpyDeviceToHost, s));
-            SHAMX_HIP(hipStreamSynchronize(s));
-            if (hd[1]) throw std::runtime_error("tree build: broken parent links");
-            for (u32 d = hd[0] + 1; d-- > 0;)
-                k_merge_level<<<div_up(I, 256), 256, 0, s>>>(I, L, d, t.depth.data(),
-                                                            t.nodes.data(), err);
+                I, t.parent.data(), t.depth.data(), err);
+            static_assert(MERGE_LEVELS % BAND == 0, "bands tile the merge levels");
+            for (u32 b0 = MERGE_LEVELS; b0 > 0;) {
+                b0 -= BAND;
+                k_merge_band<<<div_up(I, 256), 256, 0, s>>>(I, L, b0, t.depth.data(),
+                                                           t.nodes.data(), err);
+            }
         }
+        // the flags reach the host with the next synchronisation of the stream (tree_check)
+        t.host.read_async(s, t.counters.data() + 5, 1, 5);
+        t.err_pending = true;
         SHAMX_HIP(hipGetLastError());
+    }
+
+    void tree_check(TreeBuffers &t) {
+        if (!t.err_pending) return;
+        t.err_pending = false;
+        const u32 e   = t.host[5];
+        if (e & 1u) throw std::runtime_error("tree build: broken parent links");
+        if (e & 2u) throw std::runtime_error("tree build: child index out of range");
+        if (e & 4u) throw std::runtime_error("tree build: internal cell deeper than 31");
     }
 
     void tree_stage_finish(hipStream_t s, TreeBuffers &t, const f64 *x, const f64 *y,
