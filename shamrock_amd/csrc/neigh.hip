@@ -26,11 +26,13 @@ namespace shamx {
         __device__ inline Box node_box(const f64 *__restrict rec) {
             return Box{{rec[0], rec[1], rec[2]}, {rec[3], rec[4], rec[5]}};
         }
-        // BBAA::cella_neigh_b<f64_3> (geometry_utils.hpp:125-135)
+        // BBAA::cella_neigh_b<f64_3> (geometry_utils.hpp:125-135).  The three comparisons are
+        // combined with & rather than &&: the same value (they have no side effects), without
+        // the branch and exec-mask updates the short-circuit form compiles to in a walk loop.
         __device__ inline bool cella_neigh_b(const f64 *alo, const f64 *ahi, const f64 *blo,
                                              const f64 *bhi) {
-            return (fmax(alo[0], blo[0]) <= fmin(ahi[0], bhi[0])) &&
-                   (fmax(alo[1], blo[1]) <= fmin(ahi[1], bhi[1])) &&
+            return (fmax(alo[0], blo[0]) <= fmin(ahi[0], bhi[0])) &
+                   (fmax(alo[1], blo[1]) <= fmin(ahi[1], bhi[1])) &
                    (fmax(alo[2], blo[2]) <= fmin(ahi[2], bhi[2]));
         }
 
@@ -167,7 +169,7 @@ namespace shamx {
                     f64 nlo[3] = {rec[0], rec[1], rec[2]}, nhi[3] = {rec[3], rec[4], rec[5]};
                     f64 xlo[3] = {nlo[0] - r, nlo[1] - r, nlo[2] - r};
                     f64 xhi[3] = {nhi[0] + r, nhi[1] + r, nhi[2] + r};
-                    return cella_neigh_b(alo, ahi, xlo, xhi) || cella_neigh_b(elo, ehi, nlo, nhi);
+                    return cella_neigh_b(alo, ahi, xlo, xhi) | cella_neigh_b(elo, ehi, nlo, nhi);
                 },
                 [&](u32 node) {
                     if (FILL && (!STRIDE || c < ce)) ids[c] = node;
@@ -209,7 +211,7 @@ namespace shamx {
                 f64 nlo[3] = {rec[0], rec[1], rec[2]}, nhi[3] = {rec[3], rec[4], rec[5]};
                 f64 xlo[3] = {nlo[0] - r, nlo[1] - r, nlo[2] - r};
                 f64 xhi[3] = {nhi[0] + r, nhi[1] + r, nhi[2] + r};
-                return cella_neigh_b(alo, ahi, xlo, xhi) || cella_neigh_b(elo, ehi, nlo, nhi);
+                return cella_neigh_b(alo, ahi, xlo, xhi) | cella_neigh_b(elo, ehi, nlo, nhi);
             };
             const u32 ncell = I + L;
             constexpr u32 ACC = 0x80000000u; // frontier entry: a leaf already accepted
@@ -349,8 +351,8 @@ namespace shamx {
             bool ok   = wave_traverse(
                 nodes, I, I + L, live,
                 [&](const f64 *rec) { // is_coord_in_range_incl_max (geometry_utils.hpp:74-81)
-                    return (rec[0] <= p[0]) && (p[0] <= rec[3]) && (rec[1] <= p[1]) &&
-                           (p[1] <= rec[4]) && (rec[2] <= p[2]) && (p[2] <= rec[5]);
+                    return (rec[0] <= p[0]) & (p[0] <= rec[3]) & (rec[1] <= p[1]) &
+                           (p[1] <= rec[4]) & (rec[2] <= p[2]) & (p[2] <= rec[5]);
                 },
                 [&](u32 node) { found = node - I; });
             if (!ok && __lane_id() == __builtin_ctzll(live)) atomicOr(err, 1u);
@@ -575,7 +577,7 @@ namespace shamx {
                         f64 ey = q.y < bly ? bly - q.y : (q.y > bhy ? q.y - bhy : 0.0);
                         f64 ez = q.z < blz ? blz - q.z : (q.z > bhz ? q.z - bhz : 0.0);
                         f64 d2 = (ex * ex + ey * ey + ez * ez) * (1.0 - 1e-12);
-                        const bool near = valid && !(d2 > br2 && d2 > rb2);
+                        const bool near = valid & !((d2 > br2) & (d2 > rb2));
                         const u64 nm    = __ballot(near);
                         const u32 pos   = (u32) __popcll(nm & lt);
                         cn              = (u32) __popcll(nm);
@@ -593,7 +595,7 @@ namespace shamx {
                             const double4 q = cpos[wid][cl];
                             f64 dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
                             f64 rab2 = dx * dx + dy * dy + dz * dz;
-                            keep     = !(rab2 > me.w && rab2 > q.w);
+                            keep     = !((rab2 > me.w) & (rab2 > q.w));
                         }
                         const u64 km = __ballot(keep);
                         const u64 kt = km & mine;

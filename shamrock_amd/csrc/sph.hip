@@ -442,7 +442,7 @@ namespace shamx {
                             else th = mass * ka.template dhW<false>(rab);
                         }
                     }
-                    if constexpr (FILTER) keep = !(rab2 > ca && rab2 > q.w * q.w * Rker2);
+                    if constexpr (FILTER) keep = !((rab2 > ca) & (rab2 > q.w * q.w * Rker2));
                 };
                 // fold the G terms of one pass, then append the kept ids in order
                 auto emit = [&](f64 th, bool keep, u32 id) {
@@ -749,7 +749,7 @@ namespace shamx {
                     if (k >= k1) return;
                     f64 dx = xa - B.x, dy = ya - B.y, dz = za - B.z;
                     f64 rab2 = dx * dx + dy * dy + dz * dz;
-                    if (rab2 > ca && rab2 > rec_hR2<K>(B)) return;
+                    if ((rab2 > ca) & (rab2 > rec_hR2<K>(B))) return;
                     npair++;
                     f64 rab = sqrt(rab2);
                     // dW = norm f'(r / h) / h^4 (sphkernels.hpp:2329-2343) and
@@ -881,7 +881,7 @@ namespace shamx {
                     f64 rab2 = dx * dx + dy * dy + dz * dz;
                     // neighbour-side criterion in FP32 (the solver's rows are prefiltered in
                     // FP64 already; this only matters for unfiltered reference-layout rows)
-                    if (rab2 > ca && (f32) rab2 * (B.inv_h * B.inv_h) > (f32) Rker2) return;
+                    if ((rab2 > ca) & ((f32) rab2 * (B.inv_h * B.inv_h) > (f32) Rker2)) return;
                     npair++;
                     f32 rab  = sqrt_approx((f32) rab2);
                     f32 inv  = (rab >= 1e-9f) ? rcp_approx(rab) : 0.f;
@@ -1335,9 +1335,11 @@ namespace shamx {
         }
 
         __device__ inline bool in_cut(const CutBoxes &cb, int o, f64 px, f64 py, f64 pz) {
-            return cb.active[o] && (cb.lo[o][0] <= px) && (px < cb.hi[o][0]) &&
-                   (cb.lo[o][1] <= py) && (py < cb.hi[o][1]) && (cb.lo[o][2] <= pz) &&
-                   (pz < cb.hi[o][2]);
+            // cb.active[o] is wave-uniform (its branch skips empty offsets); the coordinate
+            // tests are combined without short-circuit branches
+            return cb.active[o] && ((cb.lo[o][0] <= px) & (px < cb.hi[o][0]) &
+                                    (cb.lo[o][1] <= py) & (py < cb.hi[o][1]) &
+                                    (cb.lo[o][2] <= pz) & (pz < cb.hi[o][2]));
         }
 
         __global__ void __launch_bounds__(BLK)
