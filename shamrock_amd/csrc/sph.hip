@@ -713,11 +713,20 @@ namespace shamx {
             k_force(u32 n, Rows r, Src src, ForceOut o, ForceArgs fa, u64 *__restrict part) {
             using KK = Kern<K, f64>;
             __shared__ u64 lds[2][BLK / WAVE];
+            // G = 4: the four lanes of a particle share the ordered folds out through LDS, lane
+            // c folding component c (x, y, z, du) over the group's terms in lane order; slot 5
+            // holds +0.0, added by lanes 0-2 where lane 3 adds du's second term (a sum that
+            // starts at +0.0 never becomes -0.0 under round-to-nearest, so adding +0.0 leaves it
+            // unchanged).  Other G: every lane folds all four and the first lane's sums count.
+            constexpr bool SHARE = G == 4;
+            __shared__ f64 fold[SHARE ? BLK : 1][6];
             const u32 j    = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / G;
             const u32 lg   = threadIdx.x % G;
             const bool act = j < n;
             u64 npair = 0;
             f64 fx = 0, fy = 0, fz = 0, tdu = 0, vsig_max = 0;
+            f64 facc = 0; // SHARE: this lane's component
+            if constexpr (SHARE) fold[threadIdx.x][5] = 0.;
             u32 row = 0;
             FRec A{};
             if (act) {
@@ -804,10 +813,28 @@ namespace shamx {
                 auto step = [&](const FRec &B, u32 k) {
                     f64 tx = 0, ty = 0, tz = 0, tu1 = 0, tu2 = 0;
                     pair(B, k, tx, ty, tz, tu1, tu2);
-                    fx  = ordered_fold<G>(fx, tx);
-                    fy  = ordered_fold<G>(fy, ty);
-                    fz  = ordered_fold<G>(fz, tz);
-                    tdu = ordered_fold2<G>(tdu, tu1, tu2);
+                    if constexpr (SHARE) {
+                        f64 *m = fold[threadIdx.x];
+                        m[0]   = tx;
+                        m[1]   = ty;
+                        m[2]   = tz;
+                        m[3]   = tu1;
+                        m[4]   = tu2;
+                        wave_lds_sync();
+                        const f64(*grp)[6] = fold + (threadIdx.x & ~3u);
+                        const u32 sa = lg, sb = lg == 3 ? 4u : 5u;
+#pragma unroll
+                        for (int L = 0; L < 4; L++) {
+                            facc = facc + grp[L][sa];
+                            facc = facc + grp[L][sb];
+                        }
+                        wave_lds_sync();
+                    } else {
+                        fx  = ordered_fold<G>(fx, tx);
+                        fy  = ordered_fold<G>(fy, ty);
+                        fz  = ordered_fold<G>(fz, tz);
+                        tdu = ordered_fold2<G>(tdu, tu1, tu2);
+                    }
                 };
                 // two records in flight, used in turn: the next neighbour's record is loaded
                 // while the current one is used (3.21 -> 2.98 ms per evaluation at Sedov
@@ -829,6 +856,12 @@ namespace shamx {
                 };
                 if (A.rh != 0) sweep(std::true_type{});
                 else sweep(std::false_type{});
+            }
+            if constexpr (SHARE) {
+                fx  = facc;
+                fy  = group_lane<G, 1>(facc);
+                fz  = group_lane<G, 2>(facc);
+                tdu = group_lane<G, 3>(facc);
             }
             vsig_max = group_max<G>(vsig_max);
             u64 key  = ORDERED_POS_INF;
