@@ -844,12 +844,17 @@ namespace shamx {
                 // a cached line), so no record has to be kept across a skipped load.
                 if (k0 < k1) {
                     const u32 last = k1 - 1;
+                    // the id of the next B1 is loaded an iteration ahead, so its record load
+                    // does not wait for an id load first
+                    u32 id1 = r.ids[min(k0 + lg + G, last)];
                     FRec B0 = src.get(r.ids[min(k0 + lg, last)]);
                     for (u32 base = k0; base < k1; base += 2 * G) {
                         u32 k          = base + lg;
-                        const FRec B1  = src.get(r.ids[min(k + G, last)]);
+                        const FRec B1  = src.get(id1);
+                        const u32 id0  = r.ids[min(k + 2 * G, last)];
+                        id1            = r.ids[min(k + 3 * G, last)];
                         step(B0, k);
-                        B0 = src.get(r.ids[min(k + 2 * G, last)]);
+                        B0 = src.get(id0);
                         if (base + G < k1) step(B1, k + G);
                     }
                 }
