@@ -260,14 +260,18 @@ namespace shamx {
                     // group-uniform trip count; terms folded in list order
                     auto sweep = [&](auto fast) {
                         constexpr bool FAST = decltype(fast)::value;
-                        // the next neighbour's position is loaded while this one is used
+                        // the next neighbour's position is loaded while this one is used,
+                        // its id a pass earlier still
                         double3 qn{0, 0, 0};
+                        u32 idn = 0;
                         if (k0 + lg < k1) qn = src.pos(r.ids[k0 + lg]);
+                        if (k0 + lg + G < k1) idn = r.ids[k0 + lg + G];
                         for (u32 base = k0; base < k1; base += G) {
                             u32 k  = base + lg;
                             f64 tw = 0, th = 0;
                             const double3 qc = qn;
-                            if (k + G < k1) qn = src.pos(r.ids[k + G]);
+                            if (k + G < k1) qn = src.pos(idn);
+                            if (k + 2 * G < k1) idn = r.ids[k + 2 * G];
                             if (k < k1) {
                                 double3 q = qc;
                                 f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
@@ -953,12 +957,15 @@ namespace shamx {
                 // loads past the row end read its last entry again (a cached line)
                 if (k0 < k1) {
                     const u32 last = k1 - 1;
+                    u32 id1   = r.ids[min(k0 + lg + G, last)]; // an iteration ahead
                     FRec32 B0 = src.get(r.ids[min(k0 + lg, last)]);
                     for (u32 base = k0; base < k1; base += 2 * G) {
                         u32 k           = base + lg;
-                        const FRec32 B1 = src.get(r.ids[min(k + G, last)]);
+                        const FRec32 B1 = src.get(id1);
+                        const u32 id0   = r.ids[min(k + 2 * G, last)];
+                        id1             = r.ids[min(k + 3 * G, last)];
                         if (k < k1) pair(B0);
-                        B0 = src.get(r.ids[min(k + 2 * G, last)]);
+                        B0 = src.get(id0);
                         if (k + G < k1) pair(B1);
                     }
                 }
