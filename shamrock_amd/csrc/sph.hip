@@ -244,8 +244,16 @@ namespace shamx {
                     f32 ih     = (f32) (1. / h_a);
                     f32 ih3    = ih * ih * ih, ih4 = ih3 * ih;
                     f32 rs = 0, rc = 0, ds = 0, dc = 0;
+                    // the next entry's position is loaded while this one is used, its id a
+                    // pass earlier still (each lane sums its own entries in order)
+                    double3 qn{0, 0, 0};
+                    u32 idn = 0;
+                    if (k0 + lg < k1) qn = src.pos(r.ids[k0 + lg]);
+                    if (k0 + lg + G < k1) idn = r.ids[k0 + lg + G];
                     for (u32 k = k0 + lg; k < k1; k += G) {
-                        double3 q = src.pos(r.ids[k]);
+                        const double3 q = qn;
+                        if (k + G < k1) qn = src.pos(idn);
+                        if (k + 2 * G < k1) idn = r.ids[k + 2 * G];
                         f64 dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
                         f64 rab2 = dx * dx + dy * dy + dz * dz;
                         if (rab2 > dint) continue;
