@@ -490,11 +490,10 @@ namespace shamx {
                         }
                     }
                 };
-                // two entries per lane per pass (k and k + G), both loaded before either is used
-                for (u32 base = k0; base < k1; base += 2 * G) {
-                    u32 k = base + lg, k2 = k + G;
-                    bool v1 = k < k1, v2 = k2 < k1;
-                    u32 i1 = r.ids[v1 ? k : k0], i2 = r.ids[v2 ? k2 : k0];
+                // four entries per lane per pass (k, k + G, k + 2G, k + 3G), all loaded before
+                // any is used, folded in that (list) order
+                constexpr int U = 4;
+                for (u32 base = k0; base < k1; base += U * G) {
                     auto ld = [&](u32 id) -> double4 { // h only needed by the filter
                         if constexpr (FILTER) return src.pos4(id);
                         else {
@@ -502,13 +501,24 @@ namespace shamx {
                             return make_double4(q.x, q.y, q.z, 0.);
                         }
                     };
-                    double4 q1 = ld(i1), q2 = ld(i2);
-                    f64 th1 = 0, th2 = 0;
-                    bool keep1 = false, keep2 = false;
-                    entry(v1, i1, q1, th1, keep1);
-                    entry(v2, i2, q2, th2, keep2);
-                    emit(th1, keep1, i1);
-                    emit(th2, keep2, i2);
+                    bool v[U];
+                    u32 iu[U];
+                    double4 q[U];
+#pragma unroll
+                    for (int t = 0; t < U; t++) {
+                        const u32 k = base + lg + (u32) t * G;
+                        v[t]        = k < k1;
+                        iu[t]       = r.ids[v[t] ? k : k0];
+                    }
+#pragma unroll
+                    for (int t = 0; t < U; t++) q[t] = ld(iu[t]);
+#pragma unroll
+                    for (int t = 0; t < U; t++) {
+                        f64 th    = 0;
+                        bool keep = false;
+                        entry(v[t], iu[t], q[t], th, keep);
+                        emit(th, keep, iu[t]);
+                    }
                 }
                 if constexpr (FILTER) { // the last partial run
                     wave_lds_sync();
