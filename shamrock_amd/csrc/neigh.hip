@@ -180,25 +180,32 @@ namespace shamx {
             if (STRIDE && valid) offset[g] = c0;
         }
 
-        // One listed leaf per wave, its walk spread over the lanes: the tree is expanded breadth
-        // first into an ordered frontier of at least 64 subtrees (or of all its accepted
-        // leaves), then each lane walks a contiguous run of frontier entries depth first, left
-        // child first, with its own stack.  The frontier stays in left-to-right order and every
-        // lane walk is left-first, so the lanes' outputs in lane order are the leaf's list in
-        // the order of the reference's walk.  Each lane counts first; the wave's total must
-        // equal cnt[g] (the count of the fixed-capacity pass) before anything is written at
-        // offset[g] (err bit 4 otherwise).
-        constexpr int WIDE_FR = 256, WIDE_ST = 48;
-        __global__ void __launch_bounds__(WAVE)
+        // One listed leaf per block of WIDE_W waves, its walk spread over the block's threads:
+        // the first wave expands the tree breadth first into an ordered frontier of at least
+        // one subtree per thread (or of all its accepted leaves), then each thread walks a
+        // contiguous run of frontier entries depth first, left child first, with its own
+        // stack.  The frontier stays in left-to-right order and every thread's walk is
+        // left-first, so the threads' outputs in thread order are the leaf's list in the order
+        // of the reference's walk.  Each thread counts first; the block's total must equal
+        // cnt[g] (the count of the fixed-capacity pass) before anything is written at
+        // offset[g] (err bit 4 otherwise).  The listed leaves are the ones with the longest
+        // lists, and the kernel lasts as long as its longest walk: a block per leaf spreads
+        // that walk over four times the lanes of one wave.
+        constexpr int WIDE_W = 4, WIDE_T = WIDE_W * WAVE, WIDE_FR = 4 * WIDE_T, WIDE_ST = 40;
+        __global__ void __launch_bounds__(WIDE_T)
             k_leaf_walk_wide(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
                              const u32 *__restrict list, u32 n_list, const u32 *__restrict cnt,
                              const u32 *__restrict offset, u32 *__restrict ids,
                              u32 *__restrict err) {
             __shared__ u32 fr[2][WIDE_FR];
-            __shared__ u32 stk[WIDE_ST][WAVE]; // [depth][lane]
+            __shared__ u32 stk[WIDE_ST][WIDE_T]; // [depth][thread]
+            __shared__ u32 wsum[WIDE_W];
+            __shared__ u32 sh_F, sh_cur;
+            const u32 tid  = threadIdx.x;
             const u32 lane = (u32) __lane_id();
+            const u32 wv   = tid / WAVE;
             const u32 j    = blockIdx.x;
-            if (j >= n_list) return;
+            if (j >= n_list) return; // block-uniform
             const u32 g     = list[j];
             const f64 *ra   = nodes + (u64) (I + g) * NODE_F64;
             const f64 rint_a = ra[6] * Rkern;
@@ -215,61 +222,70 @@ namespace shamx {
             };
             const u32 ncell = I + L;
             constexpr u32 ACC = 0x80000000u; // frontier entry: a leaf already accepted
-            // breadth-first expansion (wave-uniform control)
-            u32 F = 1, cur = 0;
-            if (lane == 0) fr[0][0] = 0; // the root
-            wave_lds_sync();
             bool bad = false;
-            for (int round = 0;; round++) {
-                if (round == 128) { // deeper than any tree of u64 codes: a broken tree
-                    bad = true;
-                    break;
-                }
-                u32 total = 0;
-                bool internal = false, full = false;
-                for (u32 b = 0; b < F; b += WAVE) {
-                    const u32 e = b + lane;
-                    u32 no = 0, o0 = 0, o1 = 0;
-                    if (e < F) {
-                        const u32 node = fr[cur][e];
-                        if (node & ACC) {
-                            no = 1;
-                            o0 = node;
-                        } else if (node < ncell && crit(node)) {
-                            if (node >= I) {
-                                no = 1;
-                                o0 = node | ACC;
-                            } else {
-                                node_children(nodes, node, o0, o1);
-                                no = (o0 < ncell && o1 < ncell) ? 2u : 0u;
-                                bad |= no == 0;
-                            }
-                        }
-                    }
-                    u32 inc = no;
-                    for (int o = 1; o < WAVE; o <<= 1) {
-                        u32 t = __shfl_up(inc, o, WAVE);
-                        if ((int) lane >= o) inc += t;
-                    }
-                    const u32 ct = __shfl(inc, WAVE - 1, WAVE);
-                    if (total + ct > WIDE_FR) {
-                        full = true;
+            if (wv == 0) {
+                // breadth-first expansion (wave-uniform control)
+                u32 F = 1, cur = 0;
+                if (lane == 0) fr[0][0] = 0; // the root
+                wave_lds_sync();
+                for (int round = 0;; round++) {
+                    if (round == 128) { // deeper than any tree of these codes: a broken tree
+                        bad = true;
                         break;
                     }
-                    const u32 pos = total + inc - no;
-                    if (no >= 1) fr[cur ^ 1][pos] = o0;
-                    if (no == 2) fr[cur ^ 1][pos + 1] = o1;
-                    internal |= __ballot(no == 2) != 0;
-                    total += ct;
+                    u32 total = 0;
+                    bool internal = false, full = false;
+                    for (u32 b = 0; b < F; b += WAVE) {
+                        const u32 e = b + lane;
+                        u32 no = 0, o0 = 0, o1 = 0;
+                        if (e < F) {
+                            const u32 node = fr[cur][e];
+                            if (node & ACC) {
+                                no = 1;
+                                o0 = node;
+                            } else if (node < ncell && crit(node)) {
+                                if (node >= I) {
+                                    no = 1;
+                                    o0 = node | ACC;
+                                } else {
+                                    node_children(nodes, node, o0, o1);
+                                    no = (o0 < ncell && o1 < ncell) ? 2u : 0u;
+                                    bad |= no == 0;
+                                }
+                            }
+                        }
+                        u32 inc = no;
+                        for (int o = 1; o < WAVE; o <<= 1) {
+                            u32 t = __shfl_up(inc, o, WAVE);
+                            if ((int) lane >= o) inc += t;
+                        }
+                        const u32 ct = __shfl(inc, WAVE - 1, WAVE);
+                        if (total + ct > WIDE_FR) {
+                            full = true;
+                            break;
+                        }
+                        const u32 pos = total + inc - no;
+                        if (no >= 1) fr[cur ^ 1][pos] = o0;
+                        if (no == 2) fr[cur ^ 1][pos + 1] = o1;
+                        internal |= __ballot(no == 2) != 0;
+                        total += ct;
+                    }
+                    wave_lds_sync();
+                    if (full) break; // keep the current frontier
+                    cur ^= 1;
+                    F = total;
+                    if (!internal || F >= WIDE_T) break;
                 }
-                wave_lds_sync();
-                if (full) break; // keep the current frontier
-                cur ^= 1;
-                F = total;
-                if (!internal || F >= WAVE) break;
+                if (lane == 0) {
+                    sh_F   = F;
+                    sh_cur = cur;
+                }
             }
-            // lane k: entries [k F / 64, (k + 1) F / 64) in order, each walked depth first
-            const u32 b0 = (lane * F) / WAVE, b1 = ((lane + 1) * F) / WAVE;
+            __syncthreads();
+            const u32 F = sh_F, cur = sh_cur;
+            // thread t: entries [t F / WIDE_T, (t + 1) F / WIDE_T) in order, each walked depth
+            // first
+            const u32 b0 = (tid * F) / WIDE_T, b1 = ((tid + 1) * F) / WIDE_T;
             auto walk = [&](auto emit) {
                 for (u32 e = b0; e < b1; e++) {
                     const u32 node = fr[cur][e];
@@ -277,10 +293,10 @@ namespace shamx {
                         emit(node & ~ACC);
                         continue;
                     }
-                    int sp               = 0;
-                    stk[sp++][lane]      = node;
+                    int sp            = 0;
+                    stk[sp++][tid]    = node;
                     while (sp > 0) {
-                        const u32 nd = stk[--sp][lane];
+                        const u32 nd = stk[--sp][tid];
                         if (nd >= ncell || !crit(nd)) continue;
                         if (nd >= I) {
                             emit(nd);
@@ -292,25 +308,31 @@ namespace shamx {
                             bad = true;
                             break;
                         }
-                        stk[sp++][lane] = cr; // the left child is popped first
-                        stk[sp++][lane] = cl;
+                        stk[sp++][tid] = cr; // the left child is popped first
+                        stk[sp++][tid] = cl;
                     }
                 }
             };
             u32 c = 0;
             walk([&](u32) { c++; });
+            // block-wide exclusive scan of the counts
             u32 inc = c;
             for (int o = 1; o < WAVE; o <<= 1) {
                 u32 t = __shfl_up(inc, o, WAVE);
                 if ((int) lane >= o) inc += t;
             }
-            const u32 tot = __shfl(inc, WAVE - 1, WAVE);
-            const bool wave_bad = __ballot(bad) != 0;
-            if (wave_bad || tot != cnt[g]) {
-                if (lane == 0) atomicOr(err, wave_bad ? 1u : 4u);
+            if (lane == WAVE - 1) wsum[wv] = inc;
+            const bool block_bad = __syncthreads_or(bad);
+            u32 before = 0, tot = 0;
+            for (u32 w = 0; w < (u32) WIDE_W; w++) {
+                if (w < wv) before += wsum[w];
+                tot += wsum[w];
+            }
+            if (block_bad || tot != cnt[g]) {
+                if (tid == 0) atomicOr(err, block_bad ? 1u : 4u);
                 return;
             }
-            u32 pos = offset[g] + inc - c;
+            u32 pos = offset[g] + before + inc - c;
             walk([&](u32 nd) { ids[pos++] = nd; });
         }
 
@@ -1029,9 +1051,9 @@ namespace shamx {
             k_over_list<<<div_up(L, 256), 256, 0, s>>>(L, nb.oflag.data(), nb.ofpos.data(),
                                                         nb.opos.data(), (u32) base, nullptr,
                                                         nb.olist.data(), nb.leaf_offset.data());
-            // one leaf per wave, its walk spread over the lanes (the listed leaves are far
+            // one leaf per block, its walk spread over the threads (the listed leaves are far
             // apart, and a wave walks the union of its lanes' walks)
-            k_leaf_walk_wide<<<n_over, WAVE, 0, s>>>(L, I, t.nodes.data(), Rkern,
+            k_leaf_walk_wide<<<n_over, WIDE_T, 0, s>>>(L, I, t.nodes.data(), Rkern,
                                                       nb.olist.data(), n_over,
                                                       nb.leaf_cnt.data(), nb.leaf_offset.data(),
                                                       nb.leaf_ids.data(), err);
