@@ -190,8 +190,8 @@ namespace shamx {
         // cnt[g] (the count of the fixed-capacity pass) before anything is written at
         // offset[g] (err bit 4 otherwise).  The listed leaves are the ones with the longest
         // lists, and the kernel lasts as long as its longest walk: a block per leaf spreads
-        // that walk over four times the lanes of one wave.
-        constexpr int WIDE_W = 4, WIDE_T = WIDE_W * WAVE, WIDE_FR = 4 * WIDE_T, WIDE_ST = 40;
+        // that walk over eight times the lanes of one wave.
+        constexpr int WIDE_W = 8, WIDE_T = WIDE_W * WAVE, WIDE_FR = 4 * WIDE_T, WIDE_ST = 40;
         __global__ void __launch_bounds__(WIDE_T)
             k_leaf_walk_wide(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
                              const u32 *__restrict list, u32 n_list, const u32 *__restrict cnt,
