@@ -164,7 +164,11 @@ namespace shamx {
             u32 c        = c0;
             bool ok      = wave_traverse(
                 nodes, I, I + L, live,
-                [&](const f64 *rec) {
+                [&](const f64 *rec) -> bool {
+                    // STRIDE: a lane whose list is past its capacity stops entering cells (its
+                    // count, then above cap, only flags the overflow; the true count comes
+                    // from the overflow walk), so its wave is not held for the whole list
+                    if (STRIDE && c - c0 > cap) return false;
                     f64 r      = rec[6] * Rkern;
                     f64 nlo[3] = {rec[0], rec[1], rec[2]}, nhi[3] = {rec[3], rec[4], rec[5]};
                     f64 xlo[3] = {nlo[0] - r, nlo[1] - r, nlo[2] - r};
@@ -191,10 +195,13 @@ namespace shamx {
         // offset[g] (err bit 4 otherwise).  The listed leaves are the ones with the longest
         // lists, and the kernel lasts as long as its longest walk: a block per leaf spreads
         // that walk over eight times the lanes of one wave.
+        // COUNT: the pass that writes each listed leaf's list length to cnt[g] (the
+        // fixed-capacity pass stops counting past the capacity), nothing else.
         constexpr int WIDE_W = 8, WIDE_T = WIDE_W * WAVE, WIDE_FR = 4 * WIDE_T, WIDE_ST = 40;
+        template<bool COUNT>
         __global__ void __launch_bounds__(WIDE_T)
             k_leaf_walk_wide(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
-                             const u32 *__restrict list, u32 n_list, const u32 *__restrict cnt,
+                             const u32 *__restrict list, u32 n_list, u32 *__restrict cnt,
                              const u32 *__restrict offset, u32 *__restrict ids,
                              u32 *__restrict err) {
             __shared__ u32 fr[2][WIDE_FR];
@@ -327,6 +334,13 @@ namespace shamx {
             for (u32 w = 0; w < (u32) WIDE_W; w++) {
                 if (w < wv) before += wsum[w];
                 tot += wsum[w];
+            }
+            if (COUNT) {
+                if (tid == 0) {
+                    if (block_bad) atomicOr(err, 1u);
+                    else cnt[g] = tot;
+                }
+                return;
             }
             if (block_bad || tot != cnt[g]) {
                 if (tid == 0) atomicOr(err, block_bad ? 1u : 4u);
@@ -1042,21 +1056,44 @@ namespace shamx {
             exclusive_scan_u32(s, nb.ocnt.data(), nb.opos.data(), n, nb.counters.data() + c + 1,
                                nb.scratch.data());
         };
-        // the listed leaves walked again into the overflow region of leaf_ids
-        auto resolve_leaves = [&](u32 cl, u32 n_over, u32 tot) {
+        // the listed leaves (n_over, from the fixed-capacity pass's counts: above the capacity
+        // for exactly these) walked again into the overflow region of leaf_ids: their true
+        // list lengths first (one leaf per block, its walk spread over the threads: the listed
+        // leaves are far apart, and a wave walks the union of its lanes' walks), then the
+        // maximum, the total and the overflow offsets over the true lengths (counters 8, 10,
+        // 12, 13, read back into hc: one more host read), then the lists
+        auto resolve_leaves = [&](u32 cl, u32 n_over) {
             const u64 base = (u64) L * cl;
+            k_over_list<<<div_up(L, 256), 256, 0, s>>>(L, nb.oflag.data(), nb.ofpos.data(),
+                                                        nb.opos.data(), (u32) base, nullptr,
+                                                        nb.olist.data(), nb.leaf_offset.data());
+            k_leaf_walk_wide<true><<<n_over, WIDE_T, 0, s>>>(L, I, t.nodes.data(), Rkern,
+                                                            nb.olist.data(), n_over,
+                                                            nb.leaf_cnt.data(), nullptr,
+                                                            nullptr, err);
+            fill_u32(s, nb.counters.data() + 8, 0, 1);
+            max_u32(s, L, nb.leaf_cnt.data(), nb.counters.data() + 8);
+            exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.psum.data(), L, nb.counters.data() + 10,
+                               nb.scratch.data());
+            mark(L, nb.leaf_cnt.data(), cl, 12);
+            SHAMX_HIP(hipMemcpyAsync(hc, nb.counters.data(), 16 * sizeof(u32),
+                                     hipMemcpyDeviceToHost, s));
+            SHAMX_HIP(hipStreamSynchronize(s));
+            check_walk_flags(hc[4]);
+            const u32 tot = hc[13];
+            if (hc[12] != n_over)
+                throw std::runtime_error("neighbour cache: overflowing leaf lists changed");
             if (base + tot > 0xffffffffull)
                 throw std::runtime_error("neighbour cache: leaf lists exceed u32 entry indices");
             nb.leaf_ids.resize_keep(std::max<u64>(base + tot, 1), s);
             k_over_list<<<div_up(L, 256), 256, 0, s>>>(L, nb.oflag.data(), nb.ofpos.data(),
                                                         nb.opos.data(), (u32) base, nullptr,
                                                         nb.olist.data(), nb.leaf_offset.data());
-            // one leaf per block, its walk spread over the threads (the listed leaves are far
-            // apart, and a wave walks the union of its lanes' walks)
-            k_leaf_walk_wide<<<n_over, WIDE_T, 0, s>>>(L, I, t.nodes.data(), Rkern,
-                                                      nb.olist.data(), n_over,
-                                                      nb.leaf_cnt.data(), nb.leaf_offset.data(),
-                                                      nb.leaf_ids.data(), err);
+            k_leaf_walk_wide<false><<<n_over, WIDE_T, 0, s>>>(L, I, t.nodes.data(), Rkern,
+                                                             nb.olist.data(), n_over,
+                                                             nb.leaf_cnt.data(),
+                                                             nb.leaf_offset.data(),
+                                                             nb.leaf_ids.data(), err);
         };
         for (int attempt = 0;; attempt++) {
             if (attempt == 4) throw std::runtime_error("neighbour cache: capacities do not settle");
@@ -1102,7 +1139,7 @@ namespace shamx {
                 SHAMX_HIP(hipStreamSynchronize(s));
                 tree_check(t);
                 if (hc[4]) throw std::runtime_error("neighbour cache: tree traversal stack overflow");
-                if (hc[12]) resolve_leaves(cl, hc[12], hc[13]);
+                if (hc[12]) resolve_leaves(cl, hc[12]);
             }
             const u32 leaf_cap = resolve_first ? 0u : cl; // lists past cl are truncated
             if ((u64) nm * cp > 0xffffffffull || packed_rows_forced()) {
@@ -1114,7 +1151,7 @@ namespace shamx {
                     tree_check(t);
                     if (hc[4])
                         throw std::runtime_error("neighbour cache: tree traversal stack overflow");
-                    if (hc[12]) resolve_leaves(cl, hc[12], hc[13]);
+                    if (hc[12]) resolve_leaves(cl, hc[12]);
                 }
                 const u32 n_leaf_pairs = hc[10];
                 nb.leaf_overflowed     = hc[12] > 0;
