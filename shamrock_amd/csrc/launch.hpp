@@ -124,6 +124,7 @@ namespace shamx {
         u32 n_mismatch = 0;
         DevArray<u32> counters;
         PinnedWords<u32, 16> host; // host copies of counters
+        DevArray<u32> wide_cnt;    // per-thread list counts of the overflow walk
         DevArray<unsigned char> scratch;
     };
     // NeighbourCache::start_neighbors_cache_2stages (NeighbourCache.cpp:222-604).

@@ -196,14 +196,16 @@ namespace shamx {
         // lists, and the kernel lasts as long as its longest walk: a block per leaf spreads
         // that walk over eight times the lanes of one wave.
         // COUNT: the pass that writes each listed leaf's list length to cnt[g] (the
-        // fixed-capacity pass stops counting past the capacity), nothing else.
+        // fixed-capacity pass stops counting past the capacity) and each thread's count to
+        // tcnt[j * WIDE_T + thread]; the FILL pass rebuilds the same frontier and reads those
+        // counts for the threads' write positions, so each list is walked once per pass.
         constexpr int WIDE_W = 8, WIDE_T = WIDE_W * WAVE, WIDE_FR = 4 * WIDE_T, WIDE_ST = 40;
         template<bool COUNT>
         __global__ void __launch_bounds__(WIDE_T)
             k_leaf_walk_wide(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
                              const u32 *__restrict list, u32 n_list, u32 *__restrict cnt,
-                             const u32 *__restrict offset, u32 *__restrict ids,
-                             u32 *__restrict err) {
+                             u32 *__restrict tcnt, const u32 *__restrict offset,
+                             u32 *__restrict ids, u32 *__restrict err) {
             __shared__ u32 fr[2][WIDE_FR];
             __shared__ u32 stk[WIDE_ST][WIDE_T]; // [depth][thread]
             __shared__ u32 wsum[WIDE_W];
@@ -321,7 +323,12 @@ namespace shamx {
                 }
             };
             u32 c = 0;
-            walk([&](u32) { c++; });
+            if constexpr (COUNT) {
+                walk([&](u32) { c++; });
+                tcnt[(u64) j * WIDE_T + tid] = c;
+            } else {
+                c = tcnt[(u64) j * WIDE_T + tid];
+            }
             // block-wide exclusive scan of the counts
             u32 inc = c;
             for (int o = 1; o < WAVE; o <<= 1) {
@@ -1067,9 +1074,11 @@ namespace shamx {
             k_over_list<<<div_up(L, 256), 256, 0, s>>>(L, nb.oflag.data(), nb.ofpos.data(),
                                                         nb.opos.data(), (u32) base, nullptr,
                                                         nb.olist.data(), nb.leaf_offset.data());
+            nb.wide_cnt.resize((size_t) n_over * WIDE_T);
             k_leaf_walk_wide<true><<<n_over, WIDE_T, 0, s>>>(L, I, t.nodes.data(), Rkern,
                                                             nb.olist.data(), n_over,
-                                                            nb.leaf_cnt.data(), nullptr,
+                                                            nb.leaf_cnt.data(),
+                                                            nb.wide_cnt.data(), nullptr,
                                                             nullptr, err);
             fill_u32(s, nb.counters.data() + 8, 0, 1);
             max_u32(s, L, nb.leaf_cnt.data(), nb.counters.data() + 8);
@@ -1092,6 +1101,7 @@ namespace shamx {
             k_leaf_walk_wide<false><<<n_over, WIDE_T, 0, s>>>(L, I, t.nodes.data(), Rkern,
                                                              nb.olist.data(), n_over,
                                                              nb.leaf_cnt.data(),
+                                                             nb.wide_cnt.data(),
                                                              nb.leaf_offset.data(),
                                                              nb.leaf_ids.data(), err);
         };
