@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace shamx {
 
@@ -184,23 +185,23 @@ namespace shamx {
             if (STRIDE && valid) offset[g] = c0;
         }
 
-        // One listed leaf per block of WIDE_W waves, its walk spread over the block's threads:
-        // the first wave expands the tree breadth first into an ordered frontier of at least
-        // one subtree per thread (or of all its accepted leaves), then each thread walks a
-        // contiguous run of frontier entries depth first, left child first, with its own
-        // stack.  The frontier stays in left-to-right order and every thread's walk is
-        // left-first, so the threads' outputs in thread order are the leaf's list in the order
-        // of the reference's walk.  Each thread counts first; the block's total must equal
-        // cnt[g] (the count of the fixed-capacity pass) before anything is written at
-        // offset[g] (err bit 4 otherwise).  The listed leaves are the ones with the longest
-        // lists, and the kernel lasts as long as its longest walk: a block per leaf spreads
-        // that walk over eight times the lanes of one wave.
-        // COUNT: the pass that writes each listed leaf's list length to cnt[g] (the
-        // fixed-capacity pass stops counting past the capacity) and each thread's count to
-        // tcnt[j * WIDE_T + thread]; the FILL pass rebuilds the same frontier and reads those
-        // counts for the threads' write positions, so each list is walked once per pass.
-        constexpr int WIDE_W = 8, WIDE_T = WIDE_W * WAVE, WIDE_FR = 4 * WIDE_T, WIDE_ST = 40;
-        template<bool COUNT>
+        // The lists longer than their fixed capacity, one listed leaf per block of WIDE_W
+        // waves, its walk spread over the block's threads: the first wave expands the tree
+        // breadth first into an ordered frontier of at least one subtree per thread (or of all
+        // its accepted leaves), then each thread walks a contiguous run of frontier entries
+        // depth first, left child first, with its own stack.  The frontier stays in
+        // left-to-right order and every thread's walk is left-first, so the threads' outputs
+        // in thread order are the leaf's list in the order of the reference's walk.
+        // COUNT pass: each thread's count to tcnt[j * WIDE_T + thread] and the block's total to
+        // cnt[g] (the fixed-capacity pass stops counting past the capacity).  FILL pass: the
+        // same frontier again, the threads' write positions from those counts, each list
+        // walked once more and written at offset[g]; the block's total must equal cnt[g]
+        // (err bit 4 otherwise).  The kernel lasts as long as its longest walk, so a block per
+        // leaf spreads that walk over several waves: 8 for the few long lists of a settled
+        // build, 1 when many leaves pass a capacity set too low (the first build), whose lists
+        // are only a little longer.
+        constexpr int WIDE_ST = 40;
+        template<int WIDE_W, bool COUNT, int WIDE_T = WIDE_W * WAVE, int WIDE_FR = 4 * WIDE_T>
         __global__ void __launch_bounds__(WIDE_T)
             k_leaf_walk_wide(u32 L, u32 I, const f64 *__restrict nodes, f64 Rkern,
                              const u32 *__restrict list, u32 n_list, u32 *__restrict cnt,
@@ -1063,6 +1064,8 @@ namespace shamx {
             exclusive_scan_u32(s, nb.ocnt.data(), nb.opos.data(), n, nb.counters.data() + c + 1,
                                nb.scratch.data());
         };
+        // blocks of 8 waves per listed leaf while there are few of them, 1 wave beyond
+        constexpr u32 WIDE_FEW = 8192;
         // the listed leaves (n_over, from the fixed-capacity pass's counts: above the capacity
         // for exactly these) walked again into the overflow region of leaf_ids: their true
         // list lengths first (one leaf per block, its walk spread over the threads: the listed
@@ -1074,12 +1077,20 @@ namespace shamx {
             k_over_list<<<div_up(L, 256), 256, 0, s>>>(L, nb.oflag.data(), nb.ofpos.data(),
                                                         nb.opos.data(), (u32) base, nullptr,
                                                         nb.olist.data(), nb.leaf_offset.data());
-            nb.wide_cnt.resize((size_t) n_over * WIDE_T);
-            k_leaf_walk_wide<true><<<n_over, WIDE_T, 0, s>>>(L, I, t.nodes.data(), Rkern,
-                                                            nb.olist.data(), n_over,
-                                                            nb.leaf_cnt.data(),
-                                                            nb.wide_cnt.data(), nullptr,
-                                                            nullptr, err);
+            const bool few = n_over <= WIDE_FEW;
+            nb.wide_cnt.resize((size_t) n_over * (few ? 8 : 1) * WAVE);
+            auto walk_wide = [&](auto count, u32 *offset, u32 *ids_) {
+                constexpr bool C = decltype(count)::value;
+                if (few)
+                    k_leaf_walk_wide<8, C><<<n_over, 8 * WAVE, 0, s>>>(
+                        L, I, t.nodes.data(), Rkern, nb.olist.data(), n_over, nb.leaf_cnt.data(),
+                        nb.wide_cnt.data(), offset, ids_, err);
+                else
+                    k_leaf_walk_wide<1, C><<<n_over, WAVE, 0, s>>>(
+                        L, I, t.nodes.data(), Rkern, nb.olist.data(), n_over, nb.leaf_cnt.data(),
+                        nb.wide_cnt.data(), offset, ids_, err);
+            };
+            walk_wide(std::true_type{}, nullptr, nullptr);
             fill_u32(s, nb.counters.data() + 8, 0, 1);
             max_u32(s, L, nb.leaf_cnt.data(), nb.counters.data() + 8);
             exclusive_scan_u32(s, nb.leaf_cnt.data(), nb.psum.data(), L, nb.counters.data() + 10,
@@ -1098,12 +1109,7 @@ namespace shamx {
             k_over_list<<<div_up(L, 256), 256, 0, s>>>(L, nb.oflag.data(), nb.ofpos.data(),
                                                         nb.opos.data(), (u32) base, nullptr,
                                                         nb.olist.data(), nb.leaf_offset.data());
-            k_leaf_walk_wide<false><<<n_over, WIDE_T, 0, s>>>(L, I, t.nodes.data(), Rkern,
-                                                             nb.olist.data(), n_over,
-                                                             nb.leaf_cnt.data(),
-                                                             nb.wide_cnt.data(),
-                                                             nb.leaf_offset.data(),
-                                                             nb.leaf_ids.data(), err);
+            walk_wide(std::false_type{}, nb.leaf_offset.data(), nb.leaf_ids.data());
         };
         for (int attempt = 0;; attempt++) {
             if (attempt == 4) throw std::runtime_error("neighbour cache: capacities do not settle");
