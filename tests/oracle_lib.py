@@ -14,6 +14,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "_build", "libsph_oracle.so")
+# a prebuilt oracle to load instead (the sanitizer build of test_oracle_sanitized.py)
+LIB_OVERRIDE = os.environ.get("SHAMX_ORACLE_LIB")
 
 KERNELS = {"M4": 0, "M6": 1, "M8": 2, "C2": 3, "C4": 4, "C6": 5}
 
@@ -66,8 +68,11 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        build()
-        L = C.CDLL(LIB_PATH)
+        if LIB_OVERRIDE:
+            L = C.CDLL(LIB_OVERRIDE)
+        else:
+            build()
+            L = C.CDLL(LIB_PATH)
         L.orc_last_error.restype = C.c_char_p
         L.orc_kernel_eval.argtypes = [C.c_int, C.c_uint32, _f64p, _f64p, _f64p, _f64p, _f64p,
                                       _f64p, _f64p, _f64p]
