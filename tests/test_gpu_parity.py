@@ -192,13 +192,24 @@ def _same_modulo_ties(codes_sorted, perm_a, perm_b):
 
 
 @pytest.mark.parametrize("case,split", [("lattice", False), ("clustered", False),
-                                        ("clustered", True)])
+                                        ("clustered", True), ("single", False),
+                                        ("coincident", False), ("pair", True)])
 def test_tree_build_bitexact(ctx, case, split):
+    """Whole-tree parity with the oracle.  The small cases are the one-leaf trees of the
+    reference's one-cell test (KarrasRadixTreeAABBTests.cpp:241-380: every object in a single
+    reduced cell, no internal cell): one particle, seven at one point (a zero-width bounding
+    box), and two close particles merged by the reduction."""
     say(f"[tree_build] case={case} split={split}")
     rng = np.random.default_rng(7)
     if case == "lattice":
         pos = setups.lattice_cubic(1 / 24, (0, 0, 0), (1, 1, 1))
         pos = pos + rng.uniform(-0.1, 0.1, pos.shape) / 24
+    elif case == "single":
+        pos = np.array([[0.3, -0.2, 0.7]])
+    elif case == "coincident":
+        pos = np.tile([[0.3, -0.2, 0.7]], (7, 1))
+    elif case == "pair":
+        pos = np.array([[0.0, 0.0, 0.0], [1e-3, 2e-3, -1e-3]])
     else:  # heavy Morton-cell collisions: a Plummer-like core
         r = rng.pareto(2.0, 60_000)[:, None] * 1e-3
         d = rng.normal(size=(60_000, 3))
@@ -219,6 +230,8 @@ def test_tree_build_bitexact(ctx, case, split):
     np.testing.assert_array_equal(g["aabb_min"], t.aabb_min)
     np.testing.assert_array_equal(g["aabb_max"], t.aabb_max)
     np.testing.assert_array_equal(g["rint"], t.field_max(h) * 1.1)
+    if case in ("single", "coincident", "pair"):
+        assert g["L"] == 1 and g["I"] == 0
 
 
 def test_cfl_closed_form():
@@ -307,12 +320,20 @@ def _neigh_gpu(ctx, pos, h, n_local, htol, Rkern):
     return out, g
 
 
-@pytest.mark.parametrize("case", ["jittered", "clustered", "two_scales"])
+@pytest.mark.parametrize("case", ["jittered", "clustered", "two_scales", "few", "coincident"])
 def test_neighbour_cache_sets_match_oracle(ctx, case):
-    """Per-particle neighbour sets and counts, owner leaves and leaf lists vs the oracle."""
+    """Per-particle neighbour sets and counts, owner leaves and leaf lists vs the oracle.  The
+    last two are tiny sets: nine particles that all see each other, and nine at one point (a
+    one-leaf tree with no internal cell)."""
     say(f"[neigh] case={case}")
     rng = np.random.default_rng(21)
-    if case == "jittered":
+    if case == "few":
+        pos = rng.uniform(0, 1e-2, (9, 3))
+        h = np.full(9, 0.05)
+    elif case == "coincident":
+        pos = np.tile([[0.25, 0.5, -0.75]], (9, 1))
+        h = np.full(9, 0.01)
+    elif case == "jittered":
         pos = setups.lattice_cubic(1 / 20, (0, 0, 0), (1, 1, 1))
         pos = pos + rng.uniform(-0.45, 0.45, pos.shape) / 20
         h = np.full(pos.shape[0], 1.2 / 20)
