@@ -95,7 +95,7 @@ def test_morton_closed_forms(ctx):
     assert c[2] == 0x24924924 and c[3] == 0x12492492 and c[4] == 0x09249249
 
 
-@pytest.mark.parametrize("n,bits,seed", [(1, 30, 0), (17, 30, 1), (4097, 30, 2),
+@pytest.mark.parametrize("n,bits,seed", [(0, 30, 0), (1, 30, 0), (17, 30, 1), (4097, 30, 2),
                                          (1_000_003, 30, 3), (300_000, 8, 4),
                                          (2_500_000, 30, 5)])
 def test_radix_sort_stable(ctx, n, bits, seed):
@@ -176,6 +176,16 @@ def _tree_gpu(ctx, pos, h, level, htol, split=False):
     res["aabb_max"] = res["aabb_max"].reshape(-1, 3)
     res.update(L=L, I=I, x=x, y=y, z=z, h=hh)
     return res
+
+
+def test_tree_build_rejects_empty(ctx):
+    """An empty set is refused on the host, before any launch, as the reference refuses it
+    ("obj_cnt is 0, cannot build a CompressedLeafBVH")."""
+    say("[tree_build] empty")
+    x = dev(np.zeros(1))
+    with pytest.raises(ValueError, match="obj_cnt is 0"):
+        check(lib().shamx_tree_build(ctx, stream(), 0, x.data_ptr(), x.data_ptr(), x.data_ptr(),
+                                     x.data_ptr(), 3, 1.1))
 
 
 def _same_modulo_ties(codes_sorted, perm_a, perm_b):
