@@ -1469,15 +1469,13 @@ namespace shamx {
     } // namespace
 
     namespace {
-        // lanes per particle in the neighbour loops: 1, 2, 4 or 16 (SHAMX_PAIR_LANES; the FP64
-        // sums are taken in list order whatever the choice, so results do not depend on it)
+        // lanes per particle in the neighbour loops: 1, 2, 4 or 16 (SHAMX_PAIR_LANES, read at
+        // every call so a test can compare the settings in one process; the FP64 sums are taken
+        // in list order whatever the choice, so results do not depend on it)
         int pair_lanes() {
-            static const int g = [] {
-                const char *e = std::getenv("SHAMX_PAIR_LANES");
-                int v         = e ? std::atoi(e) : 4;
-                return (v == 1 || v == 2 || v == 4 || v == 16) ? v : 4;
-            }();
-            return g;
+            const char *e = std::getenv("SHAMX_PAIR_LANES");
+            int v         = e ? std::atoi(e) : 4;
+            return (v == 1 || v == 2 || v == 4 || v == 16) ? v : 4;
         }
         template<class F>
         void dispatch_lanes(F &&f) {

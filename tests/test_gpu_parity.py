@@ -699,3 +699,30 @@ def test_record_refresh_equals_full_pack(mode, monkeypatch):
         np.testing.assert_array_equal(ra, rb_)
         for k in fa:
             np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+
+
+@pytest.mark.parametrize("lanes", ["1", "2", "16"])
+def test_pair_lanes_same_bits(lanes, monkeypatch):
+    """SHAMX_PAIR_LANES picks how many lanes share a particle's row in the h iteration, the
+    Omega pass and the force loop; the sums take their terms in list order whatever the
+    choice.  Three Sedov 20^3 steps (corrector reruns included) at the default four lanes and
+    at `lanes` must leave every field the same bits."""
+    cfg, f = setups.sedov(20)
+    runs = {}
+    for g_lanes in ("4", lanes):
+        monkeypatch.setenv("SHAMX_PAIR_LANES", g_lanes)
+        g = GpuSolver(cfg, f)
+        out = []
+        for step in range(3):
+            say(f"[pair lanes] {g_lanes} step {step}")
+            lg = g.evolve_once()
+            out.append((lg["corrector_iterations"], lg["h_iterations"],
+                        {k: g.get(k) for k in ("x", "vx", "vy", "vz", "ax", "ay", "az", "u",
+                                               "du", "h")}))
+        g.close()
+        runs[g_lanes] = out
+    monkeypatch.delenv("SHAMX_PAIR_LANES", raising=False)
+    for (ca, ha, fa), (cb, hb, fb) in zip(runs["4"], runs[lanes]):
+        assert (ca, ha) == (cb, hb)
+        for k in fa:
+            np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
